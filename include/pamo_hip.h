@@ -1,0 +1,110 @@
+/* C ABI of the MI355X (gfx950) stage-2 simplifier, libpamo_hip.so.
+ *
+ * Every pointer named d_* is a device pointer on the context's GPU; every
+ * stream argument is a hipStream_t passed as void* (NULL = default stream).
+ * Functions return 0 on success and a negative code on error; the message is
+ * available from pamo_last_error().  No function aborts the process.
+ *
+ * Ownership: inputs are borrowed and copied into context buffers, so the
+ * caller's buffers are never modified (as simp_cuda/src/cusimp_free.cu:926-946
+ * copies them).  Output views point into context memory and stay valid until
+ * the next call on the same context.  A context is not thread-safe; two
+ * contexts may be used from two threads.
+ */
+#ifndef PAMO_HIP_H
+#define PAMO_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pamo_ctx pamo_ctx;
+
+/* shuffle_mode for the init round: 1 = thrust's Feistel shuffle with
+ * minstd_rand seed 1 (what simp_cuda/src/cusimp_free.cu:937-941 calls),
+ * 0 = identity. */
+#define PAMO_SHUFFLE_IDENTITY 0
+#define PAMO_SHUFFLE_FEISTEL 1
+
+/* Replaces the CUDSP_Free constructor (simp_cuda/src/pybind.cpp:16-20,191-193):
+ * creates the context that owns all device state of one simplifier. */
+int pamo_ctx_create(pamo_ctx** out, int device, int shuffle_mode);
+
+/* Replaces ~CUDSP_Free (simp_cuda/src/pybind.cpp:21-49): syncs and frees. */
+void pamo_ctx_destroy(pamo_ctx* ctx);
+
+/* Last error message of the context (empty string when none). */
+const char* pamo_last_error(const pamo_ctx* ctx);
+
+/* Replaces CUDSP_Free.forward / CUSimp_Free::forward
+ * (simp_cuda/src/pybind.cpp:52-72, simp_cuda/src/cusimp_free.cu:891-1048):
+ * one guarded simplification round.  d_points [n_pts,3] f32, d_tris
+ * [n_tris,3] i32 (rows of -1 are removed faces), d_undo [n_undo] i32 (the
+ * vertex undo list returned by the previous round). */
+int pamo_free_forward(pamo_ctx* ctx, void* stream, const float* d_points, const int* d_tris, const int* d_undo,
+                      int n_undo, int n_pts, int n_tris, float scale, float threshold, int is_stuck, int init);
+
+/* The five outputs of the last forward (simp_cuda/src/pybind.cpp:74-102):
+ * points [n_pts,3], tris [n_tris,3], occupancy [n_pts], vertex map [n_pts],
+ * vertex undo list [n_undo].  Device views into the context. */
+int pamo_free_outputs(pamo_ctx* ctx, float** d_points, int** d_tris, int** d_occ, int** d_map, int** d_undo,
+                      int* n_pts, int* n_tris, int* n_undo);
+
+/* Copies the five outputs of the last forward into caller device buffers
+ * (the clones of simp_cuda/src/pybind.cpp:74-100); any pointer may be NULL. */
+int pamo_free_copy_outputs(pamo_ctx* ctx, void* stream, float* d_points, int* d_tris, int* d_occ, int* d_map,
+                           int* d_undo);
+
+/* The stage-2 loop of PaMO.run (simp_cuda/pamo/__init__.py:121-158) kept on
+ * the device: rounds of forward + compaction until the face count reaches
+ * target (or 10), or stays unchanged `tolerance` times.  d_verts are the
+ * centred vertices, scale the largest axis extent.  On return out_V and out_F
+ * hold the final sizes; fetch the mesh with pamo_run_fetch.
+ * stats (may be NULL): [0] rounds [1] exit reason (1 target, 2 <=10 faces,
+ * 3 tolerance, 4 iteration cap) [2] collapses [3] undo passes (max)
+ * [4] self-intersection checks [5] query faces [6] candidate pairs
+ * [7] faces found intersecting. */
+int pamo_run_stage2(pamo_ctx* ctx, void* stream, const float* d_verts, const int* d_faces, int n_verts, int n_faces,
+                    float scale, long long target, float threshold, int tolerance, long long iters, int* out_V,
+                    int* out_F, long long* stats);
+
+/* Face count after each round of the last pamo_run_stage2 (host array of
+ * cap entries); returns the number of rounds. */
+int pamo_run_rounds(pamo_ctx* ctx, int* host_faces, int cap);
+
+/* Copies the mesh of the last pamo_run_stage2 into caller device buffers. */
+int pamo_run_fetch(pamo_ctx* ctx, void* stream, float* d_verts_out, int* d_faces_out);
+
+/* Kernel timing (no counterpart in the reference, whose only timing is
+ * wall-clock prints, simp_cuda/pamo/__init__.py:121,160-163): HIP events
+ * around the selected kernels on the call's stream.  pamo_profile resets the
+ * totals and selects kernels by mask (1 << PAMO_K_*); pamo_profile_read
+ * returns launches, total milliseconds and total algorithmic bytes. */
+#define PAMO_K_QUADRIC 0
+#define PAMO_K_EDGE_COST 1
+#define PAMO_K_CLAIM 2
+#define PAMO_K_LBVH 3
+#define PAMO_K_TRAVERSE 4
+int pamo_profile(pamo_ctx* ctx, unsigned kernel_mask);
+int pamo_profile_read(pamo_ctx* ctx, int kernel, double out[3]);
+
+/* Parity hooks: copies an internal buffer of the last forward to host memory
+ * (returns the byte size; copies only when cap_bytes is large enough).
+ * which: 5 fan offsets, 6 fan, 7 edges, 8 quadrics (16 f32 per vertex),
+ * 9 edge costs, 10 per-face claim keys, 11 collapsed-edge list,
+ * 12 first check's intersecting-face flags, 13 round-start faces,
+ * 14 invalid-vertex table. */
+long long pamo_debug_copy(pamo_ctx* ctx, void* stream, int which, void* host_dst, long long cap_bytes);
+
+/* Parity hook: every ordered candidate pair (query, current) of one
+ * self-intersection check over all alive faces of a mesh, with the
+ * reference narrow-phase verdict (simp_cuda/src/bvh/self_intersect.cuh:151-392).
+ * host_pairs receives rows (query, current, hit); returns the pair count
+ * (writes only when cap_rows is large enough), or a negative error. */
+long long pamo_debug_pairs(pamo_ctx* ctx, void* stream, const float* d_points, const int* d_tris, int n_pts,
+                           int n_tris, int* host_pairs, long long cap_rows);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

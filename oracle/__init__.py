@@ -1,0 +1,280 @@
+"""ctypes bindings of the CPU oracle and of the exact mesh verifier.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
+and by bench.py's ``cpu_baseline`` leg, always as the checker.  The product
+package ``pamo_amd`` never imports this module.
+
+The oracle restates the reference's stage-2 round
+(simp_cuda/src/cusimp_free.cu:891-1048) and the ``PaMO.run`` loop
+(simp_cuda/pamo/__init__.py:95-178); see pamo_oracle.cpp for the arithmetic
+spec and the parity-mode definitions D0-D9.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_ORC = os.path.join(_HERE, "libpamo_oracle.so")
+_VF = os.path.join(_HERE, "libpamo_verify.so")
+
+SHUFFLE_IDENTITY = 0
+SHUFFLE_FEISTEL = 1
+
+
+def build():
+    """Compile the oracle and verifier (make in this directory)."""
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def _load(path):
+    if not os.path.exists(path):
+        build()
+    return ctypes.CDLL(path)
+
+
+_orc = None
+_vf = None
+
+
+def _lib():
+    global _orc
+    if _orc is None:
+        _orc = _load(_ORC)
+        P = ctypes.c_void_p
+        _orc.orc_create.restype = P
+        _orc.orc_destroy.argtypes = [P]
+        _orc.orc_forward.argtypes = [P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                     ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        _orc.orc_sizes.argtypes = [P, P]
+        _orc.orc_copy.argtypes = [P, ctypes.c_int, P, ctypes.c_longlong]
+        _orc.orc_copy.restype = ctypes.c_longlong
+        _orc.orc_run.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_longlong, ctypes.c_float,
+                                 ctypes.c_int, ctypes.c_longlong, ctypes.c_int, P, P, P, P, ctypes.c_int]
+        _orc.orc_shuffle_perm.argtypes = [ctypes.c_longlong, P]
+        _orc.orc_narrow.argtypes = [P, P, P, P]
+        _orc.orc_isect_raw.argtypes = [P, P, P, P]
+        _orc.orc_is_coplanar.argtypes = [P, P]
+        _orc.orc_full_check.argtypes = [P, P, ctypes.c_int, ctypes.c_int, P, ctypes.c_longlong, P, ctypes.c_longlong, P]
+        _orc.orc_set_threads.argtypes = [ctypes.c_int]
+        _orc.orc_run_timed.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_longlong,
+                                       ctypes.c_float, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, ctypes.c_double,
+                                       P, P, P, ctypes.c_int]
+    return _orc
+
+
+def _vlib():
+    global _vf
+    if _vf is None:
+        _vf = _load(_VF)
+        P = ctypes.c_void_p
+        _vf.vf_check.argtypes = [P, P, ctypes.c_int, ctypes.c_int, P, P, ctypes.c_longlong]
+        _vf.vf_orient3d.argtypes = [P, P, P, P]
+        _vf.vf_orient3d_exact.argtypes = [P, P, P, P]
+        _vf.vf_tri_pair.argtypes = [P, P, P, P]
+    return _vf
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def set_threads(n):
+    """Set the oracle's OpenMP thread count; returns the count in effect."""
+    return _lib().orc_set_threads(int(n))
+
+
+BUF = dict(points=0, tris=1, occ=2, pts_map=3, undo=4, fan_off=5, fan=6, edges=7, Q=8, edge_cost=9,
+           tri_min=10, collapsed=11, S=12, orig_tris=13, table=14)
+_DT = dict(points=np.float32, tris=np.int32, occ=np.int32, pts_map=np.int32, undo=np.int32, fan_off=np.int32,
+           fan=np.int32, edges=np.int32, Q=np.float32, edge_cost=np.uint32, tri_min=np.uint64, collapsed=np.int32,
+           S=np.uint8, orig_tris=np.int32, table=np.uint8)
+SIZE_KEYS = ["V", "F", "E", "n_vertices_undo", "n_collapsed", "n_winners", "undo_passes", "n_checks", "n_fan",
+             "undo_cut", "collapse_t", "alloc_pts", "n_queries", "n_candidates", "n_hits"]
+
+
+class Oracle:
+    """Stateful CPU restatement of ``CUSimp_Free`` (one object per context)."""
+
+    def __init__(self, shuffle=SHUFFLE_FEISTEL):
+        self._lib = _lib()
+        self._ctx = self._lib.orc_create()
+        self.shuffle = shuffle
+
+    def __del__(self):
+        try:
+            self._lib.orc_destroy(self._ctx)
+        except Exception:
+            pass
+
+    def forward(self, points, tris, undo, scale, threshold, is_stuck, init):
+        points = np.ascontiguousarray(points, np.float32)
+        tris = np.ascontiguousarray(tris, np.int32)
+        undo = np.ascontiguousarray(undo, np.int32)
+        rc = self._lib.orc_forward(self._ctx, _ptr(points), _ptr(tris), _ptr(undo) if undo.size else None,
+                                   int(undo.size), int(points.shape[0]), int(tris.shape[0]), float(scale),
+                                   float(threshold), int(bool(is_stuck)), int(bool(init)), int(self.shuffle))
+        if rc != 0:
+            raise RuntimeError("oracle forward failed with code %d" % rc)
+        return self.outputs()
+
+    def sizes(self):
+        out = np.zeros(16, np.int64)
+        self._lib.orc_sizes(self._ctx, _ptr(out))
+        return dict(zip(SIZE_KEYS, [int(x) for x in out[:len(SIZE_KEYS)]]))
+
+    def get(self, name):
+        which = BUF[name]
+        n = self._lib.orc_copy(self._ctx, which, None, 0)
+        dt = np.dtype(_DT[name])
+        a = np.empty(n // dt.itemsize, dt)
+        self._lib.orc_copy(self._ctx, which, _ptr(a), n)
+        if name in ("points", "tris", "orig_tris"):
+            a = a.reshape(-1, 3)
+        elif name == "edges":
+            a = a.reshape(-1, 2)
+        elif name == "Q":
+            a = a.reshape(-1, 16)
+        return a
+
+    def outputs(self):
+        """The five outputs of ``CUDSP_Free.forward`` (pybind.cpp:74-102)."""
+        return (self.get("points"), self.get("tris"), self.get("occ").reshape(-1, 1),
+                self.get("pts_map").reshape(-1, 1), self.get("undo"))
+
+
+def run(verts, faces, scale, target, threshold=1e-3, tolerance=4, iters=1000000, shuffle=SHUFFLE_FEISTEL,
+        return_rounds=False):
+    """The stage-2 round loop of ``PaMO.run`` on centred vertices (C++)."""
+    lib = _lib()
+    verts = np.ascontiguousarray(verts, np.float32)
+    faces = np.ascontiguousarray(faces, np.int32)
+    V, F = verts.shape[0], faces.shape[0]
+    ov = np.empty((V, 3), np.float32)
+    of = np.empty((F, 3), np.int32)
+    st = np.zeros(8, np.int64)
+    cap = 100000
+    rf = np.zeros(cap, np.int32)
+    rc = lib.orc_run(_ptr(verts), _ptr(faces), V, F, float(scale), int(target), float(threshold), int(tolerance),
+                     int(iters), int(shuffle), _ptr(ov), _ptr(of), _ptr(st), _ptr(rf), cap)
+    if rc != 0:
+        raise RuntimeError("oracle run failed with code %d" % rc)
+    stats = dict(rounds=int(st[0]), V=int(st[1]), F=int(st[2]), exit_reason=int(st[3]),
+                 total_collapsed=int(st[4]), max_undo_passes=int(st[6]))
+    out = (ov[:stats["V"]].copy(), of[:stats["F"]].copy(), stats)
+    if return_rounds:
+        return out + (rf[:stats["rounds"]].copy(),)
+    return out
+
+
+def run_timed(verts, faces, scale, target, max_seconds, threshold=1e-3, tolerance=4, shuffle=SHUFFLE_FEISTEL):
+    """The same loop stopped after the first round that ends past
+    max_seconds.  Returns (stats, faces after each round, seconds elapsed at
+    the end of each round)."""
+    lib = _lib()
+    verts = np.ascontiguousarray(verts, np.float32)
+    faces = np.ascontiguousarray(faces, np.int32)
+    st = np.zeros(8, np.int64)
+    cap = 100000
+    rf = np.zeros(cap, np.int32)
+    rs = np.zeros(cap, np.float64)
+    rc = lib.orc_run_timed(_ptr(verts), _ptr(faces), verts.shape[0], faces.shape[0], float(scale), int(target),
+                           float(threshold), int(tolerance), 1000000, int(shuffle), float(max_seconds), _ptr(st),
+                           _ptr(rf), _ptr(rs), cap)
+    if rc != 0:
+        raise RuntimeError("oracle run failed with code %d" % rc)
+    n = int(st[0])
+    return dict(rounds=n, exit_reason=int(st[3])), rf[:n].copy(), rs[:n].copy()
+
+
+def shuffle_perm(m):
+    """Output slot j of the init shuffle gathers input face perm[j]."""
+    out = np.empty(max(m, 0), np.int64)
+    _lib().orc_shuffle_perm(int(m), _ptr(out))
+    return out
+
+
+def narrow(cur, qry, ci, qi):
+    cur = np.ascontiguousarray(cur, np.float32).reshape(9)
+    qry = np.ascontiguousarray(qry, np.float32).reshape(9)
+    ci = np.ascontiguousarray(ci, np.int32).reshape(3)
+    qi = np.ascontiguousarray(qi, np.int32).reshape(3)
+    return int(_lib().orc_narrow(_ptr(cur), _ptr(qry), _ptr(ci), _ptr(qi)))
+
+
+def isect_raw(a, b):
+    a = np.ascontiguousarray(a, np.float32).reshape(9)
+    b = np.ascontiguousarray(b, np.float32).reshape(9)
+    r1 = np.zeros(3, np.float32)
+    r2 = np.zeros(3, np.float32)
+    hit = _lib().orc_isect_raw(_ptr(a), _ptr(b), _ptr(r1), _ptr(r2))
+    return bool(hit), r1, r2
+
+
+def is_coplanar(a, b):
+    a = np.ascontiguousarray(a, np.float32).reshape(9)
+    b = np.ascontiguousarray(b, np.float32).reshape(9)
+    return bool(_lib().orc_is_coplanar(_ptr(a), _ptr(b)))
+
+
+def full_check(points, tris):
+    """All ordered candidate pairs (q, c) and intersecting pairs of a mesh."""
+    points = np.ascontiguousarray(points, np.float32)
+    tris = np.ascontiguousarray(tris, np.int32)
+    cnt = np.zeros(2, np.int64)
+    _lib().orc_full_check(_ptr(points), _ptr(tris), points.shape[0], tris.shape[0], None, 0, None, 0, _ptr(cnt))
+    cand = np.empty((int(cnt[0]), 2), np.int32)
+    hits = np.empty((int(cnt[1]), 2), np.int32)
+    _lib().orc_full_check(_ptr(points), _ptr(tris), points.shape[0], tris.shape[0], _ptr(cand), cand.size + 1,
+                          _ptr(hits), hits.size + 1, _ptr(cnt))
+    return cand, hits
+
+
+VERIFY_KEYS = ["faces", "vertices", "edges", "boundary_edges", "nonmanifold_edges", "misoriented_edges",
+               "nonmanifold_vertices", "unreferenced_vertices", "degenerate_faces", "intersecting_pairs",
+               "euler", "bad_indices"]
+
+
+def verify(points, tris, max_pairs=64):
+    """Exact check: manifold, watertight, degenerate faces, self-intersections."""
+    points = np.ascontiguousarray(points, np.float32)
+    tris = np.ascontiguousarray(tris, np.int32)
+    st = np.zeros(12, np.int64)
+    pairs = np.zeros((max_pairs, 2), np.int32)
+    _vlib().vf_check(_ptr(points), _ptr(tris), points.shape[0], tris.shape[0], _ptr(st), _ptr(pairs), max_pairs)
+    res = dict(zip(VERIFY_KEYS, [int(x) for x in st]))
+    res["pairs"] = pairs[:min(max_pairs, res["intersecting_pairs"])]
+    res["manifold"] = (res["nonmanifold_edges"] == 0 and res["nonmanifold_vertices"] == 0
+                       and res["misoriented_edges"] == 0 and res["bad_indices"] == 0)
+    res["watertight"] = res["boundary_edges"] == 0 and res["nonmanifold_edges"] == 0
+    res["ok"] = (res["manifold"] and res["watertight"] and res["degenerate_faces"] == 0
+                 and res["intersecting_pairs"] == 0)
+    return res
+
+
+def orient3d(a, b, c, d, exact_only=False):
+    f = _vlib().vf_orient3d_exact if exact_only else _vlib().vf_orient3d
+    arrs = [np.ascontiguousarray(x, np.float64).reshape(3) for x in (a, b, c, d)]
+    return int(f(*[_ptr(x) for x in arrs]))
+
+
+def tri_pair_intersects(a, b, ia, ib):
+    a = np.ascontiguousarray(a, np.float32).reshape(9)
+    b = np.ascontiguousarray(b, np.float32).reshape(9)
+    ia = np.ascontiguousarray(ia, np.int32).reshape(3)
+    ib = np.ascontiguousarray(ib, np.int32).reshape(3)
+    return bool(_vlib().vf_tri_pair(_ptr(a), _ptr(b), _ptr(ia), _ptr(ib)))
+
+
+def preprocess(points, faces):
+    """Centring and scale of ``PaMO.run`` with stage 1 off
+    (simp_cuda/pamo/__init__.py:60-64,101,117,126), in numpy float32."""
+    points = np.asarray(points, np.float32)
+    faces = np.asarray(faces)
+    tris = points[faces]
+    tris_mean = tris.mean(axis=1).mean(axis=0)
+    verts = points - tris_mean
+    ext = verts.max(axis=0) - verts.min(axis=0)
+    scale = max(max(ext[0], ext[1]), ext[2])
+    return verts.astype(np.float32), np.float32(scale), tris_mean

@@ -1,0 +1,1000 @@
+// CPU ORACLE for PaMO stage 2 (robust parallel simplification).
+//
+// TEST INFRASTRUCTURE ONLY.  This file restates, in plain C++17, the
+// behaviour of the reference's stage-2 round (`CUSimp_Free::forward`,
+// simp_cuda/src/cusimp_free.cu:891-1048) and of the `PaMO.run` round loop
+// (simp_cuda/pamo/__init__.py:95-178).  It is the checker that the HIP
+// library is compared against, and the CPU baseline that bench.py times.
+// Nothing in the product path (pamo_amd/) may load or call it.
+//
+// Arithmetic: every float expression below keeps the reference's operation
+// order.  The file must be compiled with -ffp-contract=off and without
+// -ffast-math (the reference is built with nvcc --fmad=false,
+// simp_cuda/setup.py:34); division and sqrt are IEEE correctly rounded.
+//
+// Definitions for behaviour the reference leaves undefined (all pinned the
+// same way in the HIP library):
+//   D0 fan (vertex -> incident faces) order is ascending face index; the
+//      reference fills fans with atomic slots (cusimp_free.cu:291-304).
+//   D1 pts_map, D2 edge_cost, D3 the invalid-vertex table and the undo lists
+//      are zero-filled whenever the reference (re)allocates them
+//      (cusimp_free.cu:47-85,180-194), instead of holding uninitialised memory.
+//   D4 pow(x,2) is x*x (cusimp_free.cu:533,573).
+//   D5 remove_line_edge_collapse runs sequentially in ascending edge index
+//      (cusimp_free.cu:234-274 is racy when two degenerate edges are adjacent).
+//   D6 undo-list slots that the reference counts but never writes (the pass
+//      after the 5th undo iteration, cusimp_free.cu:1036-1039) hold 0.
+//   D7 the init shuffle is the Feistel-bijection shuffle of thrust/rocThrust
+//      (thrust/system/detail/generic/shuffle.inl) with minstd_rand seed 1, or
+//      the identity by option.
+//   D8 the reference caps the intersecting-pair list at F-1 pairs
+//      (self_intersect.cuh:213-216); the cap is not modelled (it is counted).
+//   D9 the collapsed-edge list and the undo list are kept in ascending order
+//      of collapsed-list position; the reference fills them in atomicAdd order.
+//   collapse threshold conversion saturates like the device conversion (the
+//      host conversion at cusimp_free.cu:923 is undefined only for tau >= 10).
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+typedef unsigned long long u64;
+
+struct V3 { float x, y, z; };
+static inline V3 vsub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+static inline V3 vadd(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+static inline float vdot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline V3 vcross(V3 a, V3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+// Vertex::norm, cusimp_free.h:43-46 (sqrt of a float is the float sqrt)
+static inline float vnorm(V3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+static inline V3 vdivs(V3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+
+// cusimp_free.cu:29-34
+static inline float fmin_ref(float a, float b) { return a < b ? a : b; }
+static inline float fmax_ref(float a, float b) { return a > b ? a : b; }
+static inline float clamp_ref(float x, float a, float b) { return fmin_ref(fmax_ref(a, x), b); }
+
+static const float COST_RANGE = 10.0f;          // cusimp_free.cu:15
+static const float U32F = 4294967296.0f;        // (float)UINT32_MAX
+
+// float -> uint32 with the device's saturating conversion.
+static inline uint32_t u32sat(float x) {
+  if (!(x > 0.0f)) return 0u;                   // NaN and negatives -> 0
+  if (x >= U32F) return 0xFFFFFFFFu;
+  return (uint32_t)x;
+}
+
+static inline uint32_t collapse_threshold(float tau) {   // cusimp_free.cu:923
+  return u32sat(clamp_ref(tau, 0.0f, COST_RANGE) / COST_RANGE * U32F);
+}
+
+// ---------------------------------------------------------------- shuffle
+// Feistel bijection of thrust's shuffle (D7), keys from minstd_rand(1).
+struct Feistel {
+  uint64_t lbits, rbits, lmask, rmask;
+  uint32_t key[24];
+  explicit Feistel(uint64_t m) {
+    uint64_t total;
+    if (m <= 16) total = 4;
+    else { total = 0; uint64_t t = m - 1; while (t) { total++; t >>= 1; } }
+    lbits = total / 2; lmask = (1ull << lbits) - 1;
+    rbits = total - lbits; rmask = (1ull << rbits) - 1;
+    uint64_t x = 1;                              // minstd_rand default seed
+    for (int i = 0; i < 24; i++) { x = (x * 48271ull) % 2147483647ull; key[i] = (uint32_t)x; }
+  }
+  uint64_t n() const { return 1ull << (lbits + rbits); }
+  uint64_t operator()(uint64_t val) const {
+    uint32_t s0 = (uint32_t)(val >> rbits), s1 = (uint32_t)(val & rmask);
+    for (int i = 0; i < 24; i++) {
+      uint64_t prod = 0xD2B74407B1CE6E93ull * (uint64_t)s0;
+      uint32_t hi = (uint32_t)(prod >> 32), lo = (uint32_t)prod;
+      lo = (lo << (rbits - lbits)) | (s1 >> lbits);
+      s0 = ((hi ^ key[i]) ^ s1) & (uint32_t)lmask;
+      s1 = lo & (uint32_t)rmask;
+    }
+    return ((uint64_t)s0 << rbits) | (uint64_t)s1;
+  }
+};
+
+// perm[j] = source index gathered into output slot j.
+static void shuffle_permutation(int64_t m, std::vector<int64_t>& perm) {
+  perm.clear();
+  if (m <= 0) return;
+  Feistel f((uint64_t)m);
+  uint64_t n = f.n();
+  perm.reserve(m);
+  for (uint64_t i = 0; i < n; i++) {
+    uint64_t k = f(i);
+    if (k < (uint64_t)m) perm.push_back((int64_t)k);
+  }
+}
+
+// ---------------------------------------------------------- narrow phase
+// Restatement of self_intersect.cuh:229-378 for one ordered pair
+// (current face c, query face q), together with tri_tri_3d.cuh:100-276,
+// math.cuh:43-51 and self_intersect.cuh:25-48,129-149.
+static inline double dotdf(const double a[3], const float b[3]) {
+  return a[0] * (double)b[0] + a[1] * (double)b[1] + a[2] * (double)b[2];
+}
+static inline void crossd(double r[3], const double a[3], const double b[3]) {
+  r[0] = a[1] * b[2] - a[2] * b[1];
+  r[1] = a[2] * b[0] - a[0] * b[2];
+  r[2] = a[0] * b[1] - a[1] * b[0];
+}
+static inline void plane_of(double pl[4], const float t[3][3]) {
+  double ba[3], bc[3];
+  for (int k = 0; k < 3; k++) {
+    ba[k] = (double)t[0][k] - (double)t[1][k];
+    bc[k] = (double)t[2][k] - (double)t[1][k];
+  }
+  crossd(pl, ba, bc);
+  pl[3] = -dotdf(pl, t[1]);
+}
+
+static bool is_coplanar_ref(const float a[3][3], const float b[3][3]) {
+  float scale = 0.0f;
+  for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) scale = fmaxf(scale, fabsf(a[i][j]));
+  for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) scale = fmaxf(scale, fabsf(b[i][j]));
+  double pl[4];
+  plane_of(pl, a);
+  double s0 = dotdf(pl, b[0]) + pl[3];
+  double s1 = dotdf(pl, b[1]) + pl[3];
+  double s2 = dotdf(pl, b[2]) + pl[3];
+  double eps = (double)scale * 1e-6;
+  return fabs(s0) <= eps && fabs(s1) <= eps && fabs(s2) <= eps;
+}
+
+static inline void interp3(float r[3], const float a[3], const float b[3], float t) {
+  const float s = 1.0f - t;
+  r[0] = s * a[0] + t * b[0];
+  r[1] = s * a[1] + t * b[1];
+  r[2] = s * a[2] + t * b[2];
+}
+
+static bool isect_tri_tri_ref(const float ta[3][3], const float tb[3][3], float ri1[3], float ri2[3]) {
+  struct { float mn, mx; float loc[2][3]; } range[2];
+  float side[2][3];
+  double pa[4], pb[4];
+  plane_of(pa, ta);
+  for (int k = 0; k < 3; k++) side[1][k] = (float)(dotdf(pa, tb[k]) + pa[3]);
+  if (!side[1][0] && !side[1][1] && !side[1][2]) return false;
+  if ((side[1][0] && side[1][1] && side[1][2]) && (side[1][0] < 0.0f) == (side[1][1] < 0.0f) &&
+      (side[1][0] < 0.0f) == (side[1][2] < 0.0f))
+    return false;
+  plane_of(pb, tb);
+  for (int k = 0; k < 3; k++) side[0][k] = (float)(dotdf(pb, ta[k]) + pb[3]);
+  if ((side[0][0] && side[0][1] && side[0][2]) && (side[0][0] < 0.0f) == (side[0][1] < 0.0f) &&
+      (side[0][0] < 0.0f) == (side[0][2] < 0.0f))
+    return false;
+  double dir[3];
+  crossd(dir, pa, pb);
+  for (int i = 0; i < 2; i++) {
+    const float(*tri)[3] = i == 0 ? ta : tb;
+    int ti[3];
+    if ((side[i][0] && side[i][1]) && (side[i][0] < 0.0f) == (side[i][1] < 0.0f)) { ti[0] = 1; ti[1] = 2; ti[2] = 0; }
+    else if ((side[i][1] && side[i][2]) && (side[i][1] < 0.0f) == (side[i][2] < 0.0f)) { ti[0] = 2; ti[1] = 0; ti[2] = 1; }
+    else if (!side[i][1] && (side[i][2] < 0.0f) != (side[i][0] < 0.0f)) { ti[0] = 1; ti[1] = 2; ti[2] = 0; }
+    else { ti[0] = 0; ti[1] = 1; ti[2] = 2; }
+    double dot_b = dotdf(dir, tri[ti[1]]);
+    float sidec = side[i][ti[1]];
+    if (sidec) {
+      double dot_a = dotdf(dir, tri[ti[0]]);
+      double dot_c = dotdf(dir, tri[ti[2]]);
+      float fac0 = sidec / (sidec - side[i][ti[0]]);
+      float fac1 = sidec / (sidec - side[i][ti[2]]);
+      double off0 = fac0 * (dot_a - dot_b);
+      double off1 = fac1 * (dot_c - dot_b);
+      if (off0 > off1) {
+        std::swap(off0, off1); std::swap(fac0, fac1); std::swap(ti[0], ti[2]);
+      }
+      range[i].mn = (float)(dot_b + off0);
+      range[i].mx = (float)(dot_b + off1);
+      interp3(range[i].loc[0], tri[ti[1]], tri[ti[0]], fac0);
+      interp3(range[i].loc[1], tri[ti[1]], tri[ti[2]], fac1);
+    } else {
+      range[i].mn = range[i].mx = (float)dot_b;
+      for (int k = 0; k < 3; k++) range[i].loc[0][k] = range[i].loc[1][k] = tri[ti[1]][k];
+    }
+  }
+  if ((range[0].mx > range[1].mn) && (range[0].mn < range[1].mx)) {
+    if (range[0].mn >= range[1].mn) {
+      memcpy(ri1, range[0].loc[0], 12);
+      if (range[0].mx <= range[1].mx) memcpy(ri2, range[0].loc[1], 12);
+      else memcpy(ri2, range[1].loc[1], 12);
+    } else {
+      if (range[0].mx <= range[1].mx) { memcpy(ri1, range[0].loc[1], 12); memcpy(ri2, range[1].loc[0], 12); }
+      else { memcpy(ri1, range[1].loc[0], 12); memcpy(ri2, range[1].loc[1], 12); }
+    }
+    return true;
+  }
+  return false;
+}
+
+// Outcome codes of one ordered-pair test (debug export distinguishes them).
+enum { NP_MISS = 0, NP_HIT = 1, NP_COPLANAR = 2, NP_SHARED_EDGE_IDX = 3, NP_SHARED_EDGE_COORD = 4, NP_SHORT_SHARED_VERTEX = 5 };
+
+// cur/qry: 9 floats each (3 vertices); ci/qi: vertex indices.
+static int narrow_pair(const float cur[9], const float qry[9], const int ci[3], const int qi[3]) {
+  float p[6][3];
+  for (int v = 0; v < 3; v++) for (int k = 0; k < 3; k++) { p[v][k] = cur[3 * v + k]; p[3 + v][k] = qry[3 * v + k]; }
+  int mean[3];
+  for (int k = 0; k < 3; k++) {
+    float m = (p[0][k] + p[1][k] + p[2][k] + p[3][k] + p[4][k] + p[5][k]) / 6.0f;
+    mean[k] = (int)floorf(m);
+  }
+  for (int v = 0; v < 6; v++) for (int k = 0; k < 3; k++) p[v][k] -= (float)mean[k];
+  int nc = 0;
+  for (int j = 0; j < 3; j++) for (int k = 0; k < 3; k++) if (ci[j] == qi[k]) nc++;
+  float ta[3][3], tb[3][3];
+  memcpy(ta, p[0], 36);
+  memcpy(tb, p[3], 36);
+  if (is_coplanar_ref(ta, tb)) return NP_COPLANAR;
+  if (nc == 2) return NP_SHARED_EDGE_IDX;
+  {
+    const float eps = 1e-6f;
+    auto same = [&](const float* a, const float* b) {
+      return std::abs(a[0] - b[0]) < eps && std::abs(a[1] - b[1]) < eps && std::abs(a[2] - b[2]) < eps;
+    };
+    int sh = 0;
+    for (int v = 0; v < 3; v++) sh += same(ta[v], tb[0]) || same(ta[v], tb[1]) || same(ta[v], tb[2]);
+    if (sh >= 2) return NP_SHARED_EDGE_COORD;
+  }
+  float r1[3], r2[3];
+  if (!isect_tri_tri_ref(ta, tb, r1, r2)) return NP_MISS;
+  float dx = std::abs(r2[0] - r1[0]), dy = std::abs(r2[1] - r1[1]), dz = std::abs(r2[2] - r1[2]);
+  float ld = -INFINITY;
+  if (dx > ld) ld = dx;
+  if (dy > ld) ld = dy;
+  if (dz > ld) ld = dz;
+  if (ld < 1e-3f && nc == 1) return NP_SHORT_SHARED_VERTEX;
+  return NP_HIT;
+}
+
+// ------------------------------------------------------------- AABB BVH
+struct Box { float lo[3], hi[3]; };
+// aabb.cuh:20-27 (touching boxes overlap)
+static inline bool overlap(const Box& a, const Box& b) {
+  for (int k = 0; k < 3; k++) if (a.hi[k] < b.lo[k] || b.hi[k] < a.lo[k]) return false;
+  return true;
+}
+
+// Top-down median-split BVH over a set of face boxes; used only to find
+// all overlapping pairs (the set does not depend on the tree shape).
+struct CpuBVH {
+  struct Node { Box b; int left, right, first, count; };
+  std::vector<Node> nodes;
+  std::vector<int> items;
+  const std::vector<Box>* boxes = nullptr;
+
+  void build(const std::vector<Box>& bx, const std::vector<int>& ids) {
+    boxes = &bx; items = ids; nodes.clear();
+    if (items.empty()) return;
+    nodes.reserve(2 * items.size() / 2 + 8);
+    build_rec(0, (int)items.size());
+  }
+  int build_rec(int first, int count) {
+    Node n; n.left = n.right = -1; n.first = first; n.count = count;
+    Box b; for (int k = 0; k < 3; k++) { b.lo[k] = INFINITY; b.hi[k] = -INFINITY; }
+    for (int i = first; i < first + count; i++) {
+      const Box& x = (*boxes)[items[i]];
+      for (int k = 0; k < 3; k++) { b.lo[k] = fminf(b.lo[k], x.lo[k]); b.hi[k] = fmaxf(b.hi[k], x.hi[k]); }
+    }
+    n.b = b;
+    int idx = (int)nodes.size();
+    nodes.push_back(n);
+    if (count <= 4) return idx;
+    int ax = 0; float ext = -1;
+    for (int k = 0; k < 3; k++) if (b.hi[k] - b.lo[k] > ext) { ext = b.hi[k] - b.lo[k]; ax = k; }
+    int mid = first + count / 2;
+    const std::vector<Box>& B = *boxes;
+    std::nth_element(items.begin() + first, items.begin() + mid, items.begin() + first + count,
+                     [&](int a, int c) { return (B[a].lo[ax] + B[a].hi[ax]) < (B[c].lo[ax] + B[c].hi[ax]); });
+    int l = build_rec(first, mid - first);
+    int r = build_rec(mid, first + count - mid);
+    nodes[idx].left = l; nodes[idx].right = r; nodes[idx].count = 0;
+    return idx;
+  }
+  template <class F> void query(const Box& q, F&& fn) const {
+    if (nodes.empty()) return;
+    int stack[128]; int sp = 0; stack[sp++] = 0;
+    while (sp) {
+      const Node& n = nodes[stack[--sp]];
+      if (!overlap(q, n.b)) continue;
+      if (n.count) {
+        for (int i = n.first; i < n.first + n.count; i++)
+          if (overlap(q, (*boxes)[items[i]])) fn(items[i]);
+      }
+      else { stack[sp++] = n.left; stack[sp++] = n.right; }
+    }
+  }
+};
+
+// ------------------------------------------------------------ the state
+struct Ctx {
+  // persistent storage sized by the reference's growth rule (n + n/5)
+  size_t alloc_pts = 0, alloc_tris = 0, alloc_edge_cost = 0;
+  std::vector<int> pts_map;        // alloc_pts + 1
+  std::vector<uint8_t> table;      // alloc_pts + 1
+  std::vector<uint32_t> edge_cost; // alloc_edge_cost + 1
+  std::vector<int> undo_list;      // grows as needed
+  std::vector<int> tmp_undo;       // grows as needed
+
+  // per-call state
+  int V = 0, F = 0, E = 0;
+  uint32_t collapse_t = 0;
+  float tau = 0, scale = 0;
+  std::vector<V3> P, P0;
+  std::vector<int> T, T0;           // 3F
+  std::vector<int> occ;             // V + 1
+  std::vector<int> fan_off, fan;    // V + 1, 3F'
+  std::vector<int> edges;           // 2E
+  std::vector<float> Q;             // 16V
+  std::vector<u64> tri_min;         // F
+  std::vector<int> collapsed;       // collapsed-edge list
+  int n_winners = 0;                // h: winners of the collapse pass
+  int undo_cut = 0;                 // collapsed-list entries scanned for undo
+  std::vector<uint8_t> S;           // faces found in intersecting pairs (query set only)
+  std::vector<uint8_t> S_first;     // S of the first check of the call
+  int n_vertices_undo = 0;
+  int undo_passes = 0;
+  long long n_queries = 0, n_candidates = 0, n_hits = 0;  // last call, summed over checks
+  int n_checks = 0;
+  int err = 0;
+
+  void ensure_pts(size_t n) {
+    if (n > alloc_pts) {
+      alloc_pts = n + n / 5;
+      pts_map.assign(alloc_pts + 1, 0);
+      table.assign(alloc_pts + 1, 0);
+      undo_list.assign(alloc_pts + 1, 0);
+      tmp_undo.assign(alloc_pts + 1, 0);
+    }
+  }
+  void ensure_edge_cost(size_t n) {
+    if (n > alloc_edge_cost) {
+      alloc_edge_cost = n + n / 5;
+      edge_cost.assign(alloc_edge_cost + 1, 0);
+    }
+  }
+  void ensure_undo(size_t n) {
+    if (n > undo_list.size()) undo_list.resize(n, 0);
+    if (n > tmp_undo.size()) tmp_undo.resize(n, 0);
+  }
+
+  inline bool dead(int f) const { return T[3 * f] == -1 && T[3 * f + 1] == -1 && T[3 * f + 2] == -1; }
+
+  // fan CSR, ascending face order (D0); cusimp_free.cu:276-304,950-964
+  void build_fans() {
+    fan_off.assign(V + 1, 0);
+    for (int f = 0; f < F; f++) {
+      if (dead(f)) continue;
+      fan_off[T[3 * f]]++; fan_off[T[3 * f + 1]]++; fan_off[T[3 * f + 2]]++;
+    }
+    int run = 0;
+    for (int v = 0; v <= V; v++) { int c = fan_off[v]; fan_off[v] = run; run += c; }
+    fan.assign(run, 0);
+    std::vector<int> cur(fan_off.begin(), fan_off.end() - 1);
+    for (int f = 0; f < F; f++) {
+      if (dead(f)) continue;
+      fan[cur[T[3 * f]]++] = f; fan[cur[T[3 * f + 1]]++] = f; fan[cur[T[3 * f + 2]]++] = f;
+    }
+  }
+
+  // cusimp_free.cu:306-335,966-978
+  void build_edges() {
+    edges.clear();
+    for (int f = 0; f < F; f++) {
+      if (dead(f)) continue;
+      int i = T[3 * f], j = T[3 * f + 1], k = T[3 * f + 2];
+      if (i > j) { edges.push_back(j); edges.push_back(i); }
+      if (j > k) { edges.push_back(k); edges.push_back(j); }
+      if (k > i) { edges.push_back(i); edges.push_back(k); }
+    }
+    E = (int)edges.size() / 2;
+  }
+
+  // cusimp_free.cu:337-366
+  void build_quadrics() {
+    Q.assign((size_t)16 * V, 0.0f);
+#pragma omp parallel for schedule(static)
+    for (int v = 0; v < V; v++) {
+      float K[16] = {0};
+      for (int s = fan_off[v]; s < fan_off[v + 1]; s++) {
+        int f = fan[s];
+        V3 v0 = P[T[3 * f]], v1 = P[T[3 * f + 1]], v2 = P[T[3 * f + 2]];
+        V3 n = vcross(vsub(v1, v0), vsub(v2, v0));
+        n = vdivs(n, vnorm(n));
+        float off = -vdot(n, v0);
+        float p[4] = {n.x, n.y, n.z, off};
+        for (int r = 0; r < 4; r++) for (int c = 0; c < 4; c++) K[4 * r + c] += p[r] * p[c];
+      }
+      memcpy(&Q[(size_t)16 * v], K, sizeof(K));
+    }
+  }
+
+  static inline float tri_area(V3 p0, V3 p1, V3 p2) {   // cusimp_free.cu:368-375
+    float a = vnorm(vsub(p0, p1)), b = vnorm(vsub(p1, p2)), c = vnorm(vsub(p2, p0));
+    float s = (a + b + c) / 2;
+    return sqrtf(s * (s - a) * (s - b) * (s - c));
+  }
+
+  // One skinny/flip pass (cusimp_free.cu:497-575).  Returns false when a
+  // normal flips (cost becomes max).
+  bool skinny_pass(int moved, int other, V3 m, float& Qa, int& nt) const {
+    for (int s = fan_off[moved]; s < fan_off[moved + 1]; s++) {
+      int f = fan[s];
+      int i = T[3 * f], j = T[3 * f + 1], k = T[3 * f + 2];
+      if (i == other || j == other || k == other) continue;
+      V3 o0 = P[i], o1 = P[j], o2 = P[k];
+      V3 n0 = o0, n1 = o1, n2 = o2;
+      if (i == moved) n0 = m;
+      if (j == moved) n1 = m;
+      if (k == moved) n2 = m;
+      V3 on = vcross(vsub(o1, o0), vsub(o2, o0));
+      on = vdivs(on, vnorm(on));
+      V3 nn = vcross(vsub(n1, n0), vsub(n2, n0));
+      nn = vdivs(nn, vnorm(nn));
+      if (vdot(on, nn) < 0) return false;
+      float e01 = vnorm(vsub(n0, n1)), e12 = vnorm(vsub(n1, n2)), e20 = vnorm(vsub(n2, n0));
+      float term = 4.0f * sqrtf(3.0f) * tri_area(n0, n1, n2) / (e01 * e01) + e12 * e12 + e20 * e20 + 0.0000001f;
+      Qa += 1.0f - clamp_ref(term, 0.0f, 1.0f);
+      nt++;
+    }
+    return true;
+  }
+
+  static inline int succ(const int* t, int x) {
+    if (t[0] == x) return t[1];
+    if (t[1] == x) return t[2];
+    if (t[2] == x) return t[0];
+    return -2;   // unreachable: every fan face contains its vertex
+  }
+
+  // cusimp_free.cu:382-585
+  void edge_cost_one(int e, bool is_stuck) {
+    int u = edges[2 * e], v = edges[2 * e + 1];
+    V3 v0 = P[u], v1 = P[v];
+    if (table[u] && table[v]) { edge_cost[e] = 0xFFFFFFFFu; return; }
+    if (is_stuck) return;
+    int dup = 0;
+    for (int a = fan_off[u]; a < fan_off[u + 1]; a++) {
+      int iu = succ(&T[3 * fan[a]], u);
+      for (int b = fan_off[v]; b < fan_off[v + 1]; b++) {
+        int iv = succ(&T[3 * fan[b]], v);
+        if (iu == iv) dup++;
+        if (dup > 2) { edge_cost[e] = 0xFFFFFFFFu; return; }
+      }
+    }
+    if (dup != 2) return;
+    float el = 0; int ne = 0;
+    for (int w : {u, v}) {
+      for (int s = fan_off[w]; s < fan_off[w + 1]; s++) {
+        const int* t = &T[3 * fan[s]];
+        if (t[0] > t[1]) { el += vnorm(vsub(P[t[0]], P[t[1]])); ne++; }
+        if (t[1] > t[2]) { el += vnorm(vsub(P[t[1]], P[t[2]])); ne++; }
+        if (t[2] > t[0]) { el += vnorm(vsub(P[t[2]], P[t[0]])); ne++; }
+      }
+    }
+    el = el / (float)ne / scale * tau;
+    el = el + vnorm(vsub(v0, v1)) / scale * tau;
+    V3 m = vdivs(vadd(v0, v1), 2.0f);
+    float Qa = 0; int nt = 0;
+    if (!skinny_pass(u, v, m, Qa, nt)) { edge_cost[e] = 0xFFFFFFFFu; return; }
+    if (!skinny_pass(v, u, m, Qa, nt)) { edge_cost[e] = 0xFFFFFFFFu; return; }
+    Qa = 5.0f * Qa;
+    float M[16];
+    for (int r = 0; r < 16; r++) M[r] = Q[(size_t)16 * u + r] + Q[(size_t)16 * v + r];
+    float w[4] = {m.x, m.y, m.z, 1.0f};
+    float t[4];
+    for (int c = 0; c < 4; c++) t[c] = M[c] * w[0] + M[4 + c] * w[1] + M[8 + c] * w[2] + M[12 + c] * w[3];
+    float qe = t[0] * w[0] + t[1] * w[1] + t[2] * w[2] + t[3] * w[3];
+    float cost = qe / (scale * scale);
+    edge_cost[e] = u32sat(clamp_ref(cost + el + Qa / (float)nt * tau, 0.0f, COST_RANGE) / COST_RANGE * U32F);
+  }
+
+  // cusimp_free.cu:587-610
+  void claim() {
+    tri_min.assign(F, ~0ull);
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < E; e++) {
+      u64 key = ((u64)edge_cost[e] << 32) | (u64)(uint32_t)e;
+      for (int w : {edges[2 * e], edges[2 * e + 1]}) {
+        for (int s = fan_off[w]; s < fan_off[w + 1]; s++) {
+          u64* slot = &tri_min[fan[s]];
+          u64 cur = __atomic_load_n(slot, __ATOMIC_RELAXED);
+          while (key < cur && !__atomic_compare_exchange_n(slot, &cur, key, true, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {}
+        }
+      }
+    }
+  }
+
+  static bool is_skinny(V3 p, V3 q, V3 r) {   // cusimp_free.cu:617-637
+    V3 pq = vsub(q, p), pr = vsub(r, p), qr = vsub(r, q);
+    float lpq = sqrtf(vdot(pq, pq)), lpr = sqrtf(vdot(pr, pr)), lqr = sqrtf(vdot(qr, qr));
+    float ml = fmaxf(lpq, fmaxf(lpr, lqr));
+    V3 c = vcross(pq, pr);
+    float thr = (float)(1e-6 * (double)ml * (double)ml);
+    float area = 0.5f * sqrtf(c.x * c.x + c.y * c.y + c.z * c.z);
+    return area < thr;
+  }
+
+  void restore_edge(int e) {   // cusimp_free.cu:746-773,829-867
+    int u = edges[2 * e], v = edges[2 * e + 1];
+    occ[v] = 1;
+    P[u] = P0[u]; P[v] = P0[v];
+    for (int w : {u, v})
+      for (int s = fan_off[w]; s < fan_off[w + 1]; s++) {
+        int f = fan[s];
+        T[3 * f] = T0[3 * f]; T[3 * f + 1] = T0[3 * f + 1]; T[3 * f + 2] = T0[3 * f + 2];
+      }
+  }
+
+  void surgery(int u, int v) {   // face updates of cusimp_free.cu:691-715 / 249-273
+    for (int s = fan_off[u]; s < fan_off[u + 1]; s++) {
+      int* t = &T[3 * fan[s]];
+      if (t[0] == v || t[1] == v || t[2] == v) t[0] = t[1] = t[2] = -1;
+    }
+    for (int s = fan_off[v]; s < fan_off[v + 1]; s++) {
+      int* t = &T[3 * fan[s]];
+      if (t[0] == u || t[1] == u || t[2] == u) t[0] = t[1] = t[2] = -1;
+      else if (t[0] == v) t[0] = u;
+      else if (t[1] == v) t[1] = u;
+      else if (t[2] == v) t[2] = u;
+    }
+  }
+
+  bool wins(int e) const {   // claim verification, cusimp_free.cu:648-681
+    if (edge_cost[e] > collapse_t) return false;
+    u64 key = ((u64)edge_cost[e] << 32) | (u64)(uint32_t)e;
+    for (int w : {edges[2 * e], edges[2 * e + 1]})
+      for (int s = fan_off[w]; s < fan_off[w + 1]; s++)
+        if (tri_min[fan[s]] != key) return false;
+    return true;
+  }
+
+  // cusimp_free.cu:639-774 (winners are independent, so order is free)
+  void collapse() {
+    std::vector<uint8_t> win(E, 0);
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < E; e++) win[e] = wins(e) ? 1 : 0;
+    collapsed.clear();
+    for (int e = 0; e < E; e++) if (win[e]) collapsed.push_back(e);
+    n_winners = (int)collapsed.size();
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n_winners; i++) {
+      int e = collapsed[i];
+      int u = edges[2 * e], v = edges[2 * e + 1];
+      V3 m = vdivs(vadd(P[u], P[v]), 2.0f);
+      P[u] = m; P[v] = {0, 0, 0}; occ[v] = 0;
+      surgery(u, v);
+      bool sk = false;
+      for (int w : {u, v}) {
+        if (sk) break;
+        for (int s = fan_off[w]; s < fan_off[w + 1]; s++) {
+          const int* t = &T[3 * fan[s]];
+          if (t[0] == -1) continue;
+          if (is_skinny(P[t[0]], P[t[1]], P[t[2]])) { sk = true; break; }
+        }
+      }
+      if (sk) restore_edge(e);
+    }
+  }
+
+  void remove_invalid_faces() {   // cusimp_free.cu:218-232
+    for (int f = 0; f < F; f++) {
+      int* t = &T[3 * f];
+      if (t[0] == -1 && t[1] == -1 && t[2] == -1) continue;
+      if (t[0] == t[1] || t[1] == t[2] || t[2] == t[0]) t[0] = t[1] = t[2] = -1;
+    }
+  }
+
+  void remove_line_edges() {   // cusimp_free.cu:234-274, sequential (D5)
+    for (int e = 0; e < E; e++) {
+      int u = edges[2 * e], v = edges[2 * e + 1];
+      V3 p = P[u], q = P[v];
+      if (!(p.x == q.x && p.y == q.y && p.z == q.z)) continue;
+      collapsed.push_back(e);
+      P[v] = {0, 0, 0}; occ[v] = 0;
+      surgery(u, v);
+    }
+  }
+
+  // Faces of the fans of the first `cut` collapsed edges (query set).
+  void query_set(std::vector<int>& qs) const {
+    std::vector<uint8_t> in(F, 0);
+    for (int i = 0; i < undo_cut; i++) {
+      int e = collapsed[i];
+      for (int w : {edges[2 * e], edges[2 * e + 1]})
+        for (int s = fan_off[w]; s < fan_off[w + 1]; s++) in[fan[s]] = 1;
+    }
+    qs.clear();
+    for (int f = 0; f < F; f++) if (in[f] && !dead(f)) qs.push_back(f);
+  }
+
+  inline void face_pts(int f, float out[9]) const {
+    for (int v = 0; v < 3; v++) { V3 p = P[T[3 * f + v]]; out[3 * v] = p.x; out[3 * v + 1] = p.y; out[3 * v + 2] = p.z; }
+  }
+  inline Box face_box(int f) const {
+    Box b;
+    V3 a = P[T[3 * f]], c = P[T[3 * f + 1]], d = P[T[3 * f + 2]];
+    b.lo[0] = fminf(a.x, fminf(c.x, d.x)); b.lo[1] = fminf(a.y, fminf(c.y, d.y)); b.lo[2] = fminf(a.z, fminf(c.z, d.z));
+    b.hi[0] = fmaxf(a.x, fmaxf(c.x, d.x)); b.hi[1] = fmaxf(a.y, fmaxf(c.y, d.y)); b.hi[2] = fmaxf(a.z, fmaxf(c.z, d.z));
+    return b;
+  }
+
+  // One self-intersection check (self_intersect.cuh:151-392) restricted to
+  // the query set: S[f] = 1 iff f is in an ordered intersecting pair.
+  void check() {
+    n_checks++;
+    std::vector<int> qs;
+    query_set(qs);
+    S.assign(F, 0);
+    if (qs.empty()) return;
+    std::vector<Box> boxes(F);
+    std::vector<int> alive;
+    alive.reserve(F);
+    for (int f = 0; f < F; f++) if (!dead(f)) { boxes[f] = face_box(f); alive.push_back(f); }
+    CpuBVH bvh;
+    bvh.build(boxes, alive);
+    long long ncand = 0, nhit = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : ncand, nhit)
+    for (size_t i = 0; i < qs.size(); i++) {
+      int f = qs[i];
+      float pf[9]; face_pts(f, pf);
+      const int* tf = &T[3 * f];
+      bool hit = false;
+      bvh.query(boxes[f], [&](int g) {
+        if (g == f) return;
+        ncand++;
+        float pg[9]; face_pts(g, pg);
+        const int* tg = &T[3 * g];
+        if (narrow_pair(pg, pf, tg, tf) == NP_HIT) { hit = true; nhit++; }
+        if (narrow_pair(pf, pg, tf, tg) == NP_HIT) { hit = true; nhit++; }
+      });
+      if (hit) S[f] = 1;
+    }
+    n_queries += (long long)qs.size();
+    n_candidates += ncand;
+    n_hits += nhit;
+  }
+
+  // cusimp_free.cu:776-827 over the first `undo_cut` collapsed entries;
+  // returns the number of marked edges, their endpoints in tmp_undo.
+  int mark(std::vector<int>& marked) {
+    marked.clear();
+    for (int i = 0; i < undo_cut; i++) {
+      int e = collapsed[i];
+      bool flag = false;
+      for (int w : {edges[2 * e], edges[2 * e + 1]}) {
+        for (int s = fan_off[w]; s < fan_off[w + 1]; s++) if (S[fan[s]]) { flag = true; break; }
+        if (flag) break;
+      }
+      if (flag) marked.push_back(e);
+    }
+    ensure_undo(2 * marked.size() + 1);
+    for (size_t j = 0; j < marked.size(); j++) {
+      tmp_undo[2 * j] = edges[2 * marked[j]];
+      tmp_undo[2 * j + 1] = edges[2 * marked[j] + 1];
+    }
+    return (int)marked.size();
+  }
+
+  int forward(const float* pts, const int* tris, const int* undo_in, int n_u, int nV, int nF,
+              float s, float threshold, bool is_stuck, bool init, int shuffle_mode) {
+    err = 0;
+    n_queries = n_candidates = n_hits = 0; n_checks = 0; undo_passes = 0;
+    // validate (the reference has undefined behaviour on these)
+    for (int f = 0; f < nF; f++) {
+      const int* t = &tris[3 * f];
+      if (t[0] == -1 && t[1] == -1 && t[2] == -1) continue;
+      for (int k = 0; k < 3; k++) if (t[k] < 0 || t[k] >= nV) return -2;
+    }
+    // step 1: invalid table / incoming undo list, on the storage as it is
+    n_vertices_undo = 0;
+    int first = 0;
+    if (is_stuck) {
+      ensure_undo((size_t)n_u + 1);
+      for (int i = 0; i < n_u; i++) tmp_undo[i] = undo_in[i];
+      n_vertices_undo += n_u;
+      for (int i = 0; i < n_u; i++) {
+        int x = tmp_undo[i];
+        undo_list[i] = (x >= 0 && (size_t)x < pts_map.size()) ? pts_map[x] : 0;
+      }
+      first += n_u;
+    } else if (n_u != 0) {
+      std::fill(table.begin(), table.end(), 0);
+    }
+    if (n_u != 0)
+      for (int i = 0; i < n_u; i++) {
+        int x = undo_in[i];
+        if (x >= 0 && (size_t)x < table.size()) table[x] = 1;
+      }
+    tau = threshold; scale = s;
+    collapse_t = collapse_threshold(threshold);
+    V = nV; F = nF;
+    // growth reallocates (and so zero-fills) pts_map, the table and the undo
+    // lists after step 1 has written them, as cusimp_free.cu:928 does
+    ensure_pts(V);
+    P.resize(V);
+    for (int v = 0; v < V; v++) P[v] = {pts[3 * v], pts[3 * v + 1], pts[3 * v + 2]};
+    occ.assign(V + 1, 1); occ[V] = 0;
+    if ((size_t)F > alloc_tris) alloc_tris = F + F / 5;
+    T.assign(tris, tris + 3 * (size_t)F);
+    if (init && shuffle_mode == 1) {
+      std::vector<int64_t> perm;
+      shuffle_permutation(F, perm);
+      std::vector<int> src(T);
+      for (int j = 0; j < F; j++) for (int k = 0; k < 3; k++) T[3 * j + k] = src[3 * perm[j] + k];
+    }
+    P0 = P; T0 = T;
+    build_fans();
+    build_edges();
+    build_quadrics();
+    ensure_edge_cost(E);
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int e = 0; e < E; e++) edge_cost_one(e, is_stuck);
+    claim();
+    collapse();
+    remove_invalid_faces();
+    remove_line_edges();
+    {
+      int h = n_winners, tot = (int)collapsed.size();
+      undo_cut = h == 0 ? 0 : std::min(tot, ((h + 511) / 512) * 512);
+    }
+    check();
+    S_first = S;
+    std::vector<int> marked;
+    int hm = mark(marked);
+    n_vertices_undo += 2 * hm;
+    int it = 0;
+    while (hm != 0) {
+      it++;
+      for (int e : marked) restore_edge(e);
+      ensure_undo((size_t)n_vertices_undo + 1);
+      for (int j = 0; j < n_vertices_undo - first; j++) {
+        int x = tmp_undo[j];
+        undo_list[first + j] = (x >= 0 && (size_t)x < pts_map.size()) ? pts_map[x] : 0;
+      }
+      first += 2 * hm;
+      check();
+      hm = mark(marked);
+      n_vertices_undo += 2 * hm;
+      if (it == 5) break;
+    }
+    undo_passes = it;
+    ensure_undo((size_t)n_vertices_undo + 1);
+    for (int j = first; j < n_vertices_undo; j++) undo_list[j] = 0;   // D6
+    // pts_map = exclusive scan of occ over V + 1 entries
+    int run = 0;
+    for (int v = 0; v <= V; v++) { int o = occ[v]; pts_map[v] = run; run += o; }
+    return 0;
+  }
+};
+
+// ------------------------------------------------------------ the driver
+struct RunStats {
+  int rounds, final_V, final_F, exit_reason;   // 1 target, 2 <=10 faces, 3 tolerance, 4 iter cap
+  long long total_collapsed, total_undo_edges, max_undo_passes;
+};
+
+// PaMO.run stage-2 loop (simp_cuda/pamo/__init__.py:121-158) on centred
+// vertices; `target` = max(int(ratio*F0), min_verts).
+static int run_loop(Ctx& c, const float* verts, const int* faces, int V, int F, float scale, long long target,
+                    float threshold, int tolerance, long long iters, int shuffle_mode,
+                    std::vector<float>& outV, std::vector<int>& outF, RunStats& st,
+                    std::vector<int>* round_faces, double max_seconds = 0.0, std::vector<double>* round_secs = nullptr) {
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<float> cv(verts, verts + 3 * (size_t)V);
+  std::vector<int> cf(faces, faces + 3 * (size_t)F);
+  std::vector<int> undo;
+  int count = 0, is_stuck = 0;
+  bool init = true;
+  memset(&st, 0, sizeof(st));
+  st.exit_reason = 4;
+  for (long long it = 0; it < iters; it++) {
+    int nprev = (int)(cf.size() / 3);
+    int rc = c.forward(cv.data(), cf.data(), undo.data(), (int)undo.size(), (int)(cv.size() / 3), nprev,
+                       scale, threshold, is_stuck != 0, init, shuffle_mode);
+    if (rc) return rc;
+    init = false;
+    st.rounds++;
+    st.total_collapsed += c.n_winners;
+    st.max_undo_passes = std::max<long long>(st.max_undo_passes, c.undo_passes);
+    undo.assign(c.undo_list.begin(), c.undo_list.begin() + c.n_vertices_undo);
+    // compaction (pamo/__init__.py:136-140)
+    std::vector<float> nv;
+    nv.reserve(cv.size());
+    for (int v = 0; v < c.V; v++)
+      if (c.occ[v]) { nv.push_back(c.P[v].x); nv.push_back(c.P[v].y); nv.push_back(c.P[v].z); }
+    std::vector<int> nf;
+    nf.reserve(cf.size());
+    for (int f = 0; f < c.F; f++) {
+      if (c.T[3 * f] >= 0)
+        for (int k = 0; k < 3; k++) nf.push_back(c.pts_map[c.T[3 * f + k]]);
+    }
+    cv.swap(nv); cf.swap(nf);
+    int ncur = (int)(cf.size() / 3);
+    if (round_faces) round_faces->push_back(ncur);
+    double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (round_secs) round_secs->push_back(el);
+    if (max_seconds > 0 && el > max_seconds) { st.exit_reason = 5; break; }
+    if (ncur <= target || ncur <= 10) { st.exit_reason = ncur <= target ? 1 : 2; break; }
+    if (ncur == nprev) count++;
+    else { count = 0; is_stuck = 0; }
+    if (count >= 2) is_stuck = 1;
+    if (count == tolerance) { st.exit_reason = 3; break; }
+  }
+  outV = cv; outF = cf;
+  st.final_V = (int)(cv.size() / 3);
+  st.final_F = (int)(cf.size() / 3);
+  return 0;
+}
+
+}  // namespace orc
+
+// ================================================================ C API
+extern "C" {
+
+void* orc_create(void) { return new orc::Ctx(); }
+void orc_destroy(void* c) { delete (orc::Ctx*)c; }
+
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
+int orc_forward(void* ctx, const float* pts, const int* tris, const int* undo, int n_undo, int n_pts, int n_tris,
+                float scale, float threshold, int is_stuck, int init, int shuffle_mode) {
+  return ((orc::Ctx*)ctx)->forward(pts, tris, undo, n_undo, n_pts, n_tris, scale, threshold, is_stuck != 0,
+                                   init != 0, shuffle_mode);
+}
+
+// sizes: V, F, E, n_vertices_undo, n_collapsed(total), n_winners, undo_passes, n_checks,
+//        n_fan, undo_cut, collapse_t, alloc_pts
+void orc_sizes(void* ctx, long long out[16]) {
+  orc::Ctx* c = (orc::Ctx*)ctx;
+  long long v[16] = {c->V, c->F, c->E, c->n_vertices_undo, (long long)c->collapsed.size(), c->n_winners,
+                     c->undo_passes, c->n_checks, (long long)c->fan.size(), c->undo_cut, c->collapse_t,
+                     (long long)c->alloc_pts, c->n_queries, c->n_candidates, c->n_hits, 0};
+  memcpy(out, v, sizeof(v));
+}
+
+// Copies an internal buffer.  which: 0 points(3V f32) 1 tris(3F i32) 2 occ(V)
+// 3 pts_map(V) 4 undo list(n_vertices_undo) 5 fan_off(V+1) 6 fan 7 edges(2E)
+// 8 Q(16V f32) 9 edge_cost(E u32) 10 tri_min(F u64) 11 collapsed list
+// 12 S of the first check (F u8) 13 orig tris (3F) 14 table (alloc+1 u8)
+long long orc_copy(void* ctx, int which, void* dst, long long cap_bytes) {
+  orc::Ctx* c = (orc::Ctx*)ctx;
+  const void* src = nullptr; long long n = 0;
+  switch (which) {
+    case 0: src = c->P.data(); n = 12LL * c->V; break;
+    case 1: src = c->T.data(); n = 12LL * c->F; break;
+    case 2: src = c->occ.data(); n = 4LL * c->V; break;
+    case 3: src = c->pts_map.data(); n = 4LL * c->V; break;
+    case 4: src = c->undo_list.data(); n = 4LL * c->n_vertices_undo; break;
+    case 5: src = c->fan_off.data(); n = 4LL * (c->V + 1); break;
+    case 6: src = c->fan.data(); n = 4LL * (long long)c->fan.size(); break;
+    case 7: src = c->edges.data(); n = 8LL * c->E; break;
+    case 8: src = c->Q.data(); n = 64LL * c->V; break;
+    case 9: src = c->edge_cost.data(); n = 4LL * c->E; break;
+    case 10: src = c->tri_min.data(); n = 8LL * c->F; break;
+    case 11: src = c->collapsed.data(); n = 4LL * (long long)c->collapsed.size(); break;
+    case 12: src = c->S_first.data(); n = (long long)c->S_first.size(); break;
+    case 13: src = c->T0.data(); n = 12LL * c->F; break;
+    case 14: src = c->table.data(); n = (long long)c->table.size(); break;
+    default: return -1;
+  }
+  if (dst && n <= cap_bytes) memcpy(dst, src, (size_t)n);
+  return n;
+}
+
+int orc_run(const float* verts, const int* faces, int V, int F, float scale, long long target, float threshold,
+            int tolerance, long long iters, int shuffle_mode, float* outV, int* outF, long long stats[8],
+            int* round_faces, int round_cap) {
+  orc::Ctx c;
+  std::vector<float> ov; std::vector<int> of; orc::RunStats st;
+  std::vector<int> rf;
+  int rc = orc::run_loop(c, verts, faces, V, F, scale, target, threshold, tolerance, iters, shuffle_mode, ov, of, st, &rf);
+  if (rc) return rc;
+  memcpy(outV, ov.data(), ov.size() * 4);
+  memcpy(outF, of.data(), of.size() * 4);
+  long long s[8] = {st.rounds, st.final_V, st.final_F, st.exit_reason, st.total_collapsed, st.total_undo_edges,
+                    st.max_undo_passes, 0};
+  memcpy(stats, s, sizeof(s));
+  for (int i = 0; i < (int)rf.size() && i < round_cap; i++) round_faces[i] = rf[i];
+  return 0;
+}
+
+// Time-bounded run: stops after the first round that ends past max_seconds
+// (exit reason 5).  round_secs[r] = seconds elapsed at the end of round r.
+int orc_run_timed(const float* verts, const int* faces, int V, int F, float scale, long long target, float threshold,
+                  int tolerance, long long iters, int shuffle_mode, double max_seconds, long long stats[8],
+                  int* round_faces, double* round_secs, int round_cap) {
+  orc::Ctx c;
+  std::vector<float> ov; std::vector<int> of; orc::RunStats st;
+  std::vector<int> rf; std::vector<double> rs;
+  int rc = orc::run_loop(c, verts, faces, V, F, scale, target, threshold, tolerance, iters, shuffle_mode, ov, of, st,
+                         &rf, max_seconds, &rs);
+  if (rc) return rc;
+  long long s[8] = {st.rounds, st.final_V, st.final_F, st.exit_reason, st.total_collapsed, 0, st.max_undo_passes, 0};
+  memcpy(stats, s, sizeof(s));
+  for (int i = 0; i < (int)rf.size() && i < round_cap; i++) { round_faces[i] = rf[i]; round_secs[i] = rs[i]; }
+  return 0;
+}
+
+void orc_shuffle_perm(long long m, long long* out) {
+  std::vector<int64_t> p;
+  orc::shuffle_permutation(m, p);
+  for (size_t i = 0; i < p.size(); i++) out[i] = p[i];
+}
+
+// narrow phase on one ordered pair (current, query); returns the outcome code
+int orc_narrow(const float cur[9], const float qry[9], const int ci[3], const int qi[3]) {
+  return orc::narrow_pair(cur, qry, ci, qi);
+}
+
+// Raw reference interval test (tri_tri_3d.cuh:131-276), for tests.
+int orc_isect_raw(const float a[9], const float b[9], float ri1[3], float ri2[3]) {
+  float ta[3][3], tb[3][3];
+  memcpy(ta, a, 36); memcpy(tb, b, 36);
+  return orc::isect_tri_tri_ref(ta, tb, ri1, ri2) ? 1 : 0;
+}
+
+int orc_is_coplanar(const float a[9], const float b[9]) {
+  float ta[3][3], tb[3][3];
+  memcpy(ta, a, 36); memcpy(tb, b, 36);
+  return orc::is_coplanar_ref(ta, tb) ? 1 : 0;
+}
+
+// Full check over every alive face of a mesh (the reference's own query
+// set): writes the ordered candidate pairs (q,c) with AABB overlap and the
+// ordered intersecting pairs.  Returns counts in out_counts[0..1]; arrays are
+// filled up to their capacities (pairs sorted by (q,c)).
+int orc_full_check(const float* pts, const int* tris, int V, int F, int* cand, long long cand_cap, int* hits,
+                   long long hit_cap, long long out_counts[2]) {
+  orc::Ctx c;
+  c.V = V; c.F = F;
+  c.P.resize(V);
+  for (int v = 0; v < V; v++) c.P[v] = {pts[3 * v], pts[3 * v + 1], pts[3 * v + 2]};
+  c.T.assign(tris, tris + 3 * (size_t)F);
+  std::vector<orc::Box> boxes(F);
+  std::vector<int> alive;
+  for (int f = 0; f < F; f++) if (!c.dead(f)) { boxes[f] = c.face_box(f); alive.push_back(f); }
+  orc::CpuBVH bvh; bvh.build(boxes, alive);
+  std::vector<std::pair<int, int>> cp, hp;
+  for (int f : alive) {
+    float pf[9]; c.face_pts(f, pf);
+    std::vector<int> found;
+    bvh.query(boxes[f], [&](int g) { if (g != f) found.push_back(g); });
+    std::sort(found.begin(), found.end());
+    for (int g : found) {
+      cp.push_back({f, g});
+      float pg[9]; c.face_pts(g, pg);
+      if (orc::narrow_pair(pg, pf, &c.T[3 * g], &c.T[3 * f]) == orc::NP_HIT) hp.push_back({f, g});
+    }
+  }
+  out_counts[0] = (long long)cp.size(); out_counts[1] = (long long)hp.size();
+  for (long long i = 0; i < (long long)cp.size() && 2 * i + 1 < cand_cap; i++) { cand[2 * i] = cp[i].first; cand[2 * i + 1] = cp[i].second; }
+  for (long long i = 0; i < (long long)hp.size() && 2 * i + 1 < hit_cap; i++) { hits[2 * i] = hp[i].first; hits[2 * i + 1] = hp[i].second; }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,183 @@
+"""MI355X-native PaMO stage 2: guarded parallel QEM edge collapse.
+
+Drop-in for the reference's ``pamo`` package on its stage-2 path
+(simp_cuda/pamo/__init__.py): ``PaMO(mesh, use_stage1, use_stage3).run(
+points, triangles, ratio, tolerance, threshold, iter, min_verts)`` and the
+extension module ``pamo_amd._C`` (``CUDSP_Free``).  The rounds run in
+libpamo_hip.so (hand-written gfx950 HIP kernels); stages 1 and 3 are not
+part of this package.
+"""
+import time
+
+import numpy as np
+import torch
+
+from . import _C, _lib
+from ._lib import PamoError
+
+__all__ = ["PaMO", "PaSP", "PamoError", "simplify_stage2", "preprocess_numpy"]
+
+
+def _as_vf(input_mesh):
+    if isinstance(input_mesh, (tuple, list)) and len(input_mesh) == 2:
+        return np.asarray(input_mesh[0]), np.asarray(input_mesh[1])
+    return np.asarray(input_mesh.vertices), np.asarray(input_mesh.faces)
+
+
+def _preprocess(points, triangles):
+    """tris_mean exactly as simp_cuda/pamo/__init__.py:60-64 (numpy float32)."""
+    tris = points[triangles.long()]
+    tris = tris.cpu().numpy()
+    return tris.mean(axis=1).mean(axis=0)
+
+
+def preprocess_numpy(points, faces):
+    """Centring and scale of PaMO.run (simp_cuda/pamo/__init__.py:60-64,117,126)
+    on host arrays: returns (centred float32 vertices, scale, tris_mean)."""
+    points = np.asarray(points, np.float32)
+    tris_mean = points[np.asarray(faces)].mean(axis=1).mean(axis=0)
+    verts = (points - tris_mean).astype(np.float32)
+    ext = verts.max(axis=0) - verts.min(axis=0)
+    return verts, float(max(max(ext[0], ext[1]), ext[2])), tris_mean
+
+
+def _scale(verts):
+    """simp_cuda/pamo/__init__.py:126: the largest axis extent."""
+    ext = [verts[:, k].max() - verts[:, k].min() for k in range(3)]
+    return float(max(max(ext[0], ext[1]), ext[2]))
+
+
+def simplify_stage2(verts, faces, scale, target, threshold=1e-3, tolerance=4, iters=1000000, ctx=None,
+                    shuffle=_lib.SHUFFLE_FEISTEL):
+    """Device-resident stage-2 loop on centred vertices (torch CUDA tensors).
+
+    Returns (verts, faces, stats) as torch CUDA tensors and a dict."""
+    import ctypes
+    if ctx is None:
+        ctx = _lib.Context(verts.device.index if verts.device.index is not None else torch.cuda.current_device(),
+                           shuffle)
+    verts = verts.contiguous()
+    faces = faces.contiguous()
+    if verts.dtype != torch.float32 or faces.dtype != torch.int32:
+        raise RuntimeError("verts must be float32 and faces int32")
+    L = ctx._L
+    stream = torch.cuda.current_stream().cuda_stream
+    oV, oF = ctypes.c_int(), ctypes.c_int()
+    st = np.zeros(8, np.int64)
+    rc = L.pamo_run_stage2(ctx.handle, stream, verts.data_ptr(), faces.data_ptr(), int(verts.shape[0]),
+                           int(faces.shape[0]), float(scale), int(target), float(threshold), int(tolerance),
+                           int(iters), ctypes.byref(oV), ctypes.byref(oF), st.ctypes.data)
+    ctx.check(rc, "pamo_run_stage2")
+    ov = torch.empty((oV.value, 3), dtype=torch.float32, device=verts.device)
+    of = torch.empty((oF.value, 3), dtype=torch.int32, device=verts.device)
+    ctx.check(L.pamo_run_fetch(ctx.handle, stream, ov.data_ptr(), of.data_ptr()), "pamo_run_fetch")
+    stats = dict(rounds=int(st[0]), exit_reason=int(st[1]), collapses=int(st[2]), max_undo_passes=int(st[3]),
+                 checks=int(st[4]), query_faces=int(st[5]), candidate_pairs=int(st[6]), hit_faces=int(st[7]))
+    rf = np.zeros(max(stats["rounds"], 1), np.int32)
+    L.pamo_run_rounds(ctx.handle, rf.ctypes.data, rf.size)
+    stats["round_faces"] = rf[:stats["rounds"]]
+    return ov, of, stats
+
+
+class PaMO(torch.nn.Module):
+    """simp_cuda/pamo/__init__.py:18-178 restricted to stage 2.
+
+    ``engine="device"`` (default) keeps the whole round loop on the GPU;
+    ``engine="rounds"`` runs the reference's Python loop over
+    ``CUDSP_Free.forward`` with torch compaction.  Both give identical
+    results."""
+
+    def __init__(self, input_mesh, use_stage1=True, use_stage3=True, verbose=True, engine="device",
+                 shuffle=_lib.SHUFFLE_FEISTEL):
+        super().__init__()
+        if use_stage1:
+            raise NotImplementedError("stage 1 (SDF remeshing) is not part of pamo_amd; pass use_stage1=False")
+        if use_stage3:
+            raise NotImplementedError("stage 3 (safe projection) is not part of pamo_amd; pass use_stage3=False")
+        self.use_stage1 = use_stage1
+        self.use_stage3 = use_stage3
+        self.verbose = verbose
+        self.engine = engine
+        if verbose:
+            print("Stage1 : ", self.use_stage1)
+            print("Stage3 : ", self.use_stage3)
+        v, _ = _as_vf(input_mesh)
+        self.bbox = np.stack([v.min(0), v.max(0)])
+        self.pamo = _C.CUDSP_Free(shuffle)
+        self.target_faces = None
+        self.last_stats = None
+        self.last_stage2_seconds = None
+
+    def run(self, points, triangles, ratio, tolerance=4, threshold=1e-3, iter=1000000, min_verts=10000000000):
+        self.target_faces = max(int(ratio * len(triangles)), min_verts)
+        if self.verbose:
+            print("Target faces : {}".format(self.target_faces))
+        tris_mean = _preprocess(points, triangles)
+        verts = points - torch.from_numpy(tris_mean).to(points.device)
+        faces = triangles
+        start = time.time()
+        scale = _scale(verts)
+        if self.engine == "device":
+            verts, faces, stats = simplify_stage2(verts, faces, scale, self.target_faces, threshold, tolerance,
+                                                  iter, ctx=self.pamo._ctx)
+            if stats["exit_reason"] == 3 and self.verbose:
+                print("Not enough edges available to be collapsed.")
+        else:
+            verts, faces, stats = self._run_rounds(verts, faces, scale, tolerance, threshold, iter)
+        torch.cuda.synchronize()
+        end = time.time()
+        self.last_stats = stats
+        self.last_stage2_seconds = end - start
+        verts = verts.cpu().numpy() + tris_mean
+        faces = faces.cpu().numpy()
+        if self.verbose:
+            print(f"Time for Simplification: {end - start} sec")
+        return verts, faces
+
+    def _run_rounds(self, verts, faces, scale, tolerance, threshold, iters):
+        """The reference's loop (simp_cuda/pamo/__init__.py:122-158) verbatim
+        in behaviour, one CUDSP_Free.forward per round."""
+        verts_undo = torch.empty(0, dtype=torch.int32, device=verts.device)
+        n_verts_undo = 0
+        count = 0
+        is_stuck = 0
+        init = True
+        rounds = 0
+        exit_reason = 4
+        for it in range(iters):
+            num_faces_prev = faces.shape[0]
+            verts, faces, verts_occ, verts_map, verts_undo = self.pamo.forward(
+                verts, faces, verts_undo, n_verts_undo, scale, threshold, is_stuck, init)
+            init = False
+            rounds += 1
+            n_verts_undo = verts_undo.shape[0]
+            verts = verts[verts_occ.view(-1).bool()]
+            faces = faces[faces[:, 0] >= 0]
+            faces[:, 0] = verts_map[faces[:, 0].long()].view(-1)
+            faces[:, 1] = verts_map[faces[:, 1].long()].view(-1)
+            faces[:, 2] = verts_map[faces[:, 2].long()].view(-1)
+            num_faces_current = faces.shape[0]
+            if num_faces_current <= self.target_faces or num_faces_current <= 10:
+                exit_reason = 1 if num_faces_current <= self.target_faces else 2
+                break
+            if num_faces_current == num_faces_prev:
+                count += 1
+            else:
+                count = 0
+                is_stuck = 0
+            if count >= 2:
+                is_stuck = 1
+            if count == tolerance:
+                if self.verbose:
+                    print("Not enough edges available to be collapsed.")
+                exit_reason = 3
+                break
+        return verts, faces, dict(rounds=rounds, exit_reason=exit_reason)
+
+
+class PaSP(torch.nn.Module):
+    """Unguarded simplifier (simp_cuda/pamo/__init__.py:181-227)."""
+
+    def __init__(self):
+        super().__init__()
+        raise NotImplementedError("PaSP (unguarded variant) is not built yet; use PaMO")

@@ -1,0 +1,113 @@
+"""ctypes binding of libpamo_hip.so (the C ABI declared in include/pamo_hip.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (or
+``python -m pamo_amd.build``).  Loading fails loudly when it is missing:
+there is no CPU fallback anywhere in the package.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpamo_hip.so")
+
+SHUFFLE_IDENTITY = 0
+SHUFFLE_FEISTEL = 1
+
+# kernel ids of pamo_profile (include/pamo_hip.h)
+K_QUADRIC, K_EDGE_COST, K_CLAIM, K_LBVH, K_TRAVERSE = range(5)
+KERNEL_NAMES = {K_QUADRIC: "k_quadric", K_EDGE_COST: "k_edge_cost", K_CLAIM: "k_claim", K_LBVH: "lbvh_build",
+                K_TRAVERSE: "k_traverse"}
+
+_lib = None
+
+
+class PamoError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded library (raises PamoError when it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PamoError("libpamo_hip.so not found at %s: build it with __graft_entry__.build() "
+                        "(hipcc --offload-arch=gfx950)" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    I = ctypes.c_int
+    LL = ctypes.c_longlong
+    F = ctypes.c_float
+    L.pamo_ctx_create.argtypes = [ctypes.POINTER(P), I, I]
+    L.pamo_ctx_create.restype = I
+    L.pamo_ctx_destroy.argtypes = [P]
+    L.pamo_ctx_destroy.restype = None
+    L.pamo_last_error.argtypes = [P]
+    L.pamo_last_error.restype = ctypes.c_char_p
+    L.pamo_free_forward.argtypes = [P, P, P, P, P, I, I, I, F, F, I, I]
+    L.pamo_free_forward.restype = I
+    L.pamo_free_outputs.argtypes = [P] + [P] * 5 + [ctypes.POINTER(I)] * 3
+    L.pamo_free_outputs.restype = I
+    L.pamo_free_copy_outputs.argtypes = [P, P, P, P, P, P, P]
+    L.pamo_free_copy_outputs.restype = I
+    L.pamo_run_stage2.argtypes = [P, P, P, P, I, I, F, LL, F, I, LL, ctypes.POINTER(I), ctypes.POINTER(I), P]
+    L.pamo_run_stage2.restype = I
+    L.pamo_run_rounds.argtypes = [P, P, I]
+    L.pamo_run_rounds.restype = I
+    L.pamo_run_fetch.argtypes = [P, P, P, P]
+    L.pamo_run_fetch.restype = I
+    L.pamo_debug_copy.argtypes = [P, P, I, P, LL]
+    L.pamo_debug_copy.restype = LL
+    L.pamo_debug_pairs.argtypes = [P, P, P, P, I, I, P, LL]
+    L.pamo_debug_pairs.restype = LL
+    L.pamo_profile.argtypes = [P, ctypes.c_uint]
+    L.pamo_profile.restype = I
+    L.pamo_profile_read.argtypes = [P, I, P]
+    L.pamo_profile_read.restype = I
+    _lib = L
+    return L
+
+
+class Context:
+    """Owns one ``pamo_ctx`` (one simplifier's device state)."""
+
+    def __init__(self, device=0, shuffle=SHUFFLE_FEISTEL):
+        self._L = lib()
+        h = ctypes.c_void_p()
+        rc = self._L.pamo_ctx_create(ctypes.byref(h), int(device), int(shuffle))
+        if rc != 0:
+            raise PamoError("pamo_ctx_create failed (%d): no usable GPU %d" % (rc, device))
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._L.pamo_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def error(self):
+        msg = self._L.pamo_last_error(self.handle)
+        return msg.decode() if msg else ""
+
+    def profile(self, kernels):
+        mask = 0
+        for k in kernels:
+            mask |= 1 << k
+        self.check(self._L.pamo_profile(self.handle, mask), "pamo_profile")
+
+    def profile_read(self, kernel):
+        import numpy as np
+        out = np.zeros(3, np.float64)
+        self.check(self._L.pamo_profile_read(self.handle, int(kernel), out.ctypes.data), "pamo_profile_read")
+        return dict(launches=int(out[0]), ms=float(out[1]), bytes=float(out[2]))
+
+    def check(self, rc, what):
+        if rc < 0:
+            raise PamoError("%s failed (%d): %s" % (what, rc, self.error()))
+        return rc

@@ -1,0 +1,766 @@
+// Host orchestration and C ABI of libpamo_hip.so (see include/pamo_hip.h).
+//
+// One context = one simplifier on one GPU.  Its persistent state mirrors
+// the reference's CUSimp_Free (simp_cuda/src/cusimp_free.h:184-280): the
+// vertex map, the invalid-vertex table and the edge costs survive between
+// rounds and are (re)allocated by the same n + n/5 growth rule, zero-filled
+// on (re)allocation.  Everything else is round scratch, sized once for the
+// largest mesh and reused.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pamo_hip.h"
+#include "pamo_bvh.h"
+#include "pamo_kernels.h"
+
+using namespace pamo;
+
+namespace {
+
+struct Buf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+enum Counter {
+  C_E = 0, C_NFAN, C_NWIN, C_NCOLL, C_NLINE, C_CUT, C_NALIVE, C_NQUERY, C_NMARK, C_ERR, C_NEWV, C_NEWF,
+  C_COUNT = 16
+};
+
+inline unsigned nblk(long long n) { return (unsigned)std::max<long long>(1, (n + BLOCK - 1) / BLOCK); }
+
+// cusimp_free.cu:923: u32(clamp(tau, 0, 10) / 10 * UINT32_MAX), clamp as
+// cusimp_free.cu:29-34 (NaN -> upper bound), saturating conversion
+inline uint32_t collapse_threshold(float tau) {
+  float m = 0.0f > tau ? 0.0f : tau;
+  float t = m < COST_RANGE ? m : COST_RANGE;
+  float x = t / COST_RANGE * U32F;
+  if (!(x > 0.0f)) return 0u;
+  if (x >= U32F) return 0xFFFFFFFFu;
+  return (uint32_t)x;
+}
+
+}  // namespace
+
+struct pamo_ctx {
+  int device = 0;
+  int shuffle_mode = PAMO_SHUFFLE_FEISTEL;
+  hipStream_t st = nullptr;
+  std::string err;
+
+  // persistent state with the reference's growth rule
+  size_t alloc_pts = 0, alloc_edge_cost = 0;
+  Buf pts_map, table, edge_cost, undo_list, tmp_undo;
+  size_t undo_cap = 0, tmp_cap = 0;   // ints
+
+  // round buffers
+  Buf P, P0, occ, T, T0, cnt, fan_off, fan, ecnt, eoff, edges, Q, tri_min, win, winpos, collapsed;
+  Buf qflag, qf, qpos, af, apos, qlist, leaf_face, lbox_lo, lbox_hi, keys0, keys1, vals0, vals1, hist;
+  Buf kleft, kright, kparent, rfirst, rlast, flags, nodeA, nodeB, S, S_first, mflag, mpos, marked;
+  Buf scan_sums, counters, cstats, shuf_flags, shuf_pos;
+  // run loop
+  Buf RV[2], RF[2], keep, kpos, undo_in;
+  int run_cur = 0, run_V = 0, run_F = 0;
+  std::vector<int> run_faces;   // faces after each round of the last run
+
+  int* hc = nullptr;   // pinned host mirror of the counters
+
+  // optional per-kernel timing with HIP events (mask of PAMO_K_* bits)
+  unsigned prof_mask = 0;
+  struct ProfRec { int k; hipEvent_t a, b; double bytes; };
+  std::vector<ProfRec> prof_open;
+  std::vector<hipEvent_t> ev_pool;
+  double prof_ms[32] = {0}, prof_bytes[32] = {0};
+  long long prof_n[32] = {0};
+  hipEvent_t take_event() {
+    if (!ev_pool.empty()) { hipEvent_t e = ev_pool.back(); ev_pool.pop_back(); return e; }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+  int prof_begin(int k, double bytes) {
+    if (!(prof_mask & (1u << k))) return -1;
+    ProfRec r{k, take_event(), take_event(), bytes};
+    (void)hipEventRecord(r.a, st);
+    prof_open.push_back(r);
+    return (int)prof_open.size() - 1;
+  }
+  void prof_end(int h) {
+    if (h < 0) return;
+    (void)hipEventRecord(prof_open[h].b, st);
+  }
+  void prof_resolve() {
+    for (auto& r : prof_open) {
+      float ms = 0;
+      (void)hipEventSynchronize(r.b);
+      (void)hipEventElapsedTime(&ms, r.a, r.b);
+      prof_ms[r.k] += ms;
+      prof_bytes[r.k] += r.bytes;
+      prof_n[r.k] += 1;
+      ev_pool.push_back(r.a);
+      ev_pool.push_back(r.b);
+    }
+    prof_open.clear();
+  }
+
+  // last forward
+  int V = 0, F = 0, E = 0, n_fan = 0, n_vertices_undo = 0, n_winners = 0, n_collapsed = 0, cut = 0;
+  int undo_passes = 0, n_checks = 0;
+  uint32_t collapse_t_ = 0;
+  long long n_queries = 0, n_cand = 0, n_hitfaces = 0;
+  bool have_s_first = false;
+
+  bool fail(const char* fmt, ...) {
+    char b[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(b, sizeof(b), fmt, ap);
+    va_end(ap);
+    err = b;
+    return false;
+  }
+  bool hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return true;
+    return fail("%s: %s", what, hipGetErrorString(e));
+  }
+  bool check_launch(const char* what) { return hip(hipGetLastError(), what); }
+
+  // grow without preserving contents
+  bool ensure(Buf& b, size_t bytes) {
+    if (bytes <= b.bytes && b.p) return true;
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    size_t nb = std::max<size_t>(bytes + bytes / 8, 256);
+    if (!hip(hipMalloc(&b.p, nb), "hipMalloc")) return false;
+    b.bytes = nb;
+    return true;
+  }
+  // grow preserving contents (stream-ordered copy)
+  bool ensure_keep(Buf& b, size_t bytes) {
+    if (bytes <= b.bytes && b.p) return true;
+    Buf nbuf;
+    size_t nb = std::max<size_t>(bytes + bytes / 2, 256);
+    if (!hip(hipMalloc(&nbuf.p, nb), "hipMalloc")) return false;
+    nbuf.bytes = nb;
+    if (b.p) {
+      if (!hip(hipMemcpyAsync(nbuf.p, b.p, b.bytes, hipMemcpyDeviceToDevice, st), "copy")) return false;
+      if (!hip(hipStreamSynchronize(st), "sync")) return false;
+      (void)hipFree(b.p);
+    }
+    b = nbuf;
+    return true;
+  }
+  template <class T> T* ptr(Buf& b) { return (T*)b.p; }
+  int* cnt_dev(int i) { return ptr<int>(counters) + i; }
+
+  bool fill_i32(void* p, int v, long long n) {
+    if (n <= 0) return true;
+    k_fill_i32<<<nblk(n), BLOCK, 0, st>>>((int*)p, v, n);
+    return check_launch("fill");
+  }
+  bool fill_u8(void* p, uint8_t v, long long n) {
+    if (n <= 0) return true;
+    k_fill_u8<<<nblk(n), BLOCK, 0, st>>>((uint8_t*)p, v, n);
+    return check_launch("fill8");
+  }
+  bool set_counter(int i, int v) {
+    k_fill_i32<<<1, 1, 0, st>>>(cnt_dev(i), v, 1);
+    return check_launch("set counter");
+  }
+  // exclusive scan of n ints; in-place allowed; total -> counter slot (or -1)
+  bool scan(const int* in, int* out, long long n, int total_slot) {
+    if (n <= 0) {
+      if (total_slot >= 0) return set_counter(total_slot, 0);
+      return true;
+    }
+    long long nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (!ensure(scan_sums, (size_t)nb * 4)) return false;
+    k_scan_reduce<<<(unsigned)nb, BLOCK, 0, st>>>(in, n, ptr<int>(scan_sums));
+    k_scan_top<<<1, BLOCK, 0, st>>>(ptr<int>(scan_sums), (int)nb, total_slot >= 0 ? cnt_dev(total_slot) : nullptr);
+    k_scan_down<<<(unsigned)nb, BLOCK, 0, st>>>(in, out, n, ptr<int>(scan_sums));
+    return check_launch("scan");
+  }
+  bool read_counters() {
+    if (!hip(hipMemcpyAsync(hc, counters.p, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, st), "read counters"))
+      return false;
+    return hip(hipStreamSynchronize(st), "sync");
+  }
+
+  bool ensure_pts_storage(size_t n) {   // cusimp_free.cu:47-85 growth rule
+    if (n <= alloc_pts) return true;
+    alloc_pts = n + n / 5;
+    if (!ensure(pts_map, (alloc_pts + 1) * 4) || !ensure(table, alloc_pts + 1)) return false;
+    if (!fill_i32(pts_map.p, 0, (long long)alloc_pts + 1) || !fill_u8(table.p, 0, (long long)alloc_pts + 1))
+      return false;
+    // the undo lists are reallocated too: their contents are lost (D1)
+    if (!ensure_keep(undo_list, (alloc_pts + 1) * 4) || !ensure_keep(tmp_undo, (alloc_pts + 1) * 4)) return false;
+    if (!fill_i32(undo_list.p, 0, (long long)(undo_list.bytes / 4)) || !fill_i32(tmp_undo.p, 0, (long long)(tmp_undo.bytes / 4)))
+      return false;
+    return true;
+  }
+  bool ensure_edge_cost_storage(size_t n) {   // cusimp_free.cu:180-194
+    if (n <= alloc_edge_cost) return true;
+    alloc_edge_cost = n + n / 5;
+    if (!ensure(edge_cost, (alloc_edge_cost + 1) * 4)) return false;
+    return fill_i32(edge_cost.p, 0, (long long)alloc_edge_cost + 1);
+  }
+
+  bool build_bvh_and_traverse(int Fn, bool debug_pairs, std::vector<int>* pairs_out);
+  bool check_and_mark(int* n_marked);
+  int forward(const float* d_pts, const int* d_tris, const int* d_undo, int n_u, int nV, int nF, float scale,
+              float thr, int is_stuck, int init);
+  ~pamo_ctx();
+};
+
+pamo_ctx::~pamo_ctx() {
+  (void)hipSetDevice(device);
+  (void)hipDeviceSynchronize();
+  Buf* all[] = {&pts_map, &table, &edge_cost, &undo_list, &tmp_undo, &P, &P0, &occ, &T, &T0, &cnt, &fan_off, &fan,
+                &ecnt, &eoff, &edges, &Q, &tri_min, &win, &winpos, &collapsed, &qflag, &qf, &qpos, &af, &apos,
+                &qlist, &leaf_face, &lbox_lo, &lbox_hi, &keys0, &keys1, &vals0, &vals1, &hist, &kleft, &kright,
+                &kparent, &rfirst, &rlast, &flags, &nodeA, &nodeB, &S, &S_first, &mflag, &mpos, &marked,
+                &scan_sums, &counters, &cstats, &shuf_flags, &shuf_pos, &RV[0], &RV[1], &RF[0], &RF[1], &keep,
+                &kpos, &undo_in};
+  for (Buf* b : all)
+    if (b->p) (void)hipFree(b->p);
+  for (auto& r : prof_open) { ev_pool.push_back(r.a); ev_pool.push_back(r.b); }
+  for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+  if (hc) (void)hipHostFree(hc);
+}
+
+// BVH over the alive faces, then the restricted traversal (query list in
+// qlist/C_NQUERY) or, for the debug export, the full pair list.
+bool pamo_ctx::build_bvh_and_traverse(int Fn, bool debug_pairs, std::vector<int>* pairs_out) {
+  const long long Fl = std::max(Fn, 1);
+  if (!ensure(lbox_lo, Fl * 16) || !ensure(lbox_hi, Fl * 16) || !ensure(keys0, Fl * 4) || !ensure(keys1, Fl * 4) ||
+      !ensure(vals0, Fl * 4) || !ensure(vals1, Fl * 4) || !ensure(kleft, Fl * 4) || !ensure(kright, Fl * 4) ||
+      !ensure(kparent, 2 * Fl * 4) || !ensure(rfirst, Fl * 4) || !ensure(rlast, Fl * 4) || !ensure(flags, Fl * 4) ||
+      !ensure(nodeA, 2 * Fl * 16) || !ensure(nodeB, 2 * Fl * 16))
+    return false;
+  int* n_alive = cnt_dev(C_NALIVE);
+  int* scene = ptr<int>(cstats) + 8;
+  // LBVH build bytes: alive faces counted as F (the alive count is on the device)
+  int pbv = prof_begin(PAMO_K_LBVH, 12.0 * Fn + 12.0 * V + 8.0 * Fn + 8.0 * Fn + 8.0 * Fn + (2.0 * Fn - 1) * 16 +
+                                       (2.0 * Fn - 1) * 24);
+  k_scene_init<<<1, 64, 0, st>>>(scene);
+  k_leaf_boxes<<<nblk(Fl), BLOCK, 0, st>>>(ptr<int>(leaf_face), n_alive, ptr<float>(P), ptr<int>(T),
+                                          ptr<float4>(lbox_lo), ptr<float4>(lbox_hi), scene);
+  k_morton<<<nblk(Fl), BLOCK, 0, st>>>(ptr<float4>(lbox_lo), ptr<float4>(lbox_hi), n_alive, scene,
+                                      ptr<uint32_t>(keys0), ptr<int>(vals0));
+  if (!check_launch("leaf boxes")) return false;
+  // radix sort: 4 passes of 8 bits over 30-bit Morton codes
+  long long ntiles = (Fl + RS_TILE - 1) / RS_TILE;
+  if (!ensure(hist, (size_t)ntiles * 256 * 4)) return false;
+  uint32_t* kin = ptr<uint32_t>(keys0);
+  uint32_t* kout = ptr<uint32_t>(keys1);
+  int* vin = ptr<int>(vals0);
+  int* vout = ptr<int>(vals1);
+  for (int shift = 0; shift < 32; shift += 8) {
+    k_rs_hist<<<(unsigned)ntiles, BLOCK, 0, st>>>(kin, n_alive, shift, (int)ntiles, ptr<int>(hist));
+    if (!scan(ptr<int>(hist), ptr<int>(hist), ntiles * 256, -1)) return false;
+    k_rs_scatter<<<(unsigned)ntiles, BLOCK, 0, st>>>(kin, vin, n_alive, shift, (int)ntiles, ptr<int>(hist), kout,
+                                                     vout);
+    if (!check_launch("radix sort")) return false;
+    std::swap(kin, kout);
+    std::swap(vin, vout);
+  }
+  if (!fill_i32(flags.p, 0, Fl)) return false;
+  k_karras<<<nblk(Fl), BLOCK, 0, st>>>(kin, n_alive, ptr<int>(kleft), ptr<int>(kright), ptr<int>(kparent),
+                                      ptr<int>(rfirst), ptr<int>(rlast));
+  k_bottom_up<<<nblk(Fl), BLOCK, 0, st>>>(n_alive, ptr<float4>(lbox_lo), ptr<float4>(lbox_hi), vin,
+                                         ptr<int>(leaf_face), ptr<int>(kparent), ptr<int>(kleft), ptr<int>(kright),
+                                         ptr<int>(rfirst), ptr<int>(rlast), ptr<int>(flags), ptr<float4>(nodeA),
+                                         ptr<float4>(nodeB));
+  prof_end(pbv);
+  if (!check_launch("lbvh")) return false;
+  if (!debug_pairs) {
+    int pt = prof_begin(PAMO_K_TRAVERSE, 0.0);
+    if (!fill_u8(S.p, 0, Fl)) return false;
+    k_traverse<<<nblk(Fl), BLOCK, 0, st>>>(ptr<int>(qlist), cnt_dev(C_NQUERY), n_alive, ptr<float4>(nodeA),
+                                          ptr<float4>(nodeB), ptr<float>(P), ptr<int>(T), ptr<uint8_t>(S),
+                                          (unsigned long long*)cstats.p, cnt_dev(C_ERR));
+    prof_end(pt);
+    return check_launch("traverse");
+  }
+  // debug: count, scan, emit
+  if (!ensure(qpos, (Fl + 1) * 4) || !ensure(mflag, (Fl + 1) * 4)) return false;
+  if (!fill_i32(mflag.p, 0, Fl + 1)) return false;
+  k_traverse_list<<<nblk(Fl), BLOCK, 0, st>>>(ptr<int>(qlist), cnt_dev(C_NQUERY), ptr<float4>(nodeA),
+                                             ptr<float4>(nodeB), ptr<float>(P), ptr<int>(T), ptr<int>(mflag), nullptr,
+                                             nullptr);
+  if (!scan(ptr<int>(mflag), ptr<int>(qpos), Fl + 1, C_NMARK)) return false;
+  if (!read_counters()) return false;
+  long long np = hc[C_NMARK];
+  Buf pb;
+  if (!hip(hipMalloc(&pb.p, std::max<long long>(np, 1) * 12), "hipMalloc pairs")) return false;
+  k_traverse_list<<<nblk(Fl), BLOCK, 0, st>>>(ptr<int>(qlist), cnt_dev(C_NQUERY), ptr<float4>(nodeA),
+                                             ptr<float4>(nodeB), ptr<float>(P), ptr<int>(T), nullptr, ptr<int>(qpos),
+                                             (int*)pb.p);
+  pairs_out->resize(3 * np);
+  bool ok = check_launch("pairs") &&
+            hip(hipMemcpyAsync(pairs_out->data(), pb.p, np * 12, hipMemcpyDeviceToHost, st), "copy pairs") &&
+            hip(hipStreamSynchronize(st), "sync");
+  (void)hipFree(pb.p);
+  return ok;
+}
+
+// One self-intersection check restricted to the fans of the first `cut`
+// collapsed edges, then the undo marks (cusimp_free.cu:1014-1022).
+bool pamo_ctx::check_and_mark(int* n_marked) {
+  n_checks++;
+  const long long Fl = std::max(F, 1);
+  if (!ensure(qflag, Fl) || !ensure(qf, Fl * 4) || !ensure(qpos, (Fl + 1) * 4) || !ensure(af, Fl * 4) ||
+      !ensure(apos, Fl * 4) || !ensure(qlist, Fl * 4) || !ensure(leaf_face, Fl * 4) || !ensure(S, Fl))
+    return false;
+  if (!fill_u8(qflag.p, 0, Fl)) return false;
+  k_qflag_mark<<<nblk(cut), BLOCK, 0, st>>>(ptr<int>(collapsed), cnt_dev(C_CUT), ptr<int>(edges), ptr<int>(fan_off),
+                                           ptr<int>(fan), ptr<uint8_t>(qflag));
+  k_query_flags<<<nblk(F), BLOCK, 0, st>>>(ptr<uint8_t>(qflag), ptr<int>(T), F, ptr<int>(qf), ptr<int>(af));
+  if (!check_launch("query flags")) return false;
+  if (!scan(ptr<int>(qf), ptr<int>(qpos), F, C_NQUERY)) return false;
+  k_compact_idx<<<nblk(F), BLOCK, 0, st>>>(ptr<int>(qf), ptr<int>(qpos), F, ptr<int>(qlist), 0);
+  if (!scan(ptr<int>(af), ptr<int>(apos), F, C_NALIVE)) return false;
+  k_compact_idx<<<nblk(F), BLOCK, 0, st>>>(ptr<int>(af), ptr<int>(apos), F, ptr<int>(leaf_face), 0);
+  if (!check_launch("compact")) return false;
+  if (!build_bvh_and_traverse(F, false, nullptr)) return false;
+  // marks
+  if (!ensure(mflag, ((size_t)cut + 1) * 4) || !ensure(mpos, ((size_t)cut + 1) * 4) ||
+      !ensure(marked, ((size_t)cut + 1) * 4) || !ensure_keep(tmp_undo, ((size_t)2 * cut + 2) * 4))
+    return false;
+  k_mark_undo<<<nblk(cut), BLOCK, 0, st>>>(ptr<int>(collapsed), cnt_dev(C_CUT), ptr<int>(edges), ptr<int>(fan_off),
+                                          ptr<int>(fan), ptr<uint8_t>(S), ptr<int>(mflag));
+  if (!scan(ptr<int>(mflag), ptr<int>(mpos), cut, C_NMARK)) return false;
+  k_marked_pairs<<<nblk(cut), BLOCK, 0, st>>>(ptr<int>(mflag), ptr<int>(mpos), cnt_dev(C_CUT), ptr<int>(collapsed),
+                                             ptr<int>(edges), ptr<int>(marked), ptr<int>(tmp_undo));
+  if (!check_launch("marks")) return false;
+  if (!read_counters()) return false;
+  if (hc[C_ERR]) return fail("malformed LBVH (%d walks exceeded the node count)", hc[C_ERR]);
+  *n_marked = hc[C_NMARK];
+  n_queries += hc[C_NQUERY];
+  return true;
+}
+
+int pamo_ctx::forward(const float* d_pts, const int* d_tris, const int* d_undo, int n_u, int nV, int nF, float sc,
+                      float thr, int is_stuck, int init) {
+  err.clear();
+  if (nV < 0 || nF < 0 || n_u < 0) { fail("negative size"); return -1; }
+  if (!hip(hipSetDevice(device), "hipSetDevice")) return -1;
+  if (!ensure(counters, C_COUNT * 4) || !ensure(cstats, 64 * 4)) return -1;
+  if (!fill_i32(counters.p, 0, C_COUNT) || !fill_i32(cstats.p, 0, 8)) return -1;
+  n_checks = 0; undo_passes = 0; n_queries = 0; have_s_first = false;
+
+  // step 1 (cusimp_free.cu:895-916) on the storage as it stands
+  n_vertices_undo = 0;
+  int first = 0;
+  if (is_stuck) {
+    if (n_u > 0) {
+      if (!ensure_keep(tmp_undo, ((size_t)n_u + 1) * 4) || !ensure_keep(undo_list, ((size_t)n_u + 1) * 4)) return -1;
+      if (!hip(hipMemcpyAsync(tmp_undo.p, d_undo, (size_t)n_u * 4, hipMemcpyDeviceToDevice, st), "copy undo"))
+        return -1;
+      long long map_n = alloc_pts ? (long long)alloc_pts + 1 : 0;
+      k_remap_undo<<<nblk(n_u), BLOCK, 0, st>>>(ptr<int>(tmp_undo), ptr<int>(pts_map), map_n, ptr<int>(undo_list), 0,
+                                               n_u, cnt_dev(C_ERR));
+      if (!check_launch("remap undo")) return -1;
+    }
+    n_vertices_undo += n_u;
+    first += n_u;
+  } else if (n_u != 0 && alloc_pts) {
+    if (!fill_u8(table.p, 0, (long long)alloc_pts + 1)) return -1;
+  }
+  if (n_u != 0) {
+    long long tn = alloc_pts ? (long long)alloc_pts + 1 : 0;
+    k_set_table<<<nblk(n_u), BLOCK, 0, st>>>(d_undo, n_u, ptr<uint8_t>(table), tn, cnt_dev(C_ERR));
+    if (!check_launch("table")) return -1;
+  }
+  const float tau = thr;
+  collapse_t_ = collapse_threshold(thr);
+  V = nV;
+  F = nF;
+  if (!ensure_pts_storage(V)) return -1;
+  const long long Vl = std::max(V, 1), Fl = std::max(F, 1);
+  if (!ensure(P, Vl * 12) || !ensure(P0, Vl * 12) || !ensure(occ, (Vl + 1) * 4) || !ensure(T, Fl * 12) ||
+      !ensure(T0, Fl * 12))
+    return -1;
+  k_validate<<<nblk(F), BLOCK, 0, st>>>(d_tris, F, V, cnt_dev(C_ERR));
+  k_init_points<<<nblk(V + 1), BLOCK, 0, st>>>(d_pts, ptr<float>(P), ptr<float>(P0), ptr<int>(occ), V);
+  if (!check_launch("init points")) return -1;
+  if (init && shuffle_mode == PAMO_SHUFFLE_FEISTEL && F > 0) {
+    Feistel fb;
+    uint64_t m = (uint64_t)F, total;
+    if (m <= 16) total = 4;
+    else { total = 0; uint64_t t = m - 1; while (t) { total++; t >>= 1; } }
+    fb.lbits = (uint32_t)(total / 2);
+    fb.lmask = (uint32_t)((1ull << fb.lbits) - 1);
+    fb.rbits = (uint32_t)(total - fb.lbits);
+    fb.rmask = (uint32_t)((1ull << fb.rbits) - 1);
+    uint64_t x = 1;
+    for (int i = 0; i < 24; i++) { x = (x * 48271ull) % 2147483647ull; fb.key[i] = (uint32_t)x; }
+    long long n = 1LL << (fb.lbits + fb.rbits);
+    if (!ensure(shuf_flags, n * 4) || !ensure(shuf_pos, n * 4)) return -1;
+    k_shuffle_flags<<<nblk(n), BLOCK, 0, st>>>(fb, n, F, ptr<int>(shuf_flags));
+    if (!scan(ptr<int>(shuf_flags), ptr<int>(shuf_pos), n, -1)) return -1;
+    k_shuffle_scatter<<<nblk(n), BLOCK, 0, st>>>(fb, n, F, ptr<int>(shuf_pos), d_tris, ptr<int>(T), ptr<int>(T0));
+  } else {
+    k_init_tris<<<nblk(3LL * F), BLOCK, 0, st>>>(d_tris, ptr<int>(T), ptr<int>(T0), F);
+  }
+  if (!check_launch("init tris")) return -1;
+
+  // fans and edges (cusimp_free.cu:950-978)
+  if (!ensure(cnt, (Vl + 1) * 4) || !ensure(fan_off, (Vl + 1) * 4) || !ensure(ecnt, (Fl + 1) * 4) ||
+      !ensure(eoff, (Fl + 1) * 4))
+    return -1;
+  if (!fill_i32(cnt.p, 0, V + 1)) return -1;
+  k_fan_count<<<nblk(F), BLOCK, 0, st>>>(ptr<int>(T), F, ptr<int>(cnt));
+  if (!scan(ptr<int>(cnt), ptr<int>(fan_off), V + 1, C_NFAN)) return -1;
+  k_edge_count<<<nblk(F + 1), BLOCK, 0, st>>>(ptr<int>(T), F, ptr<int>(ecnt));
+  if (!scan(ptr<int>(ecnt), ptr<int>(eoff), F + 1, C_E)) return -1;
+  if (!read_counters()) return -1;
+  if (hc[C_ERR]) { fail("invalid input: %d face or undo indices out of range", hc[C_ERR]); return -2; }
+  E = hc[C_E];
+  n_fan = hc[C_NFAN];
+  const long long El = std::max(E, 1);
+  if (!ensure(fan, (size_t)std::max(n_fan, 1) * 4) || !ensure(edges, El * 8) || !ensure(Q, Vl * 40) ||
+      !ensure(tri_min, Fl * 8) || !ensure(win, (El + 1) * 4) || !ensure(winpos, (El + 1) * 4) ||
+      !ensure(collapsed, (2 * El + 1) * 4))
+    return -1;
+  if (!fill_i32(cnt.p, 0, V + 1)) return -1;
+  k_fan_fill<<<nblk(F), BLOCK, 0, st>>>(ptr<int>(T), F, ptr<int>(fan_off), ptr<int>(cnt), ptr<int>(fan));
+  k_fan_sort<<<nblk(V), BLOCK, 0, st>>>(ptr<int>(fan_off), ptr<int>(fan), V);
+  k_edge_emit<<<nblk(F), BLOCK, 0, st>>>(ptr<int>(T), F, ptr<int>(eoff), ptr<int>(edges));
+  // quadrics, costs, claims, collapse (cusimp_free.cu:980-999)
+  int ph = prof_begin(PAMO_K_QUADRIC, 12.0 * F + 12.0 * V + 4.0 * (V + 1) + 12.0 * F + 40.0 * V);
+  k_quadric<<<nblk(V), BLOCK, 0, st>>>(ptr<float>(P), ptr<int>(T), ptr<int>(fan_off), ptr<int>(fan), V, ptr<float>(Q));
+  prof_end(ph);
+  if (!check_launch("fans/edges/quadrics")) return -1;
+  if (!ensure_edge_cost_storage(E)) return -1;
+  ph = prof_begin(PAMO_K_EDGE_COST, 8.0 * E + 12.0 * V + 40.0 * V + 4.0 * (V + 1) + 12.0 * F + 12.0 * F + 1.0 * V +
+                                        4.0 * E);
+  k_edge_cost<<<nblk(E), BLOCK, 0, st>>>(ptr<float>(P), ptr<int>(T), ptr<int>(fan_off), ptr<int>(fan), ptr<int>(edges),
+                                        E, ptr<float>(Q), ptr<uint8_t>(table), ptr<uint32_t>(edge_cost), sc, tau,
+                                        is_stuck);
+  prof_end(ph);
+  k_fill_u64<<<nblk(F), BLOCK, 0, st>>>(ptr<u64>(tri_min), ~0ull, F);
+  ph = prof_begin(PAMO_K_CLAIM, 4.0 * E + 8.0 * E + 4.0 * (V + 1) + 12.0 * F + 8.0 * F);
+  k_claim<<<nblk(E), BLOCK, 0, st>>>(ptr<int>(edges), E, ptr<uint32_t>(edge_cost), ptr<int>(fan_off), ptr<int>(fan),
+                                    ptr<u64>(tri_min));
+  prof_end(ph);
+  k_collapse<<<nblk(E), BLOCK, 0, st>>>(ptr<int>(edges), E, ptr<uint32_t>(edge_cost), collapse_t_, ptr<int>(fan_off),
+                                       ptr<int>(fan), ptr<u64>(tri_min), ptr<float>(P), ptr<float>(P0), ptr<int>(T),
+                                       ptr<int>(T0), ptr<int>(occ), ptr<int>(win));
+  if (!check_launch("cost/claim/collapse")) return -1;
+  if (!scan(ptr<int>(win), ptr<int>(winpos), E, C_NWIN)) return -1;
+  k_compact_idx<<<nblk(E), BLOCK, 0, st>>>(ptr<int>(win), ptr<int>(winpos), E, ptr<int>(collapsed), 0);
+  k_remove_invalid_faces<<<nblk(F), BLOCK, 0, st>>>(ptr<int>(T), F);
+  k_line_detect<<<nblk(E), BLOCK, 0, st>>>(ptr<float>(P), ptr<int>(edges), E, cnt_dev(C_NLINE));
+  if (!check_launch("collapse list")) return -1;
+  if (!read_counters()) return -1;
+  n_winners = hc[C_NWIN];
+  n_collapsed = n_winners;
+  if (hc[C_NLINE] > 0) {
+    // the sequential pass may append up to E entries
+    if (!set_counter(C_NCOLL, n_winners)) return -1;
+    k_line_apply_seq<<<1, 1, 0, st>>>(ptr<float>(P), ptr<int>(T), ptr<int>(occ), ptr<int>(edges), E, ptr<int>(fan_off),
+                                     ptr<int>(fan), ptr<int>(collapsed), cnt_dev(C_NCOLL));
+    if (!check_launch("line edges") || !read_counters()) return -1;
+    n_collapsed = hc[C_NCOLL];
+  }
+  cut = n_winners == 0 ? 0 : std::min(n_collapsed, ((n_winners + 511) / 512) * 512);
+  if (!set_counter(C_CUT, cut)) return -1;
+
+  // self-intersection guard and undo loop (cusimp_free.cu:1014-1040)
+  int hm = 0;
+  if (cut > 0) {
+    if (!check_and_mark(&hm)) return -1;
+    if (!ensure(S_first, Fl) ||
+        !hip(hipMemcpyAsync(S_first.p, S.p, F, hipMemcpyDeviceToDevice, st), "copy S"))
+      return -1;
+    have_s_first = true;
+  }
+  n_vertices_undo += 2 * hm;
+  int it = 0;
+  while (hm != 0) {
+    it++;
+    k_fill_i32<<<1, 1, 0, st>>>(cnt_dev(C_NMARK), hm, 1);
+    k_undo<<<nblk(hm), BLOCK, 0, st>>>(ptr<int>(marked), cnt_dev(C_NMARK), ptr<int>(edges), ptr<float>(P),
+                                      ptr<float>(P0), ptr<int>(T), ptr<int>(T0), ptr<int>(occ), ptr<int>(fan_off),
+                                      ptr<int>(fan));
+    if (!ensure_keep(undo_list, ((size_t)n_vertices_undo + 1) * 4)) return -1;
+    long long map_n = (long long)alloc_pts + 1;
+    k_remap_undo<<<nblk(n_vertices_undo - first), BLOCK, 0, st>>>(ptr<int>(tmp_undo), ptr<int>(pts_map), map_n,
+                                                                  ptr<int>(undo_list), first, n_vertices_undo - first,
+                                                                  cnt_dev(C_ERR));
+    if (!check_launch("undo")) return -1;
+    first += 2 * hm;
+    if (!check_and_mark(&hm)) return -1;
+    n_vertices_undo += 2 * hm;
+    if (it == 5) break;
+  }
+  undo_passes = it;
+  if (!ensure_keep(undo_list, ((size_t)n_vertices_undo + 1) * 4)) return -1;
+  if (n_vertices_undo > first && !fill_i32(ptr<int>(undo_list) + first, 0, n_vertices_undo - first)) return -1;   // D6
+  // pts_map = exclusive scan of occupancy (cusimp_free.cu:1042-1045)
+  if (!scan(ptr<int>(occ), ptr<int>(pts_map), V + 1, C_NEWV)) return -1;
+  unsigned long long cs[2] = {0, 0};
+  if (!hip(hipMemcpyAsync(cs, cstats.p, 16, hipMemcpyDeviceToHost, st), "stats") || !hip(hipStreamSynchronize(st), "sync"))
+    return -1;
+  n_cand = (long long)cs[0];
+  n_hitfaces = (long long)cs[1];
+  return 0;
+}
+
+// ====================================================================== ABI
+extern "C" {
+
+int pamo_ctx_create(pamo_ctx** out, int device, int shuffle_mode) {
+  if (!out) return -1;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return -3;
+  if (device < 0 || device >= n) return -1;
+  if (hipSetDevice(device) != hipSuccess) return -3;
+  pamo_ctx* c = new pamo_ctx();
+  c->device = device;
+  c->shuffle_mode = shuffle_mode;
+  if (hipHostMalloc((void**)&c->hc, C_COUNT * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    delete c;
+    return -3;
+  }
+  *out = c;
+  return 0;
+}
+
+void pamo_ctx_destroy(pamo_ctx* ctx) { delete ctx; }
+
+const char* pamo_last_error(const pamo_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int pamo_free_forward(pamo_ctx* ctx, void* stream, const float* d_points, const int* d_tris, const int* d_undo,
+                      int n_undo, int n_pts, int n_tris, float scale, float threshold, int is_stuck, int init) {
+  if (!ctx) return -1;
+  ctx->st = (hipStream_t)stream;
+  return ctx->forward(d_points, d_tris, d_undo, n_undo, n_pts, n_tris, scale, threshold, is_stuck, init);
+}
+
+int pamo_free_outputs(pamo_ctx* ctx, float** d_points, int** d_tris, int** d_occ, int** d_map, int** d_undo,
+                      int* n_pts, int* n_tris, int* n_undo) {
+  if (!ctx) return -1;
+  if (d_points) *d_points = ctx->ptr<float>(ctx->P);
+  if (d_tris) *d_tris = ctx->ptr<int>(ctx->T);
+  if (d_occ) *d_occ = ctx->ptr<int>(ctx->occ);
+  if (d_map) *d_map = ctx->ptr<int>(ctx->pts_map);
+  if (d_undo) *d_undo = ctx->ptr<int>(ctx->undo_list);
+  if (n_pts) *n_pts = ctx->V;
+  if (n_tris) *n_tris = ctx->F;
+  if (n_undo) *n_undo = ctx->n_vertices_undo;
+  return 0;
+}
+
+int pamo_free_copy_outputs(pamo_ctx* ctx, void* stream, float* d_points, int* d_tris, int* d_occ, int* d_map,
+                           int* d_undo) {
+  if (!ctx) return -1;
+  pamo_ctx& c = *ctx;
+  c.st = (hipStream_t)stream;
+  struct { void* dst; void* src; size_t n; } cp[5] = {
+      {d_points, c.P.p, (size_t)c.V * 12}, {d_tris, c.T.p, (size_t)c.F * 12}, {d_occ, c.occ.p, (size_t)c.V * 4},
+      {d_map, c.pts_map.p, (size_t)c.V * 4}, {d_undo, c.undo_list.p, (size_t)c.n_vertices_undo * 4}};
+  for (auto& x : cp)
+    if (x.dst && x.n && !c.hip(hipMemcpyAsync(x.dst, x.src, x.n, hipMemcpyDeviceToDevice, c.st), "copy outputs"))
+      return -1;
+  return 0;
+}
+
+int pamo_run_stage2(pamo_ctx* ctx, void* stream, const float* d_verts, const int* d_faces, int n_verts, int n_faces,
+                    float scale, long long target, float threshold, int tolerance, long long iters, int* out_V,
+                    int* out_F, long long* stats) {
+  if (!ctx) return -1;
+  pamo_ctx& c = *ctx;
+  c.st = (hipStream_t)stream;
+  c.err.clear();
+  if (n_verts < 0 || n_faces < 0) { c.fail("negative size"); return -1; }
+  if (!c.hip(hipSetDevice(c.device), "hipSetDevice")) return -1;
+  for (int b = 0; b < 2; b++)
+    if (!c.ensure(c.RV[b], (size_t)std::max(n_verts, 1) * 12) || !c.ensure(c.RF[b], (size_t)std::max(n_faces, 1) * 12))
+      return -1;
+  if (!c.ensure(c.keep, ((size_t)n_faces + 1) * 4) || !c.ensure(c.kpos, ((size_t)n_faces + 1) * 4) ||
+      !c.ensure(c.undo_in, 256))
+    return -1;
+  int cur = 0;
+  if (n_verts && !c.hip(hipMemcpyAsync(c.RV[0].p, d_verts, (size_t)n_verts * 12, hipMemcpyDeviceToDevice, c.st), "copy in"))
+    return -1;
+  if (n_faces && !c.hip(hipMemcpyAsync(c.RF[0].p, d_faces, (size_t)n_faces * 12, hipMemcpyDeviceToDevice, c.st), "copy in"))
+    return -1;
+  int V = n_verts, F = n_faces, n_undo = 0;
+  int count = 0, is_stuck = 0;
+  bool init = true;
+  long long st[8] = {0, 4, 0, 0, 0, 0, 0, 0};
+  c.run_faces.clear();
+  for (long long it = 0; it < iters; it++) {
+    int nprev = F;
+    int rc = c.forward(c.ptr<float>(c.RV[cur]), c.ptr<int>(c.RF[cur]), c.ptr<int>(c.undo_in), n_undo, V, F, scale,
+                       threshold, is_stuck, init ? 1 : 0);
+    if (rc) return rc;
+    init = false;
+    st[0]++;
+    st[2] += c.n_winners;
+    st[3] = std::max<long long>(st[3], c.undo_passes);
+    st[4] += c.n_checks;
+    st[5] += c.n_queries;
+    st[6] += c.n_cand;
+    st[7] += c.n_hitfaces;
+    // the undo list feeds the next round
+    n_undo = c.n_vertices_undo;
+    if (!c.ensure(c.undo_in, ((size_t)n_undo + 1) * 4)) return -1;
+    if (n_undo && !c.hip(hipMemcpyAsync(c.undo_in.p, c.undo_list.p, (size_t)n_undo * 4, hipMemcpyDeviceToDevice, c.st),
+                         "copy undo"))
+      return -1;
+    // compaction (simp_cuda/pamo/__init__.py:136-140)
+    int nxt = cur ^ 1;
+    k_compact_verts<<<nblk(c.V), BLOCK, 0, c.st>>>(c.ptr<float>(c.P), c.ptr<int>(c.occ), c.ptr<int>(c.pts_map), c.V,
+                                                  c.ptr<float>(c.RV[nxt]));
+    k_face_keep<<<nblk(c.F), BLOCK, 0, c.st>>>(c.ptr<int>(c.T), c.F, c.ptr<int>(c.keep));
+    if (!c.check_launch("compact") || !c.scan(c.ptr<int>(c.keep), c.ptr<int>(c.kpos), c.F, C_NEWF)) return -1;
+    k_compact_faces<<<nblk(c.F), BLOCK, 0, c.st>>>(c.ptr<int>(c.T), c.ptr<int>(c.keep), c.ptr<int>(c.kpos),
+                                                  c.ptr<int>(c.pts_map), c.F, c.ptr<int>(c.RF[nxt]));
+    if (!c.check_launch("compact faces") || !c.read_counters()) return -1;
+    V = c.hc[C_NEWV];
+    F = c.hc[C_NEWF];
+    c.run_faces.push_back(F);
+    cur = nxt;
+    if (F <= target || F <= 10) { st[1] = F <= target ? 1 : 2; break; }
+    if (F == nprev) count++;
+    else { count = 0; is_stuck = 0; }
+    if (count >= 2) is_stuck = 1;
+    if (count == tolerance) { st[1] = 3; break; }
+  }
+  c.run_cur = cur;
+  c.run_V = V;
+  c.run_F = F;
+  if (out_V) *out_V = V;
+  if (out_F) *out_F = F;
+  if (stats) memcpy(stats, st, sizeof(st));
+  return 0;
+}
+
+int pamo_run_rounds(pamo_ctx* ctx, int* host_faces, int cap) {
+  if (!ctx) return -1;
+  int n = (int)ctx->run_faces.size();
+  if (host_faces)
+    for (int i = 0; i < n && i < cap; i++) host_faces[i] = ctx->run_faces[i];
+  return n;
+}
+
+int pamo_run_fetch(pamo_ctx* ctx, void* stream, float* d_verts_out, int* d_faces_out) {
+  if (!ctx) return -1;
+  pamo_ctx& c = *ctx;
+  c.st = (hipStream_t)stream;
+  if (c.run_V && d_verts_out &&
+      !c.hip(hipMemcpyAsync(d_verts_out, c.RV[c.run_cur].p, (size_t)c.run_V * 12, hipMemcpyDeviceToDevice, c.st), "fetch"))
+    return -1;
+  if (c.run_F && d_faces_out &&
+      !c.hip(hipMemcpyAsync(d_faces_out, c.RF[c.run_cur].p, (size_t)c.run_F * 12, hipMemcpyDeviceToDevice, c.st), "fetch"))
+    return -1;
+  return c.hip(hipStreamSynchronize(c.st), "sync") ? 0 : -1;
+}
+
+int pamo_profile(pamo_ctx* ctx, unsigned kernel_mask) {
+  if (!ctx) return -1;
+  ctx->prof_resolve();
+  ctx->prof_mask = kernel_mask;
+  for (int k = 0; k < 32; k++) { ctx->prof_ms[k] = 0; ctx->prof_bytes[k] = 0; ctx->prof_n[k] = 0; }
+  return 0;
+}
+
+int pamo_profile_read(pamo_ctx* ctx, int kernel, double out[3]) {
+  if (!ctx || kernel < 0 || kernel >= 32) return -1;
+  ctx->prof_resolve();
+  out[0] = (double)ctx->prof_n[kernel];
+  out[1] = ctx->prof_ms[kernel];
+  out[2] = ctx->prof_bytes[kernel];
+  return 0;
+}
+
+long long pamo_debug_copy(pamo_ctx* ctx, void* stream, int which, void* host_dst, long long cap_bytes) {
+  if (!ctx) return -1;
+  pamo_ctx& c = *ctx;
+  c.st = (hipStream_t)stream;
+  const void* src = nullptr;
+  long long n = 0;
+  switch (which) {
+    case 5: src = c.fan_off.p; n = 4LL * (c.V + 1); break;
+    case 6: src = c.fan.p; n = 4LL * c.n_fan; break;
+    case 7: src = c.edges.p; n = 8LL * c.E; break;
+    case 8: n = 64LL * c.V; break;
+    case 9: src = c.edge_cost.p; n = 4LL * c.E; break;
+    case 10: src = c.tri_min.p; n = 8LL * c.F; break;
+    case 11: src = c.collapsed.p; n = 4LL * c.n_collapsed; break;
+    case 12: src = c.have_s_first ? c.S_first.p : nullptr; n = c.F; break;
+    case 13: src = c.T0.p; n = 12LL * c.F; break;
+    case 14: src = c.table.p; n = c.alloc_pts ? (long long)c.alloc_pts + 1 : 0; break;
+    default: c.fail("unknown debug buffer %d", which); return -1;
+  }
+  if (!host_dst || n > cap_bytes || n == 0) return n;
+  if (which == 8) {
+    std::vector<float> q((size_t)10 * c.V);
+    if (!c.hip(hipMemcpyAsync(q.data(), c.Q.p, q.size() * 4, hipMemcpyDeviceToHost, c.st), "copy") ||
+        !c.hip(hipStreamSynchronize(c.st), "sync"))
+      return -1;
+    static const int map16[16] = {0, 1, 2, 3, 1, 4, 5, 6, 2, 5, 7, 8, 3, 6, 8, 9};
+    float* o = (float*)host_dst;
+    for (int v = 0; v < c.V; v++)
+      for (int r = 0; r < 16; r++) o[16LL * v + r] = q[10LL * v + map16[r]];
+    return n;
+  }
+  if (!src) {
+    memset(host_dst, 0, (size_t)n);
+    return n;
+  }
+  if (!c.hip(hipMemcpyAsync(host_dst, src, (size_t)n, hipMemcpyDeviceToHost, c.st), "copy") ||
+      !c.hip(hipStreamSynchronize(c.st), "sync"))
+    return -1;
+  return n;
+}
+
+long long pamo_debug_pairs(pamo_ctx* ctx, void* stream, const float* d_points, const int* d_tris, int n_pts,
+                           int n_tris, int* host_pairs, long long cap_rows) {
+  if (!ctx) return -1;
+  pamo_ctx& c = *ctx;
+  c.st = (hipStream_t)stream;
+  c.err.clear();
+  if (!c.hip(hipSetDevice(c.device), "hipSetDevice")) return -1;
+  const long long Vl = std::max(n_pts, 1), Fl = std::max(n_tris, 1);
+  if (!c.ensure(c.counters, C_COUNT * 4) || !c.ensure(c.cstats, 64 * 4) || !c.ensure(c.P, Vl * 12) ||
+      !c.ensure(c.T, Fl * 12) || !c.ensure(c.af, Fl * 4) || !c.ensure(c.apos, Fl * 4) || !c.ensure(c.qlist, Fl * 4) ||
+      !c.ensure(c.leaf_face, Fl * 4) || !c.ensure(c.S, Fl))
+    return -1;
+  c.V = n_pts;
+  c.F = n_tris;
+  if (!c.fill_i32(c.counters.p, 0, C_COUNT)) return -1;
+  k_validate<<<nblk(n_tris), BLOCK, 0, c.st>>>(d_tris, n_tris, n_pts, c.cnt_dev(C_ERR));
+  if (!c.hip(hipMemcpyAsync(c.P.p, d_points, (size_t)n_pts * 12, hipMemcpyDeviceToDevice, c.st), "copy") ||
+      !c.hip(hipMemcpyAsync(c.T.p, d_tris, (size_t)n_tris * 12, hipMemcpyDeviceToDevice, c.st), "copy"))
+    return -1;
+  if (!c.read_counters()) return -1;
+  if (c.hc[C_ERR]) { c.fail("invalid faces"); return -2; }
+  // all alive faces are both leaves and queries
+  if (!c.ensure(c.qflag, Fl) || !c.fill_u8(c.qflag.p, 1, Fl) || !c.ensure(c.qf, Fl * 4) || !c.ensure(c.qpos, (Fl + 1) * 4))
+    return -1;
+  k_query_flags<<<nblk(n_tris), BLOCK, 0, c.st>>>(c.ptr<uint8_t>(c.qflag), c.ptr<int>(c.T), n_tris, c.ptr<int>(c.qf),
+                                                 c.ptr<int>(c.af));
+  if (!c.scan(c.ptr<int>(c.af), c.ptr<int>(c.apos), n_tris, C_NALIVE)) return -1;
+  k_compact_idx<<<nblk(n_tris), BLOCK, 0, c.st>>>(c.ptr<int>(c.af), c.ptr<int>(c.apos), n_tris, c.ptr<int>(c.leaf_face), 0);
+  if (!c.scan(c.ptr<int>(c.qf), c.ptr<int>(c.qpos), n_tris, C_NQUERY)) return -1;
+  k_compact_idx<<<nblk(n_tris), BLOCK, 0, c.st>>>(c.ptr<int>(c.qf), c.ptr<int>(c.qpos), n_tris, c.ptr<int>(c.qlist), 0);
+  std::vector<int> pairs;
+  if (!c.build_bvh_and_traverse(n_tris, true, &pairs)) return -1;
+  long long np = (long long)pairs.size() / 3;
+  if (host_pairs && np <= cap_rows) memcpy(host_pairs, pairs.data(), pairs.size() * 4);
+  return np;
+}
+
+}  // extern "C"

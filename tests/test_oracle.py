@@ -1,0 +1,340 @@
+"""Pins the CPU oracle (oracle/pamo_oracle.cpp) and the exact verifier
+(oracle/verify.cpp) with checks that do not restate their formulas:
+golden vectors from rocThrust, closed forms, exact rational arithmetic,
+brute force on tiny inputs and invariants of the collapse."""
+import json
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle
+from pamo_amd import synth
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+# ------------------------------------------------------------ shuffle (D7)
+def test_shuffle_matches_rocthrust_golden():
+    """thrust::shuffle(default_random_engine) permutations produced by
+    rocThrust itself (tests/golden/make_shuffle_golden.py)."""
+    gold = json.load(open(os.path.join(HERE, "golden", "shuffle_thrust.json")))
+    assert len(gold) >= 10
+    for m, perm in gold.items():
+        p = oracle.shuffle_perm(int(m))
+        assert list(p) == perm, m
+
+
+@pytest.mark.parametrize("m", [1, 7, 16, 17, 1000, 65537])
+def test_shuffle_is_bijection(m):
+    p = oracle.shuffle_perm(m)
+    assert sorted(p.tolist()) == list(range(m))
+
+
+# ------------------------------------------------------ exact predicates
+def _orient_frac(a, b, c, d):
+    a, b, c, d = [[Fraction(float(x)) for x in p] for p in (a, b, c, d)]
+    m = [[a[i] - d[i] for i in range(3)], [b[i] - d[i] for i in range(3)], [c[i] - d[i] for i in range(3)]]
+    det = (m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+           m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]))
+    return (det > 0) - (det < 0)
+
+
+def test_orient3d_exact_vs_rationals():
+    rng = np.random.default_rng(11)
+    n = 0
+    for _ in range(400):
+        a, b, c = rng.uniform(-1, 1, (3, 3)).astype(np.float32)
+        # d nearly on the plane of abc: a float32 rounding of an exact combination
+        s, t = rng.uniform(-1, 1, 2)
+        d = (a + s * (b - a) + t * (c - a)).astype(np.float32)
+        d = d + rng.integers(-2, 3, 3).astype(np.float32) * np.spacing(np.abs(d).astype(np.float32))
+        for pts in ((a, b, c, d), (a, b, c, rng.uniform(-1, 1, 3).astype(np.float32))):
+            want = _orient_frac(*pts)
+            assert oracle.orient3d(*[p.astype(np.float64) for p in pts]) == want
+            assert oracle.orient3d(*[p.astype(np.float64) for p in pts], exact_only=True) == want
+            n += want == 0
+    # exact zeros: integer-grid points on a common plane
+    for _ in range(100):
+        a, b, c = rng.integers(-50, 50, (3, 3))
+        d = a + 3 * (b - a) - 2 * (c - a)
+        pts = [np.asarray(p, np.float64) * 2.0 ** -20 for p in (a, b, c, d)]
+        assert oracle.orient3d(*pts) == 0 == _orient_frac(*pts)
+
+
+def _tri(rng):
+    return rng.uniform(0, 1, (3, 3)).astype(np.float32)
+
+
+def test_reference_tritri_agrees_with_exact_on_generic_pairs():
+    """Away from the epsilon filters (no shared vertex, not near-coplanar,
+    no contact within 1e-3), the reference interval test and the exact test
+    agree."""
+    rng = np.random.default_rng(5)
+    agree = checked = 0
+    for _ in range(3000):
+        a, b = _tri(rng), _tri(rng) * 0.8 + 0.1
+        # skip near-degenerate configurations (orientation margins)
+        na = np.cross(a[1] - a[0], a[2] - a[0])
+        nb = np.cross(b[1] - b[0], b[2] - b[0])
+        da = np.abs((b - a[0]) @ na) / np.linalg.norm(na)
+        db = np.abs((a - b[0]) @ nb) / np.linalg.norm(nb)
+        if da.min() < 1e-3 or db.min() < 1e-3 or np.linalg.norm(np.cross(na, nb)) < 1e-3 * np.linalg.norm(na) * np.linalg.norm(nb):
+            continue
+        hit, r1, r2 = oracle.isect_raw(a, b)
+        exact = oracle.tri_pair_intersects(a, b, [0, 1, 2], [3, 4, 5])
+        if hit and np.abs(r2 - r1).max() < 1e-3:
+            continue
+        checked += 1
+        agree += hit == exact
+    assert checked > 1000
+    assert agree == checked
+
+
+def test_narrow_phase_filters():
+    """Outcome codes of self_intersect.cuh:284-375 on constructed pairs."""
+    t = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    # coplanar overlapping pair -> skipped as coplanar (code 2)
+    u = np.array([[0.2, 0.2, 0], [1.2, 0.2, 0], [0.2, 1.2, 0]], np.float32)
+    assert oracle.narrow(t, u, [0, 1, 2], [3, 4, 5]) == 2
+    # pair sharing an edge by index, folded through -> code 3
+    w = np.array([[1, 0, 0], [0, 1, 0], [0.3, 0.3, 0.5]], np.float32)
+    assert oracle.narrow(t, w, [0, 1, 2], [1, 2, 6]) == 3
+    # the same geometry with distinct indices -> shared-edge coordinates (code 4)
+    assert oracle.narrow(t, w, [0, 1, 2], [7, 8, 6]) == 4
+    # a crossing pair -> hit
+    x = np.array([[0.2, 0.2, -0.5], [0.3, 0.3, 0.5], [0.9, -0.5, 0.2]], np.float32)
+    assert oracle.narrow(t, x, [0, 1, 2], [3, 4, 5]) == 1
+    assert oracle.tri_pair_intersects(t, x, [0, 1, 2], [3, 4, 5])
+    # far apart -> miss
+    assert oracle.narrow(t, x + 5, [0, 1, 2], [3, 4, 5]) == 0
+    # shared vertex with a short contact -> filtered (code 5)
+    y = np.array([[0, 0, 0], [0.0004, 0.0004, 0.5], [0.0004, 0.0004, -0.5]], np.float32)
+    assert oracle.narrow(t, y, [0, 1, 2], [0, 4, 5]) == 5
+    assert oracle.is_coplanar(t, u) and not oracle.is_coplanar(t, x)
+
+
+# ----------------------------------------------------------- quadrics
+def _octahedron():
+    v = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], np.float32)
+    f = np.array([[0, 2, 4], [2, 1, 4], [1, 3, 4], [3, 0, 4], [2, 0, 5], [1, 2, 5], [3, 1, 5], [0, 3, 5]], np.int32)
+    return v, f
+
+
+def test_quadrics_closed_form():
+    """Each face plane (n, -n.v0) of the octahedron is (s/sqrt3, -1/sqrt3)
+    with s in {+-1}^3; Q_v = sum over the 4 faces at v of p p^T."""
+    v, f = _octahedron()
+    o = oracle.Oracle(shuffle=oracle.SHUFFLE_IDENTITY)
+    o.forward(v, f, np.zeros(0, np.int32), 2.0, 1e-3, False, True)
+    Q = o.get("Q").reshape(-1, 4, 4).astype(np.float64)
+    for vi in range(6):
+        want = np.zeros((4, 4))
+        for face in f:
+            if vi in face:
+                a, b, c = v[face].astype(np.float64)
+                n = np.cross(b - a, c - a)
+                n /= np.linalg.norm(n)
+                p = np.append(n, -n @ a)
+                want += np.outer(p, p)
+        assert np.allclose(Q[vi], want, atol=1e-6)
+        assert np.allclose(Q[vi], Q[vi].T, atol=0)            # exactly symmetric
+        assert abs(np.append(v[vi], 1) @ Q[vi] @ np.append(v[vi], 1)) < 1e-6
+
+
+def test_edge_cost_closed_form_octahedron():
+    """Edge (x+, y+) of the unit octahedron (identity shuffle): midpoint
+    m = (1/2, 1/2, 0).  QEM: the 8 fan planes give (n.m + d)^2 = 0 twice for
+    the two shared faces in each fan and 1/3 for the other two, so
+    v^T(Q_u+Q_v)v = 4/3 and the cost term is (4/3)/s^2 with s = 2.  Length
+    terms: every edge is sqrt2, so L = (sqrt2/s + sqrt2/s) tau.  Skinny
+    terms: the moved faces are (m, y-, z+-) and (m, x-, z+-); with the
+    reference precedence 4*sqrt3*A/e01^2 + e12^2 + e20^2 > 1 for all four,
+    so the penalty is 0."""
+    v, f = _octahedron()
+    tau = 1e-3
+    o = oracle.Oracle(shuffle=oracle.SHUFFLE_IDENTITY)
+    o.forward(v, f, np.zeros(0, np.int32), 2.0, tau, False, True)
+    edges = o.get("edges")
+    cost = o.get("edge_cost")
+    idx = [i for i, (a, b) in enumerate(edges) if (a, b) == (0, 2)]
+    assert len(idx) == 1
+    c = (4.0 / 3.0) / 4.0 + (np.sqrt(2) / 2 + np.sqrt(2) / 2) * tau
+    want = c / 10.0 * 2.0 ** 32
+    assert abs(int(cost[idx[0]]) - want) < 2e-6 * want
+    # by symmetry all 12 edges cost the same
+    assert len(set(cost.tolist())) == 1
+
+
+def test_cost_quantization_edges():
+    """collapse_t for tau=1e-3 is 429496 (float32 arithmetic of
+    cusimp_free.cu:923)."""
+    o = oracle.Oracle()
+    v, f = _octahedron()
+    o.forward(v, f, np.zeros(0, np.int32), 2.0, 1e-3, False, True)
+    assert o.sizes()["collapse_t"] == 429496
+    o.forward(v, f, np.zeros(0, np.int32), 2.0, 10.0, False, True)
+    assert o.sizes()["collapse_t"] == 0xFFFFFFFF
+    o.forward(v, f, np.zeros(0, np.int32), 2.0, 0.0, False, True)
+    assert o.sizes()["collapse_t"] == 0
+
+
+def test_flat_grid_has_zero_qem():
+    """On a planar patch every quadric annihilates every in-plane point, so
+    interior edge costs come only from the length and skinny terms."""
+    n = 8
+    xs, ys = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    v = np.stack([xs.ravel(), ys.ravel(), np.zeros(n * n)], 1).astype(np.float32)
+    f = []
+    for i in range(n - 1):
+        for j in range(n - 1):
+            a, b, c, d = i * n + j, (i + 1) * n + j, (i + 1) * n + j + 1, i * n + j + 1
+            f += [[a, b, c], [a, c, d]]
+    f = np.array(f, np.int32)
+    o = oracle.Oracle(shuffle=oracle.SHUFFLE_IDENTITY)
+    o.forward(v, f, np.zeros(0, np.int32), float(n - 1), 1e-3, False, True)
+    Q = o.get("Q").reshape(-1, 4, 4)
+    for vi in range(v.shape[0]):
+        if Q[vi].any():
+            p = np.append(v[vi] + np.array([0.3, -0.7, 0], np.float32), 1).astype(np.float64)
+            assert abs(p @ Q[vi] @ p) < 1e-9
+
+
+# ------------------------------------------------------ brute force / sets
+def test_candidate_pairs_brute_force():
+    v, f = synth.displaced_icosphere(2, 4)
+    cand, hits = oracle.full_check(v, f)
+    lo = v[f].min(1)
+    hi = v[f].max(1)
+    F = f.shape[0]
+    ov = np.ones((F, F), bool)
+    for k in range(3):
+        ov &= ~((hi[:, None, k] < lo[None, :, k]) | (hi[None, :, k] < lo[:, None, k]))
+    np.fill_diagonal(ov, False)
+    want = np.argwhere(ov)
+    assert np.array_equal(cand, want)
+    assert hits.shape[0] == 0   # a star-shaped surface has no intersections
+
+
+def _fans(tris, V):
+    fans = [[] for _ in range(V)]
+    for fi, t in enumerate(tris):
+        if t[0] == -1 and t[1] == -1 and t[2] == -1:
+            continue
+        for x in t:
+            fans[x].append(fi)
+    return fans
+
+
+def test_claims_and_independent_set_brute_force():
+    v, f = synth.displaced_icosphere(3, 2)
+    verts, scale, _ = oracle.preprocess(v, f)
+    o = oracle.Oracle()
+    o.forward(verts, f, np.zeros(0, np.int32), scale, 1e-3, False, True)
+    T0 = o.get("orig_tris")
+    edges = o.get("edges")
+    cost = o.get("edge_cost").astype(np.uint64)
+    fans = _fans(T0, verts.shape[0])
+    # fan order is ascending face index (D0)
+    off = o.get("fan_off")
+    fan = o.get("fan")
+    for x in range(verts.shape[0]):
+        assert fan[off[x]:off[x + 1]].tolist() == fans[x]
+    keys = (cost << np.uint64(32)) | np.arange(len(edges), dtype=np.uint64)
+    want = np.full(T0.shape[0], np.iinfo(np.uint64).max, np.uint64)
+    for e, (a, b) in enumerate(edges):
+        for fi in fans[a] + fans[b]:
+            want[fi] = min(want[fi], keys[e])
+    assert np.array_equal(o.get("tri_min"), want)
+    coll = o.get("collapsed")
+    owned = {}
+    for e in coll:
+        a, b = edges[e]
+        for fi in set(fans[a] + fans[b]):
+            assert fi not in owned          # winners own disjoint fans
+            owned[fi] = e
+        assert cost[e] <= o.sizes()["collapse_t"]
+
+
+# ---------------------------------------------------- invariants per round
+@pytest.mark.parametrize("mesh", ["ico", "torus"])
+def test_round_invariants(mesh):
+    """Each round keeps the surface closed and manifold, keeps the Euler
+    characteristic, removes exactly two faces per net collapse, and restored
+    fans equal the round-start faces bit for bit."""
+    if mesh == "ico":
+        v, f = synth.displaced_icosphere(4, 3)
+    else:
+        v, f = synth.noisy_torus(40, 16, 2)
+    verts, scale, _ = oracle.preprocess(v, f)
+    chi0 = oracle.verify(verts, f)["euler"]
+    o = oracle.Oracle()
+    undo = np.zeros(0, np.int32)
+    for r in range(12):
+        P, T, occ, vmap, undo = o.forward(verts, f, undo, scale, 1e-3, False, r == 0)
+        T0 = o.get("orig_tris")
+        alive = T[:, 0] >= 0
+        removed_v = int((occ == 0).sum())
+        assert T.shape[0] - alive.sum() == 2 * removed_v
+        changed = np.any(T != T0, axis=1)
+        assert np.all(changed <= (~alive | np.isin(T, np.nonzero(occ.ravel() == 1)[0]).all(1)))
+        verts = P[occ.ravel() == 1]
+        f = vmap.ravel()[T[alive]].astype(np.int32)
+        res = oracle.verify(verts, f)
+        assert res["manifold"] and res["watertight"], res
+        assert res["euler"] == chi0
+
+
+def test_run_reaches_target_or_tolerance():
+    v, f = synth.displaced_icosphere(4, 8)
+    verts, scale, _ = oracle.preprocess(v, f)
+    target = int(0.05 * f.shape[0])
+    ov, of, st, rf = oracle.run(verts, f, scale, target, return_rounds=True)
+    assert st["exit_reason"] in (1, 3)
+    if st["exit_reason"] == 1:
+        assert of.shape[0] <= target
+    else:
+        assert rf[-1] == rf[-2] == rf[-3] == rf[-4] == rf[-5]
+    assert np.all(np.diff(rf) <= 0)
+    assert oracle.verify(ov, of)["ok"]
+
+
+def test_undo_loop_on_intersecting_input():
+    """Two interpenetrating spheres: collapses touching the intersection are
+    undone, up to 5 passes (cusimp_free.cu:1024-1040)."""
+    v1, f1 = synth.displaced_icosphere(3, 3)
+    v2, f2 = synth.displaced_icosphere(3, 4)
+    v = np.concatenate([v1, v2 + np.array([0.9, 0.05, 0], np.float32)]).astype(np.float32)
+    f = np.concatenate([f1, f2 + v1.shape[0]]).astype(np.int32)
+    verts, scale, _ = oracle.preprocess(v, f)
+    o = oracle.Oracle()
+    o.forward(verts, f, np.zeros(0, np.int32), scale, 1e-3, False, True)
+    s = o.sizes()
+    assert s["undo_passes"] >= 1 and s["n_vertices_undo"] > 0
+    und = o.get("undo")
+    assert und.shape[0] == s["n_vertices_undo"]
+    assert np.all(und == 0)          # first round: the stale map is zero-filled (D1)
+
+
+# ------------------------------------------------------------- verifier
+def test_verifier_detects_defects():
+    v, f = synth.displaced_icosphere(2, 1)
+    assert oracle.verify(v, f)["ok"]
+    r = oracle.verify(v, f[1:])
+    assert r["boundary_edges"] == 3 and not r["watertight"]
+    g = f.copy()
+    g[0] = g[0][[0, 2, 1]]
+    assert oracle.verify(v, g)["misoriented_edges"] > 0
+    w = v.copy()
+    w[0] = -w[0] * 1.5           # pull a vertex out through the opposite side
+    assert oracle.verify(w, f)["intersecting_pairs"] > 0
+
+
+def test_sdf_generator_is_closed_manifold():
+    v, f = synth.sdf_union(40, 80, seed=9)
+    r = oracle.verify(v, f)
+    assert r["ok"], r
+    v, f = synth.sdf_union(30, 100, seed=5, n_slab_pairs=4)
+    r = oracle.verify(v, f)
+    assert r["manifold"] and r["watertight"] and r["degenerate_faces"] == 0
