@@ -1,0 +1,201 @@
+"""Batched and served stage 2 over the engine in libpamo_hip.so.
+
+``BatchEngine`` wraps one context's engine: meshes are submitted (appended to
+the packed device state), ``step`` advances every active mesh by whole
+rounds in shared launches, finished meshes are retired between rounds and
+their results taken.  Each mesh follows exactly the round loop of
+``PaMO.run`` (simp_cuda/pamo/__init__.py:121-158), so a mesh's result is the
+same alone, in a batch, or admitted mid-stream.
+
+``simplify_batch`` runs a list of meshes to completion.  ``Server`` is a
+continuous-batching request loop: jobs arriving at given times are admitted
+while the HBM budget allows, rounds run for everything on the device, and
+finished jobs are replaced by waiting ones between rounds.
+"""
+import ctypes
+import time
+
+import numpy as np
+import torch
+
+from . import _lib, _preprocess, _scale
+
+# device bytes per face of the packed state (positions, faces and their
+# round-start copies, fans, edges, costs, claims, LBVH nodes, scan scratch)
+BYTES_PER_FACE = 420
+
+
+class BatchEngine:
+    def __init__(self, device=None, shuffle=_lib.SHUFFLE_FEISTEL, ctx=None):
+        if ctx is None:
+            ctx = _lib.Context(torch.cuda.current_device() if device is None else device, shuffle)
+        self.ctx = ctx
+        L = ctx._L
+        P, I, LL, Fl = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_float
+        L.pamo_batch_submit.argtypes = [P, P, P, P, I, I, Fl, LL, Fl, I, LL, LL]
+        L.pamo_batch_submit.restype = I
+        L.pamo_batch_step.argtypes = [P, P, I, ctypes.POINTER(I)]
+        L.pamo_batch_step.restype = I
+        L.pamo_batch_poll.argtypes = [P, P, P, I]
+        L.pamo_batch_poll.restype = I
+        L.pamo_batch_result.argtypes = [P, I, ctypes.POINTER(I), ctypes.POINTER(I), P, P, I]
+        L.pamo_batch_result.restype = I
+        L.pamo_batch_take.argtypes = [P, P, I, P, P]
+        L.pamo_batch_take.restype = I
+        L.pamo_batch_state.argtypes = [P, P]
+        L.pamo_batch_state.restype = I
+        self.L = L
+
+    def _stream(self):
+        return torch.cuda.current_stream().cuda_stream
+
+    def submit(self, verts, faces, scale, target, threshold=1e-3, tolerance=4, iters=1000000, job=-1):
+        """verts: centred float32 CUDA tensor [V,3]; faces int32 [F,3]."""
+        verts = verts.contiguous()
+        faces = faces.contiguous()
+        h = self.L.pamo_batch_submit(self.ctx.handle, self._stream(), verts.data_ptr(), faces.data_ptr(),
+                                     int(verts.shape[0]), int(faces.shape[0]), float(scale), int(target),
+                                     float(threshold), int(tolerance), int(iters), int(job))
+        return self.ctx.check(h, "pamo_batch_submit")
+
+    def step(self, rounds=1):
+        na = ctypes.c_int()
+        fin = self.L.pamo_batch_step(self.ctx.handle, self._stream(), int(rounds), ctypes.byref(na))
+        self.ctx.check(fin, "pamo_batch_step")
+        return fin, na.value
+
+    def poll(self):
+        n = self.L.pamo_batch_poll(self.ctx.handle, None, None, 0)
+        h = np.zeros(max(n, 1), np.int32)
+        j = np.zeros(max(n, 1), np.int64)
+        self.L.pamo_batch_poll(self.ctx.handle, h.ctypes.data, j.ctypes.data, n)
+        return list(zip(h[:n].tolist(), j[:n].tolist()))
+
+    def result_info(self, handle):
+        nv, nf = ctypes.c_int(), ctypes.c_int()
+        st = np.zeros(8, np.int64)
+        rf = np.zeros(4096, np.int32)
+        rc = self.L.pamo_batch_result(self.ctx.handle, int(handle), ctypes.byref(nv), ctypes.byref(nf),
+                                      st.ctypes.data, rf.ctypes.data, rf.size)
+        self.ctx.check(rc, "pamo_batch_result")
+        if rc == 1:
+            return None
+        rounds = int(st[0])
+        return dict(V=nv.value, F=nf.value, rounds=rounds, exit_reason=int(st[1]), collapses=int(st[2]),
+                    max_undo_passes=int(st[3]), job=int(st[4]), V0=int(st[5]), F0=int(st[6]),
+                    seconds=float(st[7]) * 1e-6, round_faces=rf[:min(rounds, rf.size)].copy())
+
+    def take(self, handle, device=None):
+        info = self.result_info(handle)
+        if info is None:
+            raise RuntimeError("job %d has not finished" % handle)
+        dev = device or torch.device("cuda", self.ctx.device)
+        v = torch.empty((info["V"], 3), dtype=torch.float32, device=dev)
+        f = torch.empty((info["F"], 3), dtype=torch.int32, device=dev)
+        self.ctx.check(self.L.pamo_batch_take(self.ctx.handle, self._stream(), int(handle), v.data_ptr(),
+                                              f.data_ptr()), "pamo_batch_take")
+        return v, f, info
+
+    def state(self):
+        out = np.zeros(6, np.int64)
+        self.L.pamo_batch_state(self.ctx.handle, out.ctypes.data)
+        return dict(active=int(out[0]), V=int(out[1]), F=int(out[2]), checks=int(out[3]), query_faces=int(out[4]),
+                    candidate_pairs=int(out[5]))
+
+
+def prepare(points, faces, device):
+    """PaMO.run's centring and scale (simp_cuda/pamo/__init__.py:60-64,117,126)."""
+    pts = torch.as_tensor(np.asarray(points, np.float32)).to(device)
+    tri = torch.as_tensor(np.asarray(faces, np.int32)).to(device)
+    tris_mean = _preprocess(pts, tri)
+    verts = pts - torch.from_numpy(tris_mean).to(device)
+    return verts, tri, _scale(verts), tris_mean
+
+
+def simplify_batch(meshes, ratios, threshold=1e-3, tolerance=4, min_verts=0, engine=None, rounds_per_step=1):
+    """Simplify many meshes together.  meshes: list of (points, faces) host
+    arrays; ratios: one ratio or one per mesh.  Returns a list of
+    (verts, faces, info) with host numpy arrays in the input order."""
+    dev = torch.device("cuda", torch.cuda.current_device())
+    eng = engine or BatchEngine()
+    if np.isscalar(ratios):
+        ratios = [ratios] * len(meshes)
+    means = {}
+    for i, ((p, f), r) in enumerate(zip(meshes, ratios)):
+        v, t, s, m = prepare(p, f, dev)
+        target = max(int(r * len(f)), min_verts)
+        h = eng.submit(v, t, s, target, threshold, tolerance, job=i)
+        means[h] = (i, m)
+    out = [None] * len(meshes)
+    pending = len(meshes)
+    while pending:
+        _, active = eng.step(rounds_per_step)
+        for h, job in eng.poll():
+            v, f, info = eng.take(h)
+            i, m = means[h]
+            out[i] = (v.cpu().numpy() + m, f.cpu().numpy(), info)
+            pending -= 1
+        if active == 0 and pending:
+            raise RuntimeError("engine idle with %d jobs pending" % pending)
+    return out
+
+
+class Server:
+    """Continuous batching: jobs (points, faces, ratio, arrival time) are
+    admitted when they have arrived and the packed state stays under the
+    HBM budget; every loop iteration runs one round for all admitted jobs.
+    Latency = completion time - arrival time (both on the host clock)."""
+
+    def __init__(self, budget_bytes=200 << 30, threshold=1e-3, tolerance=4, min_verts=0, engine=None):
+        self.eng = engine or BatchEngine()
+        self.budget = budget_bytes
+        self.threshold = threshold
+        self.tolerance = tolerance
+        self.min_verts = min_verts
+
+    def run(self, jobs, realtime=True, on_result=None):
+        """jobs: list of dicts {points, faces, ratio, arrival} (arrival in
+        seconds from the start).  Returns per-job records."""
+        dev = torch.device("cuda", torch.cuda.current_device())
+        order = sorted(range(len(jobs)), key=lambda i: jobs[i]["arrival"])
+        waiting = []
+        nxt = 0
+        live = {}
+        used = 0
+        recs = [None] * len(jobs)
+        t0 = time.perf_counter()
+        done = 0
+        while done < len(jobs):
+            now = time.perf_counter() - t0
+            while nxt < len(order) and (jobs[order[nxt]]["arrival"] <= now or not realtime):
+                waiting.append(order[nxt])
+                nxt += 1
+            while waiting:
+                i = waiting[0]
+                need = BYTES_PER_FACE * len(jobs[i]["faces"])
+                if live and used + need > self.budget:
+                    break
+                waiting.pop(0)
+                j = jobs[i]
+                v, t, s, m = prepare(j["points"], j["faces"], dev)
+                target = max(int(j["ratio"] * len(j["faces"])), self.min_verts)
+                h = self.eng.submit(v, t, s, target, self.threshold, self.tolerance, job=i)
+                live[h] = (i, m, need, time.perf_counter() - t0)
+                used += need
+            if not live:
+                if nxt < len(order):
+                    time.sleep(max(0.0, min(0.001, jobs[order[nxt]]["arrival"] - now)))
+                continue
+            self.eng.step(1)
+            for h, _ in self.eng.poll():
+                v, f, info = self.eng.take(h)
+                i, m, need, t_adm = live.pop(h)
+                used -= need
+                t_end = time.perf_counter() - t0
+                recs[i] = dict(job=i, arrival=jobs[i]["arrival"], admitted=t_adm, finished=t_end,
+                               latency=t_end - jobs[i]["arrival"], faces_in=len(jobs[i]["faces"]), faces_out=info["F"],
+                               rounds=info["rounds"])
+                if on_result is not None:
+                    on_result(i, v.cpu().numpy() + m, f.cpu().numpy(), info)
+                done += 1
+        return recs

@@ -1,0 +1,838 @@
+// Kernels of the batched engine: many meshes packed CSR-style in one set of
+// buffers and simplified together, one launch per step for all of them.
+//
+// Packing: vertices and faces of mesh k occupy [voff[k], voff[k+1]) and
+// [foff[k], foff[k+1]); face indices are global.  vmesh/fmesh give the mesh
+// of a vertex/face.  Persistent per-mesh state (the previous round's vertex
+// map, the invalid-vertex table and the edge costs, which the reference keeps
+// between calls, simp_cuda/src/cusimp_free.h:184-280) lives in per-mesh
+// regions at fixed offsets (pmo[k] for map/table, eco[k] for edge costs)
+// addressed by LOCAL indices, exactly as one reference context per mesh.
+//
+// Edge indices are global and ascending within a mesh, so the claim keys
+// (cost << 32 | edge) order the edges of one mesh exactly as the reference's
+// local indices do (keys of different meshes are never compared: a face
+// belongs to one mesh).
+#pragma once
+#include "pamo_bvh.h"
+#include "pamo_kernels.h"
+
+namespace pamo {
+namespace eng {
+
+// per-mesh parameters of a round (device copy of a host struct array)
+struct MeshParam {
+  float scale, tau;
+  uint32_t ct;            // quantized collapse threshold
+  int stuck;
+  long long pmo, eco;     // persistent region offsets (map/table, edge costs)
+  long long pmcap;        // map/table region entries (alloc_pts + 1)
+};
+
+__device__ __forceinline__ int find_seg(const long long* __restrict__ off, int M, long long i) {
+  int lo = 0, hi = M - 1;   // largest k with off[k] <= i
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// ------------------------------------------------------- step 1 (undo in)
+// clear the table of meshes that bring an undo list and are not stuck
+// (cusimp_free.cu:905-908); one block per mesh
+__global__ void k_clear_tables(const MeshParam* __restrict__ mp, const long long* __restrict__ uioff, int M,
+                               uint8_t* __restrict__ table) {
+  int k = blockIdx.x;
+  if (k >= M) return;
+  if (mp[k].stuck || uioff[k + 1] == uioff[k]) return;
+  for (long long i = threadIdx.x; i < mp[k].pmcap; i += blockDim.x) table[mp[k].pmo + i] = 0;
+}
+// every incoming undo entry marks its mesh's table (cusimp_free.cu:910-916);
+// bounds-checked (the reference is not)
+__global__ void k_undo_table(const int* __restrict__ undo_in, const long long* __restrict__ uioff, int M,
+                             const MeshParam* __restrict__ mp, uint8_t* __restrict__ table, int* err) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= uioff[M]) return;
+  int k = find_seg(uioff, M, i);
+  int x = undo_in[i];
+  if (x >= 0 && (long long)x < mp[k].pmcap) table[mp[k].pmo + x] = 1;
+  else atomicAdd(err, 1);
+}
+// stuck meshes copy their incoming list through the stale map into the head
+// of their undo output (cusimp_free.cu:899-904)
+__global__ void k_undo_remap_in(const int* __restrict__ undo_in, const long long* __restrict__ uioff, int M,
+                                const MeshParam* __restrict__ mp, const int* __restrict__ pm,
+                                const long long* __restrict__ uobase, int* __restrict__ undo_out) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= uioff[M]) return;
+  int k = find_seg(uioff, M, i);
+  if (!mp[k].stuck) return;
+  int x = undo_in[i];
+  bool ok = x >= 0 && (long long)x < mp[k].pmcap;
+  undo_out[uobase[k] + (i - uioff[k])] = ok ? pm[mp[k].pmo + x] : 0;
+}
+__global__ void k_init_mesh_state(int M, const int* __restrict__ nust, int* __restrict__ first, int* __restrict__ nvu,
+                                  int* __restrict__ lcnt) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= M) return;
+  first[k] = nust[k];
+  nvu[k] = nust[k];
+  lcnt[k] = 0;
+}
+
+// admission: faces with local indices -> global, with or without the shuffle
+__global__ void k_copy_faces_off(const int* __restrict__ tin, int* __restrict__ T, int F, int voff) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3LL * F) return;
+  T[i] = tin[i] + voff;
+}
+__global__ void k_shuffle_scatter_off(Feistel fb, long long n, long long m, const int* __restrict__ pos,
+                                      const int* __restrict__ tin, int* __restrict__ T, int voff) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t k = feistel(fb, (uint64_t)i);
+  if (k >= (uint64_t)m) return;
+  int o = pos[i];
+  for (int c = 0; c < 3; c++) T[3LL * o + c] = tin[3 * k + c] + voff;
+}
+
+// ----------------------------------------------------------- edge cost
+// as k_edge_cost with per-mesh parameters; the table and the (persistent)
+// cost array are addressed by local indices in the mesh's regions
+__global__ void __launch_bounds__(BLOCK) k_edge_cost_b(const float* __restrict__ P, const int* __restrict__ T,
+                                                       const int* __restrict__ fan_off, const int* __restrict__ fan,
+                                                       const int* __restrict__ edges, const int* __restrict__ E_ptr,
+                                                       const float* __restrict__ Q, const uint8_t* __restrict__ table,
+                                                       uint32_t* __restrict__ cost, const int* __restrict__ vmesh,
+                                                       const long long* __restrict__ voff,
+                                                       const long long* __restrict__ eoffm,
+                                                       const MeshParam* __restrict__ mp) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= *E_ptr) return;
+  int u = edges[2 * e], v = edges[2 * e + 1];
+  int k = vmesh[u];
+  const MeshParam prm = mp[k];
+  long long lu = u - voff[k], lv = v - voff[k];
+  uint32_t* c = cost + prm.eco + (e - eoffm[k]);
+  if (table[prm.pmo + lu] && table[prm.pmo + lv]) { *c = 0xFFFFFFFFu; return; }
+  if (prm.stuck) return;
+  V3 v0 = ldp(P, u), v1 = ldp(P, v);
+  int ua = fan_off[u], ub = fan_off[u + 1], va = fan_off[v], vb = fan_off[v + 1];
+  int dup = 0;
+  for (int a = ua; a < ub; a++) {
+    int iu = succ_of(ldt(T, fan[a]), u);
+    for (int b = va; b < vb; b++) {
+      int iv = succ_of(ldt(T, fan[b]), v);
+      if (iu == iv) dup++;
+      if (dup > 2) { *c = 0xFFFFFFFFu; return; }
+    }
+  }
+  if (dup != 2) return;
+  const float scale = prm.scale, tau = prm.tau;
+  int ne = 0;
+  float el = fan_edge_len(P, T, fan_off, fan, u, 0.0f, ne);
+  el = fan_edge_len(P, T, fan_off, fan, v, el, ne);
+  el = el / (float)ne / scale * tau;
+  el = el + vnorm(vsub(v0, v1)) / scale * tau;
+  V3 m = vdivs(vadd(v0, v1), 2.0f);
+  float Qa = 0.0f;
+  int nt = 0;
+  if (!skinny_pass(P, T, fan_off, fan, u, v, m, Qa, nt)) { *c = 0xFFFFFFFFu; return; }
+  if (!skinny_pass(P, T, fan_off, fan, v, u, m, Qa, nt)) { *c = 0xFFFFFFFFu; return; }
+  Qa = 5.0f * Qa;
+  const float* qu = Q + (long long)10 * u;
+  const float* qv = Q + (long long)10 * v;
+  float s[10];
+#pragma unroll
+  for (int i = 0; i < 10; i++) s[i] = qu[i] + qv[i];
+  float t0 = s[0] * m.x + s[1] * m.y + s[2] * m.z + s[3] * 1.0f;
+  float t1 = s[1] * m.x + s[4] * m.y + s[5] * m.z + s[6] * 1.0f;
+  float t2 = s[2] * m.x + s[5] * m.y + s[7] * m.z + s[8] * 1.0f;
+  float t3 = s[3] * m.x + s[6] * m.y + s[8] * m.z + s[9] * 1.0f;
+  float qe = t0 * m.x + t1 * m.y + t2 * m.z + t3 * 1.0f;
+  float cc = qe / (scale * scale);
+  *c = u32sat(clamp_ref(cc + el + Qa / (float)nt * tau, 0.0f, COST_RANGE) / COST_RANGE * U32F);
+}
+
+// gather of the per-edge cost from the mesh regions into a flat array (the
+// claim and collapse kernels read costs by global edge index)
+__global__ void k_gather_cost(const uint32_t* __restrict__ cost_reg, const int* __restrict__ edges,
+                              const int* __restrict__ E_ptr, const int* __restrict__ vmesh,
+                              const long long* __restrict__ eoffm, const MeshParam* __restrict__ mp,
+                              uint32_t* __restrict__ flat) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= *E_ptr) return;
+  int k = vmesh[edges[2 * e]];
+  flat[e] = cost_reg[mp[k].eco + (e - eoffm[k])];
+}
+
+__global__ void __launch_bounds__(BLOCK) k_claim_b(const int* __restrict__ edges, const int* __restrict__ E_ptr,
+                                                   const uint32_t* __restrict__ cost, const int* __restrict__ fan_off,
+                                                   const int* __restrict__ fan, u64* tri_min) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= *E_ptr) return;
+  u64 key = ((u64)cost[e] << 32) | (u64)(uint32_t)e;
+  for (int w = 0; w < 2; w++) {
+    int x = edges[2 * e + w];
+    int a = fan_off[x], b = fan_off[x + 1];
+    for (int s = a; s < b; s++) atomicMin(&tri_min[fan[s]], key);
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_collapse_b(const int* __restrict__ edges, const int* __restrict__ E_ptr,
+                                                      const uint32_t* __restrict__ cost, const int* __restrict__ vmesh,
+                                                      const MeshParam* __restrict__ mp,
+                                                      const int* __restrict__ fan_off, const int* __restrict__ fan,
+                                                      const u64* __restrict__ tri_min, float* P,
+                                                      const float* __restrict__ P0, int* T, const int* __restrict__ T0,
+                                                      int* occ, int* __restrict__ win) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int E = *E_ptr;
+  if (e > E) return;
+  if (e == E) { win[E] = 0; return; }
+  uint32_t c = cost[e];
+  int u = edges[2 * e], v = edges[2 * e + 1];
+  bool ok = c <= mp[vmesh[u]].ct;
+  if (ok) {
+    u64 key = ((u64)c << 32) | (u64)(uint32_t)e;
+    for (int w = 0; w < 2 && ok; w++) {
+      int x = w == 0 ? u : v;
+      for (int s = fan_off[x]; s < fan_off[x + 1]; s++)
+        if (tri_min[fan[s]] != key) { ok = false; break; }
+    }
+  }
+  win[e] = ok ? 1 : 0;
+  if (!ok) return;
+  V3 m = vdivs(vadd(ldp(P, u), ldp(P, v)), 2.0f);
+  stp(P, u, m);
+  stp(P, v, V3{0.0f, 0.0f, 0.0f});
+  occ[v] = 0;
+  surgery(T, fan_off, fan, u, v);
+  bool sk = false;
+  for (int w = 0; w < 2 && !sk; w++) {
+    int x = w == 0 ? u : v;
+    for (int s = fan_off[x]; s < fan_off[x + 1]; s++) {
+      I3 t = ldt(T, fan[s]);
+      if (t.i == -1) continue;
+      if (is_skinny(ldp(P, t.i), ldp(P, t.j), ldp(P, t.k))) { sk = true; break; }
+    }
+  }
+  if (sk) restore_edge(u, v, P, P0, T, T0, occ, fan_off, fan);
+}
+
+// winners -> the mesh's collapsed region, in ascending edge order
+// (region of mesh k starts at cbase[k] = 2 * edge offset, capacity 2 E_k)
+__global__ void k_winners_to_regions(const int* __restrict__ win, const int* __restrict__ winpos,
+                                     const int* __restrict__ E_ptr, const int* __restrict__ edges,
+                                     const int* __restrict__ vmesh, const long long* __restrict__ eoffm,
+                                     int* __restrict__ coll) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= *E_ptr || !win[e]) return;
+  int k = vmesh[edges[2 * e]];
+  long long base = eoffm[k];
+  coll[2 * base + (winpos[e] - winpos[base])] = e;
+}
+
+// per-mesh edge offsets (faces emit their edges in order)
+__global__ void k_edge_offsets(int M, const long long* __restrict__ foff, const int* __restrict__ eoff_face,
+                               long long* __restrict__ eoffm) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > M) return;
+  eoffm[k] = eoff_face[foff[k]];
+}
+// per-mesh winner counts
+__global__ void k_win_counts(int M, const long long* __restrict__ eoffm, const int* __restrict__ winpos,
+                             int* __restrict__ h) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= M) return;
+  h[k] = winpos[eoffm[k + 1]] - winpos[eoffm[k]];
+}
+__global__ void k_part_from_cut(int M, const int* __restrict__ cut, int* __restrict__ part) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < M) part[k] = cut[k] > 0 ? 1 : 0;
+}
+__global__ void k_flat_total(const long long* __restrict__ off, int M, int* out) { *out = (int)off[M]; }
+
+__global__ void k_line_detect_b(const float* __restrict__ P, const int* __restrict__ edges,
+                                const int* __restrict__ E_ptr, const int* __restrict__ vmesh, int* lcnt) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= *E_ptr) return;
+  int u = edges[2 * e];
+  V3 p = ldp(P, u), q = ldp(P, edges[2 * e + 1]);
+  if (p.x == q.x && p.y == q.y && p.z == q.z) atomicAdd(&lcnt[vmesh[u]], 1);
+}
+// sequential per mesh (D5), one thread per mesh with candidates; appends to
+// the mesh's collapsed region; then the scan cut (cusimp_free.cu:1016)
+__global__ void k_line_apply_cut(int M, const int* __restrict__ lcnt, float* P, int* T, int* occ,
+                                 const int* __restrict__ edges, const long long* __restrict__ eoffm,
+                                 const int* __restrict__ fan_off, const int* __restrict__ fan, int* coll,
+                                 const int* __restrict__ h, int* __restrict__ ncoll, int* __restrict__ cut) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= M) return;
+  int n = h[k];
+  long long base = 2 * eoffm[k];
+  if (lcnt[k] > 0) {
+    for (long long e = eoffm[k]; e < eoffm[k + 1]; e++) {
+      int u = edges[2 * e], v = edges[2 * e + 1];
+      V3 p = ldp(P, u), q = ldp(P, v);
+      if (!(p.x == q.x && p.y == q.y && p.z == q.z)) continue;
+      coll[base + n++] = (int)e;
+      stp(P, v, V3{0.0f, 0.0f, 0.0f});
+      occ[v] = 0;
+      surgery(T, fan_off, fan, u, v);
+    }
+  }
+  ncoll[k] = n;
+  int hk = h[k];
+  cut[k] = hk == 0 ? 0 : min(n, ((hk + 511) / 512) * 512);
+}
+
+// ------------------------------------------------- flat views over meshes
+// exclusive scan of a small per-mesh int array into long long offsets (one
+// block; M may exceed the block)
+__global__ void k_small_scan(const int* __restrict__ x, int M, long long* __restrict__ off, long long mult,
+                             const int* __restrict__ extra, long long extra_mult) {
+  __shared__ long long carry;
+  __shared__ long long part[BLOCK];
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < M; base += BLOCK) {
+    int i = base + threadIdx.x;
+    long long v = 0;
+    if (i < M) v = (long long)x[i] * mult + (extra ? (long long)extra[i] * extra_mult : 0);
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < BLOCK; o <<= 1) {
+      long long y = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+      __syncthreads();
+      part[threadIdx.x] += y;
+      __syncthreads();
+    }
+    if (i < M) off[i] = carry + part[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == BLOCK - 1) carry += part[BLOCK - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) off[M] = carry;
+}
+
+// faces of the fans of the scanned collapsed entries (flat list over meshes)
+__global__ void k_qflag_b(const long long* __restrict__ cutoff, int M, const long long* __restrict__ eoffm,
+                          const int* __restrict__ coll, const int* __restrict__ edges,
+                          const int* __restrict__ fan_off, const int* __restrict__ fan,
+                          const int* __restrict__ part, uint8_t* qflag) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cutoff[M]) return;
+  int k = find_seg(cutoff, M, i);
+  if (!part[k]) return;
+  int e = coll[2 * eoffm[k] + (i - cutoff[k])];
+  for (int w = 0; w < 2; w++) {
+    int x = edges[2 * e + w];
+    for (int s = fan_off[x]; s < fan_off[x + 1]; s++) qflag[fan[s]] = 1;
+  }
+}
+__global__ void k_query_flags_b(const uint8_t* __restrict__ qflag, const int* __restrict__ T,
+                                const int* __restrict__ F_ptr, const int* __restrict__ fmesh,
+                                const int* __restrict__ part, int* __restrict__ qf, int* __restrict__ lf) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  int F = *F_ptr;
+  if (f > F) return;
+  if (f == F) { qf[F] = 0; lf[F] = 0; return; }
+  bool alive = T[3 * f] != -1 || T[3 * f + 1] != -1 || T[3 * f + 2] != -1;
+  bool p = alive && part[fmesh[f]];
+  lf[f] = p ? 1 : 0;
+  qf[f] = (p && qflag[f]) ? 1 : 0;
+}
+__global__ void k_compact_idx_dev(const int* __restrict__ flags, const int* __restrict__ pos,
+                                  const int* __restrict__ n_ptr, int* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= *n_ptr) return;
+  if (flags[i]) out[pos[i]] = i;
+}
+
+// -------------------------------------------------------- segmented LBVH
+// leaf boxes and per-mesh scene boxes (wave-uniform fast path)
+__global__ void __launch_bounds__(BLOCK) k_leaf_boxes_b(const int* __restrict__ leaf_face, const int* __restrict__ n_ptr,
+                                                        const float* __restrict__ P, const int* __restrict__ T,
+                                                        const int* __restrict__ fmesh, float4* __restrict__ lbox_lo,
+                                                        float4* __restrict__ lbox_hi, int* scene) {
+  int n = *n_ptr;
+  int l = blockIdx.x * blockDim.x + threadIdx.x;
+  bool act = l < n;
+  int k = -1;
+  float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  if (act) {
+    int f = leaf_face[l];
+    k = fmesh[f];
+    I3 t = ldt(T, f);
+    V3 a = ldp(P, t.i), b = ldp(P, t.j), c = ldp(P, t.k);
+    v[0] = fminf(a.x, fminf(b.x, c.x)); v[1] = fminf(a.y, fminf(b.y, c.y)); v[2] = fminf(a.z, fminf(b.z, c.z));
+    v[3] = fmaxf(a.x, fmaxf(b.x, c.x)); v[4] = fmaxf(a.y, fmaxf(b.y, c.y)); v[5] = fmaxf(a.z, fmaxf(b.z, c.z));
+    lbox_lo[l] = make_float4(v[0], v[1], v[2], 0.0f);
+    lbox_hi[l] = make_float4(v[3], v[4], v[5], 0.0f);
+  }
+  int k0 = __shfl(k, 0, 64);
+  bool uniform = __all(k == k0 || !act);
+  if (uniform && __any(act)) {
+    // first active lane's mesh: all active lanes share it
+    unsigned long long m = __ballot(act);
+    int leader = __ffsll((long long)m) - 1;
+    int kk = __shfl(k, leader, 64);
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      float x = v[q];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        float y = __shfl_xor(x, o, 64);
+        x = q < 3 ? fminf(x, y) : fmaxf(x, y);
+      }
+      v[q] = x;
+    }
+    if ((threadIdx.x & 63) == leader) {
+      for (int q = 0; q < 6; q++) {
+        if (q < 3) atomicMin(&scene[6 * kk + q], f2ord(v[q]));
+        else atomicMax(&scene[6 * kk + q], f2ord(v[q]));
+      }
+    }
+  } else if (act) {
+    for (int q = 0; q < 6; q++) {
+      if (q < 3) atomicMin(&scene[6 * k + q], f2ord(v[q]));
+      else atomicMax(&scene[6 * k + q], f2ord(v[q]));
+    }
+  }
+}
+__global__ void k_scene_init_b(int* scene, int M) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 6 * M) return;
+  scene[i] = (i % 6) < 3 ? 0x7FFFFFFF : (int)0x80000000;
+}
+// 64-bit keys: mesh index above the 30-bit Morton code of the box centre in
+// the mesh's own scene box
+__global__ void k_morton_b(const float4* __restrict__ lbox_lo, const float4* __restrict__ lbox_hi,
+                           const int* __restrict__ n_ptr, const int* __restrict__ leaf_face,
+                           const int* __restrict__ fmesh, const int* __restrict__ scene, u64* __restrict__ keys,
+                           int* __restrict__ vals) {
+  int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= *n_ptr) return;
+  int k = fmesh[leaf_face[l]];
+  const int* s = scene + 6 * k;
+  float4 lo = lbox_lo[l], hi = lbox_hi[l];
+  float c[3] = {0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
+  uint32_t q[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    float slo = ord2f(s[a]), shi = ord2f(s[3 + a]);
+    float ext = shi - slo;
+    float t = ext > 0.0f ? (c[a] - slo) / ext : 0.0f;
+    t = fminf(fmaxf(t * 1024.0f, 0.0f), 1023.0f);
+    q[a] = (uint32_t)t;
+  }
+  keys[l] = ((u64)(uint32_t)k << 30) | (u64)(expand10(q[0]) * 4 + expand10(q[1]) * 2 + expand10(q[2]));
+  vals[l] = l;
+}
+
+// stable LSD radix sort of (u64 key, int value), 8-bit digits (as
+// k_rs_hist/k_rs_scatter, 64-bit keys)
+__global__ void __launch_bounds__(BLOCK) k_rs_hist64(const u64* __restrict__ keys, const int* __restrict__ n_ptr,
+                                                     int shift, int ntiles, int* __restrict__ hist) {
+  __shared__ int hh[256];
+  int n = *n_ptr;
+  hh[threadIdx.x] = 0;
+  __syncthreads();
+  long long base = (long long)blockIdx.x * RS_TILE;
+  for (int i = 0; i < RS_ITEMS; i++) {
+    long long idx = base + (long long)i * BLOCK + threadIdx.x;
+    if (idx < n) atomicAdd(&hh[(int)((keys[idx] >> shift) & 255ull)], 1);
+  }
+  __syncthreads();
+  hist[(long long)threadIdx.x * ntiles + blockIdx.x] = hh[threadIdx.x];
+}
+__global__ void __launch_bounds__(BLOCK) k_rs_scatter64(const u64* __restrict__ kin, const int* __restrict__ vin,
+                                                        const int* __restrict__ n_ptr, int shift, int ntiles,
+                                                        const int* __restrict__ hoff, u64* __restrict__ kout,
+                                                        int* __restrict__ vout) {
+  __shared__ u64 sk[RS_TILE];
+  __shared__ int sv[RS_TILE];
+  __shared__ int lds[BLOCK / 64];
+  __shared__ int dstart[256];
+  __shared__ int goff[256];
+  int n = *n_ptr;
+  long long base = (long long)blockIdx.x * RS_TILE;
+  if (base >= n) return;
+  int cnt = (int)min((long long)RS_TILE, (long long)n - base);
+  u64 k[RS_ITEMS];
+  int v[RS_ITEMS];
+#pragma unroll
+  for (int i = 0; i < RS_ITEMS; i++) {
+    int li = threadIdx.x * RS_ITEMS + i;
+    if (li < cnt) { k[i] = kin[base + li]; v[i] = vin[base + li]; }
+    else { k[i] = ~0ull; v[i] = -1; }
+  }
+  goff[threadIdx.x] = hoff[(long long)threadIdx.x * ntiles + blockIdx.x];
+  for (int bit = 0; bit < 8; bit++) {
+    int ones = 0;
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; i++) ones += (int)((k[i] >> (shift + bit)) & 1ull);
+    int total_zeros;
+    int zbefore = block_exclusive_scan(RS_ITEMS - ones, lds, total_zeros);
+    int obefore = threadIdx.x * RS_ITEMS - zbefore;
+    int zc = 0, oc = 0;
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; i++) {
+      int b = (int)((k[i] >> (shift + bit)) & 1ull);
+      int dst = b ? (total_zeros + obefore + oc++) : (zbefore + zc++);
+      sk[dst] = k[i];
+      sv[dst] = v[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; i++) { k[i] = sk[threadIdx.x * RS_ITEMS + i]; v[i] = sv[threadIdx.x * RS_ITEMS + i]; }
+    __syncthreads();
+  }
+  dstart[threadIdx.x] = RS_TILE;
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < RS_ITEMS; i++) {
+    int li = threadIdx.x * RS_ITEMS + i;
+    if (li < cnt) {
+      int d = (int)((k[i] >> shift) & 255ull);
+      if (li == 0 || (int)((sk[li - 1] >> shift) & 255ull) != d) dstart[d] = li;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < RS_ITEMS; i++) {
+    int li = threadIdx.x * RS_ITEMS + i;
+    if (li < cnt) {
+      int d = (int)((k[i] >> shift) & 255ull);
+      long long o = (long long)goff[d] + (li - dstart[d]);
+      kout[o] = k[i];
+      vout[o] = v[i];
+    }
+  }
+}
+
+__device__ __forceinline__ int kdelta64(const u64* __restrict__ key, int n, int i, int j) {
+  if (j < 0 || j > n - 1) return -1;
+  u64 a = key[i], b = key[j];
+  if (a == b) return 64 + __clz((uint32_t)(i ^ j));
+  return __clzll((long long)(a ^ b));
+}
+__global__ void k_karras64(const u64* __restrict__ key, const int* __restrict__ n_ptr, int* __restrict__ left,
+                           int* __restrict__ right, int* __restrict__ parent, int* __restrict__ rfirst,
+                           int* __restrict__ rlast) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = *n_ptr;
+  if (i >= n - 1) return;
+  int d = (kdelta64(key, n, i, i + 1) - kdelta64(key, n, i, i - 1)) >= 0 ? 1 : -1;
+  int dmin = kdelta64(key, n, i, i - d);
+  int lmax = 2;
+  while (kdelta64(key, n, i, i + lmax * d) > dmin) lmax *= 2;
+  int l = 0;
+  for (int t = lmax / 2; t >= 1; t /= 2)
+    if (kdelta64(key, n, i, i + (l + t) * d) > dmin) l += t;
+  int j = i + l * d;
+  int dnode = kdelta64(key, n, i, j);
+  int s = 0;
+  for (int div = 2;; div *= 2) {
+    int t = (l + div - 1) / div;
+    if (kdelta64(key, n, i, i + (s + t) * d) > dnode) s += t;
+    if (t == 1) break;
+  }
+  int g = i + s * d + (d < 0 ? -1 : 0);
+  int lo = min(i, j), hi = max(i, j);
+  int lc = (lo == g) ? (n - 1 + g) : g;
+  int rc = (hi == g + 1) ? (n - 1 + g + 1) : (g + 1);
+  left[i] = lc;
+  right[i] = rc;
+  parent[lc] = i;
+  parent[rc] = i;
+  rfirst[i] = lo;
+  rlast[i] = hi;
+}
+// sorted leaf ranges per mesh: seg_a[k] / seg_b[k] (first / last sorted leaf)
+__global__ void k_seg_bounds(const u64* __restrict__ key, const int* __restrict__ n_ptr, int* __restrict__ seg_a,
+                             int* __restrict__ seg_b) {
+  int l = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = *n_ptr;
+  if (l >= n) return;
+  int k = (int)(key[l] >> 30);
+  if (l == 0 || (int)(key[l - 1] >> 30) != k) seg_a[k] = l;
+  if (l == n - 1 || (int)(key[l + 1] >> 30) != k) seg_b[k] = l;
+}
+// subtree root of mesh k (the node whose range is exactly [a, b]: a radix
+// tree has one, and a Karras node is numbered by one end of its range) and
+// the node after that subtree in depth-first order
+__global__ void k_mesh_roots(int M, const int* __restrict__ n_ptr, const int* __restrict__ seg_a,
+                             const int* __restrict__ seg_b, const int* __restrict__ rfirst,
+                             const int* __restrict__ rlast, int* __restrict__ root, int* __restrict__ root_end) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= M) return;
+  int n = *n_ptr;
+  int a = seg_a[k], b = seg_b[k];
+  if (a < 0) { root[k] = -1; root_end[k] = -1; return; }
+  int r;
+  if (a == b) r = n - 1 + a;
+  else if (a <= n - 2 && rfirst[a] == a && rlast[a] == b) r = a;
+  else r = b;
+  root[k] = r;
+  if (b == n - 1) root_end[k] = -1;
+  else root_end[k] = (b + 1 <= n - 2 && rfirst[b + 1] == b + 1) ? (b + 1) : (n - 1 + b + 1);
+}
+
+// traversal from the query's mesh root to the end of that subtree
+__global__ void __launch_bounds__(BLOCK) k_traverse_b(const int* __restrict__ qlist, const int* __restrict__ nq_ptr,
+                                                      const int* __restrict__ n_ptr, const int* __restrict__ fmesh,
+                                                      const int* __restrict__ root, const int* __restrict__ root_end,
+                                                      const float4* __restrict__ nodeA, const float4* __restrict__ nodeB,
+                                                      const float* __restrict__ P, const int* __restrict__ T,
+                                                      uint8_t* __restrict__ S, unsigned long long* __restrict__ stats,
+                                                      int* err) {
+  int qi = blockIdx.x * blockDim.x + threadIdx.x;
+  int nq = *nq_ptr;
+  unsigned cand = 0;
+  bool hit = false;
+  if (qi < nq) {
+    const long long cap = 2LL * (2LL * (*n_ptr) - 1) + 8;
+    int f = qlist[qi];
+    int k = fmesh[f];
+    I3 tf = ldt(T, f);
+    float pf[3][3];
+    {
+      V3 a = ldp(P, tf.i), b = ldp(P, tf.j), c = ldp(P, tf.k);
+      pf[0][0] = a.x; pf[0][1] = a.y; pf[0][2] = a.z;
+      pf[1][0] = b.x; pf[1][1] = b.y; pf[1][2] = b.z;
+      pf[2][0] = c.x; pf[2][1] = c.y; pf[2][2] = c.z;
+    }
+    float qlo[3], qhi[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      qlo[a] = fminf(pf[0][a], fminf(pf[1][a], pf[2][a]));
+      qhi[a] = fmaxf(pf[0][a], fmaxf(pf[1][a], pf[2][a]));
+    }
+    int node = root[k], end = root_end[k];
+    long long steps = 0;
+    while (node != end && node >= 0) {
+      if (++steps > cap) { atomicAdd(err, 1); break; }
+      float4 A = nodeA[node], B = nodeB[node];
+      bool ov = !(qhi[0] < A.x || B.x < qlo[0]) && !(qhi[1] < A.y || B.y < qlo[1]) && !(qhi[2] < A.z || B.z < qlo[2]);
+      int esc = __float_as_int(A.w);
+      if (ov) {
+        int c = __float_as_int(B.w);
+        if (c >= 0) { node = c; continue; }
+        int g = ~c;
+        if (g != f) {
+          cand++;
+          I3 tg = ldt(T, g);
+          float pg[3][3];
+          V3 a = ldp(P, tg.i), b = ldp(P, tg.j), cc = ldp(P, tg.k);
+          pg[0][0] = a.x; pg[0][1] = a.y; pg[0][2] = a.z;
+          pg[1][0] = b.x; pg[1][1] = b.y; pg[1][2] = b.z;
+          pg[2][0] = cc.x; pg[2][1] = cc.y; pg[2][2] = cc.z;
+          if (narrow_hit(pg, pf, tg, tf) || narrow_hit(pf, pg, tf, tg)) { hit = true; break; }
+        }
+      }
+      node = esc;
+    }
+    S[f] = hit ? 1 : 0;
+  }
+  if (stats) {
+    unsigned long long c64 = cand, h64 = hit ? 1ull : 0ull;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      c64 += __shfl_xor(c64, o, 64);
+      h64 += __shfl_xor(h64, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && (c64 || h64)) {
+      atomicAdd(&stats[0], c64);
+      atomicAdd(&stats[1], h64);
+    }
+  }
+}
+
+// --------------------------------------------------------------- marks
+__global__ void k_mark_b(const long long* __restrict__ cutoff, int M, const long long* __restrict__ eoffm,
+                         const int* __restrict__ coll, const int* __restrict__ edges,
+                         const int* __restrict__ fan_off, const int* __restrict__ fan, const int* __restrict__ part,
+                         const uint8_t* __restrict__ S, int* __restrict__ mflag) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long tot = cutoff[M];
+  if (i > tot) return;
+  if (i == tot) { mflag[tot] = 0; return; }
+  int k = find_seg(cutoff, M, i);
+  int hit = 0;
+  if (part[k]) {
+    int e = coll[2 * eoffm[k] + (i - cutoff[k])];
+    for (int w = 0; w < 2 && !hit; w++) {
+      int x = edges[2 * e + w];
+      for (int s = fan_off[x]; s < fan_off[x + 1]; s++) if (S[fan[s]]) { hit = 1; break; }
+    }
+  }
+  mflag[i] = hit;
+}
+// marked entries -> per-mesh marked list (flat, base cutoff[k]) and endpoint
+// pairs (per mesh, base 2 cutoff[k]); hm[k] = marks of mesh k
+__global__ void k_marked_b(const long long* __restrict__ cutoff, int M, const long long* __restrict__ eoffm,
+                           const int* __restrict__ coll, const int* __restrict__ edges, const int* __restrict__ mflag,
+                           const int* __restrict__ mpos, int* __restrict__ marked, int* __restrict__ tmp) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cutoff[M] || !mflag[i]) return;
+  int k = find_seg(cutoff, M, i);
+  long long j = mpos[i] - mpos[cutoff[k]];
+  int e = coll[2 * eoffm[k] + (i - cutoff[k])];
+  marked[cutoff[k] + j] = e;
+  tmp[2 * cutoff[k] + 2 * j] = edges[2 * e];
+  tmp[2 * cutoff[k] + 2 * j + 1] = edges[2 * e + 1];
+}
+__global__ void k_mark_counts(int M, const long long* __restrict__ cutoff, const int* __restrict__ mpos,
+                              int* __restrict__ hm, int* __restrict__ part, int* __restrict__ nvu,
+                              int* __restrict__ n_active) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= M) return;
+  int c = part[k] ? (mpos[cutoff[k + 1]] - mpos[cutoff[k]]) : 0;
+  hm[k] = c;
+  nvu[k] += 2 * c;
+  part[k] = c > 0 ? 1 : 0;
+  if (c > 0) atomicAdd(n_active, 1);
+}
+// undo of the marked edges of participating meshes (cusimp_free.cu:829-867)
+__global__ void k_undo_b(const long long* __restrict__ cutoff, int M, const int* __restrict__ hm,
+                         const int* __restrict__ marked, const int* __restrict__ edges, float* P,
+                         const float* __restrict__ P0, int* T, const int* __restrict__ T0, int* occ,
+                         const int* __restrict__ fan_off, const int* __restrict__ fan) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cutoff[M]) return;
+  int k = find_seg(cutoff, M, i);
+  long long j = i - cutoff[k];
+  if (j >= hm[k]) return;
+  int e = marked[cutoff[k] + j];
+  restore_edge(edges[2 * e], edges[2 * e + 1], P, P0, T, T0, occ, fan_off, fan);
+}
+// undo-list entries of this pass through the stale map (cusimp_free.cu:869-878)
+__global__ void k_undo_remap_b(const long long* __restrict__ cutoff, int M, const int* __restrict__ hm,
+                               const int* __restrict__ tmp, const long long* __restrict__ voff,
+                               const MeshParam* __restrict__ mp, const int* __restrict__ pm,
+                               const long long* __restrict__ uobase, int* __restrict__ first,
+                               int* __restrict__ undo_out, int* err) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 2 * cutoff[M]) return;
+  int k = find_seg(cutoff, M, i / 2);
+  long long j = i - 2 * cutoff[k];
+  if (j >= 2LL * hm[k]) return;
+  int x = tmp[2 * cutoff[k] + j] - (int)voff[k];   // local vertex index
+  int r = 0;
+  if (x >= 0 && (long long)x < mp[k].pmcap) r = pm[mp[k].pmo + x];
+  else atomicAdd(err, 1);
+  undo_out[uobase[k] + first[k] + j] = r;
+}
+__global__ void k_advance_first(int M, const int* __restrict__ hm, int* __restrict__ first) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < M) first[k] += 2 * hm[k];
+}
+// unwritten tail of each undo list (D6) -> 0
+__global__ void k_undo_tail(int M, const long long* __restrict__ uobase, const int* __restrict__ first,
+                            const int* __restrict__ nvu, int* __restrict__ undo_out) {
+  int k = blockIdx.x;
+  if (k >= M) return;
+  for (int j = first[k] + threadIdx.x; j < nvu[k]; j += blockDim.x) undo_out[uobase[k] + j] = 0;
+}
+
+// ---------------------------------------------------- end of round
+// local vertex maps into the mesh regions (the scan over occupancy is
+// global; local = global - value at the mesh start)
+__global__ void k_local_map(const int* __restrict__ g, const long long* __restrict__ voff, int M,
+                            const int* __restrict__ vmesh, const MeshParam* __restrict__ mp, const int* __restrict__ V_ptr,
+                            int* __restrict__ pm) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  int V = *V_ptr;
+  if (v >= V) return;
+  int k = vmesh[v];
+  pm[mp[k].pmo + (v - voff[k])] = g[v] - g[voff[k]];
+  if (v == voff[k + 1] - 1) pm[mp[k].pmo + (v - voff[k]) + 1] = g[v + 1] - g[voff[k]];
+}
+__global__ void k_compact_verts_b(const float* __restrict__ P, const int* __restrict__ occ, const int* __restrict__ g,
+                                  const int* __restrict__ vmesh, const int* __restrict__ V_ptr, float* __restrict__ outP,
+                                  int* __restrict__ outm) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= *V_ptr || !occ[v]) return;
+  int o = g[v];
+  outP[3 * o] = P[3 * v]; outP[3 * o + 1] = P[3 * v + 1]; outP[3 * o + 2] = P[3 * v + 2];
+  outm[o] = vmesh[v];
+}
+__global__ void k_face_keep_b(const int* __restrict__ T, const int* __restrict__ F_ptr, int* __restrict__ keep) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  int F = *F_ptr;
+  if (f > F) return;
+  keep[f] = f < F && T[3 * f] >= 0 ? 1 : 0;
+}
+__global__ void k_compact_faces_b(const int* __restrict__ T, const int* __restrict__ keep, const int* __restrict__ pos,
+                                  const int* __restrict__ g, const int* __restrict__ fmesh, const int* __restrict__ F_ptr,
+                                  int* __restrict__ outT, int* __restrict__ outm) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= *F_ptr || !keep[f]) return;
+  int o = pos[f];
+  outT[3 * o] = g[T[3 * f]]; outT[3 * o + 1] = g[T[3 * f + 1]]; outT[3 * o + 2] = g[T[3 * f + 2]];
+  outm[o] = fmesh[f];
+}
+// new per-mesh offsets and sizes
+__global__ void k_new_offsets(int M, const long long* __restrict__ voff, const long long* __restrict__ foff,
+                              const int* __restrict__ g, const int* __restrict__ fpos, long long* __restrict__ nvoff,
+                              long long* __restrict__ nfoff) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > M) return;
+  nvoff[k] = g[voff[k]];
+  nfoff[k] = fpos[foff[k]];
+}
+// next round's packed undo input from the per-mesh undo outputs
+__global__ void k_pack_undo(int M, const long long* __restrict__ uioff_new, const long long* __restrict__ uobase,
+                            const int* __restrict__ undo_out, int* __restrict__ undo_in) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= uioff_new[M]) return;
+  int k = find_seg(uioff_new, M, i);
+  undo_in[i] = undo_out[uobase[k] + (i - uioff_new[k])];
+}
+
+// ---------------------------------------------------------- retirement
+__global__ void k_faces_local(const int* __restrict__ T, long long n, int voff, int* __restrict__ out) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = T[i] - voff;
+}
+// keep[i] = 1 when element i belongs to a mesh that stays (keep[n] = 0)
+__global__ void k_keep_by_mesh(const int* __restrict__ mesh, long long n, const int* __restrict__ newk,
+                               int* __restrict__ keep) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  keep[i] = (i < n && newk[mesh[i]] >= 0) ? 1 : 0;
+}
+__global__ void k_retire_verts(const float* __restrict__ P, const int* __restrict__ vmesh, const int* __restrict__ keep,
+                               const int* __restrict__ pos, const int* __restrict__ newk, long long V,
+                               float* __restrict__ outP, int* __restrict__ outm) {
+  long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V || !keep[v]) return;
+  int o = pos[v];
+  outP[3LL * o] = P[3 * v]; outP[3LL * o + 1] = P[3 * v + 1]; outP[3LL * o + 2] = P[3 * v + 2];
+  outm[o] = newk[vmesh[v]];
+}
+__global__ void k_retire_faces(const int* __restrict__ T, const int* __restrict__ fmesh, const int* __restrict__ keep,
+                               const int* __restrict__ pos, const int* __restrict__ vpos, const int* __restrict__ newk,
+                               long long F, int* __restrict__ outT, int* __restrict__ outm) {
+  long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F || !keep[f]) return;
+  int o = pos[f];
+  outT[3LL * o] = vpos[T[3 * f]]; outT[3LL * o + 1] = vpos[T[3 * f + 1]]; outT[3LL * o + 2] = vpos[T[3 * f + 2]];
+  outm[o] = newk[fmesh[f]];
+}
+// undo inputs of kept meshes to their new packed offsets (newuo[k] < 0: dropped)
+__global__ void k_retire_undo2(const int* __restrict__ undo_in, const long long* __restrict__ uioff, int M,
+                               const long long* __restrict__ newuo, int* __restrict__ out) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= uioff[M]) return;
+  int k = find_seg(uioff, M, i);
+  if (newuo[k] < 0) return;
+  out[newuo[k] + (i - uioff[k])] = undo_in[i];
+}
+
+}  // namespace eng
+}  // namespace pamo

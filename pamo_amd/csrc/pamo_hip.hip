@@ -9,7 +9,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdarg>
+#include <initializer_list>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -17,6 +19,7 @@
 
 #include "../../include/pamo_hip.h"
 #include "pamo_bvh.h"
+#include "pamo_engine_kernels.h"
 #include "pamo_kernels.h"
 
 using namespace pamo;
@@ -30,7 +33,7 @@ struct Buf {
 
 enum Counter {
   C_E = 0, C_NFAN, C_NWIN, C_NCOLL, C_NLINE, C_CUT, C_NALIVE, C_NQUERY, C_NMARK, C_ERR, C_NEWV, C_NEWF,
-  C_COUNT = 16
+  C_V, C_F, C_CUTTOT, C_COUNT = 16
 };
 
 inline unsigned nblk(long long n) { return (unsigned)std::max<long long>(1, (n + BLOCK - 1) / BLOCK); }
@@ -48,7 +51,10 @@ inline uint32_t collapse_threshold(float tau) {
 
 }  // namespace
 
+struct Engine;
+
 struct pamo_ctx {
+  Engine* eng = nullptr;   // batched engine state (pamo_engine.inc)
   int device = 0;
   int shuffle_mode = PAMO_SHUFFLE_FEISTEL;
   hipStream_t st = nullptr;
@@ -68,6 +74,7 @@ struct pamo_ctx {
   Buf RV[2], RF[2], keep, kpos, undo_in;
   int run_cur = 0, run_V = 0, run_F = 0;
   std::vector<int> run_faces;   // faces after each round of the last run
+  int run_handle = -1;          // engine slot of the last pamo_run_stage2
 
   int* hc = nullptr;   // pinned host mirror of the counters
 
@@ -187,6 +194,17 @@ struct pamo_ctx {
     k_scan_down<<<(unsigned)nb, BLOCK, 0, st>>>(in, out, n, ptr<int>(scan_sums));
     return check_launch("scan");
   }
+  // the same scan over a capacity n whose live length is on the device
+  // (*d_n + add); entries past the live length count as zero
+  bool scan_dev(const int* in, int* out, long long n, const int* d_n, int add) {
+    if (n <= 0) return true;
+    long long nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (!ensure(scan_sums, (size_t)nb * 4)) return false;
+    k_scan_reduce_dev<<<(unsigned)nb, BLOCK, 0, st>>>(in, n, d_n, add, ptr<int>(scan_sums));
+    k_scan_top<<<1, BLOCK, 0, st>>>(ptr<int>(scan_sums), (int)nb, nullptr);
+    k_scan_down_dev<<<(unsigned)nb, BLOCK, 0, st>>>(in, out, n, d_n, add, ptr<int>(scan_sums));
+    return check_launch("scan");
+  }
   bool read_counters() {
     if (!hip(hipMemcpyAsync(hc, counters.p, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, st), "read counters"))
       return false;
@@ -219,9 +237,12 @@ struct pamo_ctx {
   ~pamo_ctx();
 };
 
+static void eng_free(pamo_ctx* c);
+
 pamo_ctx::~pamo_ctx() {
   (void)hipSetDevice(device);
   (void)hipDeviceSynchronize();
+  eng_free(this);
   Buf* all[] = {&pts_map, &table, &edge_cost, &undo_list, &tmp_undo, &P, &P0, &occ, &T, &T0, &cnt, &fan_off, &fan,
                 &ecnt, &eoff, &edges, &Q, &tri_min, &win, &winpos, &collapsed, &qflag, &qf, &qpos, &af, &apos,
                 &qlist, &leaf_face, &lbox_lo, &lbox_hi, &keys0, &keys1, &vals0, &vals1, &hist, &kleft, &kright,
@@ -515,8 +536,138 @@ int pamo_ctx::forward(const float* d_pts, const int* d_tris, const int* d_undo, 
   return 0;
 }
 
+#include "pamo_engine.inc"
+
+static void eng_free(pamo_ctx* c) {
+  if (!c->eng) return;
+  Engine& g = *c->eng;
+  Buf* all[] = {&g.P, &g.P0, &g.T, &g.T0, &g.vmesh, &g.fmesh, &g.nP, &g.nT, &g.nvm, &g.nfm, &g.undo_in, &g.undo_in2,
+                &g.undo_out, &g.tmp, &g.marked, &g.pm, &g.table, &g.ecost, &g.mp, &g.voff, &g.foff, &g.eoffm,
+                &g.uioff, &g.uioff2, &g.nustk, &g.h, &g.lcnt, &g.ncoll, &g.cut, &g.cutoff, &g.uobase, &g.hm, &g.part,
+                &g.first, &g.nvu, &g.nvoff, &g.nfoff, &g.seg_a, &g.seg_b, &g.root, &g.root_end, &g.scene,
+                &g.flat_cost, &g.g, &g.keep, &g.kpos, &g.newk, &g.vkeep, &g.vpos};
+  for (Buf* b : all)
+    if (b->p) (void)hipFree(b->p);
+  for (auto& s : g.slots) {
+    if (s.outV.p) (void)hipFree(s.outV.p);
+    if (s.outF.p) (void)hipFree(s.outF.p);
+  }
+  if (g.hb) (void)hipHostFree(g.hb);
+  delete c->eng;
+  c->eng = nullptr;
+}
+
+static bool eng_ready(pamo_ctx& c) {
+  if (!c.hip(hipSetDevice(c.device), "hipSetDevice")) return false;
+  if (!c.eng) c.eng = new Engine();
+  return c.ensure(c.counters, C_COUNT * 4) && c.ensure(c.cstats, 64 * 4);
+}
+
 // ====================================================================== ABI
 extern "C" {
+
+int pamo_batch_submit(pamo_ctx* ctx, void* stream, const float* d_verts, const int* d_faces, int n_verts, int n_faces,
+                      float scale, long long target, float threshold, int tolerance, long long iters, long long job) {
+  if (!ctx) return -1;
+  pamo_ctx& c = *ctx;
+  c.st = (hipStream_t)stream;
+  c.err.clear();
+  if (n_verts < 0 || n_faces < 0 || tolerance < 0 || iters < 1) { c.fail("invalid arguments"); return -1; }
+  if (!eng_ready(c)) return -1;
+  // faces must index [0, n_verts) (rows of -1 are removed faces)
+  if (!c.fill_i32(c.counters.p, 0, C_COUNT)) return -1;
+  k_validate<<<nblk(n_faces), BLOCK, 0, c.st>>>(d_faces, n_faces, n_verts, c.cnt_dev(C_ERR));
+  if (!c.check_launch("validate") || !c.read_counters()) return -1;
+  if (c.hc[C_ERR]) { c.fail("invalid input: %d face indices out of range", c.hc[C_ERR]); return -2; }
+  int h = eng_admit(c, d_verts, d_faces, n_verts, n_faces, scale, target, threshold, tolerance, iters, job);
+  if (h < 0) return -1;
+  return c.hip(hipStreamSynchronize(c.st), "sync") ? h : -1;
+}
+
+int pamo_batch_step(pamo_ctx* ctx, void* stream, int max_rounds, int* n_active) {
+  if (!ctx) return -1;
+  pamo_ctx& c = *ctx;
+  c.st = (hipStream_t)stream;
+  c.err.clear();
+  if (!eng_ready(c)) return -1;
+  int finished = 0;
+  for (int r = 0; r < max_rounds && c.eng->M > 0; r++) {
+    int before = c.eng->M;
+    if (!eng_round(c)) return -1;
+    finished += before - c.eng->M;
+  }
+  if (n_active) *n_active = c.eng->M;
+  return finished;
+}
+
+int pamo_batch_poll(pamo_ctx* ctx, int* handles, long long* jobs, int cap) {
+  if (!ctx || !ctx->eng) return 0;
+  int n = 0;
+  for (int h = 0; h < (int)ctx->eng->slots.size(); h++) {
+    const EngSlot& s = ctx->eng->slots[h];
+    if (!s.done || s.taken) continue;
+    if (n < cap) {
+      if (handles) handles[n] = h;
+      if (jobs) jobs[n] = s.job;
+    }
+    n++;
+  }
+  return n;
+}
+
+int pamo_batch_result(pamo_ctx* ctx, int handle, int* n_verts, int* n_faces, long long* stats, int* round_faces,
+                      int cap) {
+  if (!ctx || !ctx->eng || handle < 0 || handle >= (int)ctx->eng->slots.size()) return -1;
+  const EngSlot& s = ctx->eng->slots[handle];
+  if (!s.done) return 1;
+  if (n_verts) *n_verts = s.rV;
+  if (n_faces) *n_faces = s.rF;
+  if (stats) {
+    long long v[8] = {s.rounds, s.exit_reason, s.collapses, s.max_passes, (long long)s.job, s.V0, s.F0,
+                      (long long)((s.t_done - s.t_admit) * 1e6)};
+    memcpy(stats, v, sizeof(v));
+  }
+  if (round_faces)
+    for (int i = 0; i < (int)s.round_faces.size() && i < cap; i++) round_faces[i] = s.round_faces[i];
+  return 0;
+}
+
+int pamo_batch_take(pamo_ctx* ctx, void* stream, int handle, float* d_verts_out, int* d_faces_out) {
+  if (!ctx || !ctx->eng || handle < 0 || handle >= (int)ctx->eng->slots.size()) return -1;
+  pamo_ctx& c = *ctx;
+  c.st = (hipStream_t)stream;
+  EngSlot& s = c.eng->slots[handle];
+  if (!s.done || s.taken) { c.fail("job %d is not finished or already taken", handle); return -1; }
+  if (s.rV && d_verts_out &&
+      !c.hip(hipMemcpyAsync(d_verts_out, s.outV.p, (size_t)s.rV * 12, hipMemcpyDeviceToDevice, c.st), "take"))
+    return -1;
+  if (s.rF && d_faces_out &&
+      !c.hip(hipMemcpyAsync(d_faces_out, s.outF.p, (size_t)s.rF * 12, hipMemcpyDeviceToDevice, c.st), "take"))
+    return -1;
+  if (!c.hip(hipStreamSynchronize(c.st), "sync")) return -1;
+  (void)hipFree(s.outV.p);
+  (void)hipFree(s.outF.p);
+  s.outV = Buf();
+  s.outF = Buf();
+  s.taken = true;
+  return 0;
+}
+
+int pamo_batch_state(pamo_ctx* ctx, long long out[6]) {
+  if (!ctx) return -1;
+  long long v[6] = {0, 0, 0, 0, 0, 0};
+  if (ctx->eng) {
+    Engine& g = *ctx->eng;
+    v[0] = g.M;
+    v[1] = g.V;
+    v[2] = g.F;
+    v[3] = g.st_checks;
+    v[4] = g.st_queries;
+    v[5] = g.st_cand;
+  }
+  memcpy(out, v, sizeof(v));
+  return 0;
+}
 
 int pamo_ctx_create(pamo_ctx** out, int device, int shuffle_mode) {
   if (!out) return -1;
@@ -579,6 +730,36 @@ int pamo_run_stage2(pamo_ctx* ctx, void* stream, const float* d_verts, const int
                     float scale, long long target, float threshold, int tolerance, long long iters, int* out_V,
                     int* out_F, long long* stats) {
   if (!ctx) return -1;
+  pamo_ctx& c = *ctx;
+  int h = pamo_batch_submit(ctx, stream, d_verts, d_faces, n_verts, n_faces, scale, target, threshold, tolerance,
+                            iters, -1);
+  if (h < 0) return h;
+  while (!c.eng->slots[h].done) {
+    int na = 0;
+    if (pamo_batch_step(ctx, stream, 1, &na) < 0) return -1;
+    if (na == 0 && !c.eng->slots[h].done) { c.fail("engine stopped before the job finished"); return -1; }
+  }
+  const EngSlot& s = c.eng->slots[h];
+  c.run_faces = s.round_faces;
+  c.run_handle = h;
+  c.run_V = s.rV;
+  c.run_F = s.rF;
+  if (out_V) *out_V = s.rV;
+  if (out_F) *out_F = s.rF;
+  if (stats) {
+    Engine& g = *c.eng;
+    long long v[8] = {s.rounds, s.exit_reason, s.collapses, s.max_passes, g.st_checks, g.st_queries, g.st_cand,
+                      g.st_hits};
+    memcpy(stats, v, sizeof(v));
+  }
+  return 0;
+}
+
+int pamo_run_stage2_rounds(pamo_ctx* ctx, void* stream, const float* d_verts, const int* d_faces, int n_verts,
+                           int n_faces, float scale, long long target, float threshold, int tolerance, long long iters,
+                           int* out_V, int* out_F, long long* stats) {
+  if (!ctx) return -1;
+  ctx->run_handle = -1;
   pamo_ctx& c = *ctx;
   c.st = (hipStream_t)stream;
   c.err.clear();
@@ -659,6 +840,7 @@ int pamo_run_fetch(pamo_ctx* ctx, void* stream, float* d_verts_out, int* d_faces
   if (!ctx) return -1;
   pamo_ctx& c = *ctx;
   c.st = (hipStream_t)stream;
+  if (c.run_handle >= 0) return pamo_batch_take(ctx, stream, c.run_handle, d_verts_out, d_faces_out);
   if (c.run_V && d_verts_out &&
       !c.hip(hipMemcpyAsync(d_verts_out, c.RV[c.run_cur].p, (size_t)c.run_V * 12, hipMemcpyDeviceToDevice, c.st), "fetch"))
     return -1;

@@ -145,6 +145,51 @@ __global__ void __launch_bounds__(BLOCK) k_scan_down(const int* __restrict__ in,
   }
 }
 
+// device-count variants: live length min(n, *d_n + add)
+__global__ void __launch_bounds__(BLOCK) k_scan_reduce_dev(const int* __restrict__ in, long long n,
+                                                           const int* __restrict__ d_n, int add, int* __restrict__ sums) {
+  __shared__ int lds[BLOCK / 64];
+  long long live = d_n ? min(n, (long long)*d_n + add) : n;
+  long long base = (long long)blockIdx.x * SCAN_TILE;
+  int s = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; i++) {
+    long long idx = base + (long long)i * BLOCK + threadIdx.x;
+    if (idx < live) s += in[idx];
+  }
+  int tot;
+  block_exclusive_scan(s, lds, tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+__global__ void __launch_bounds__(BLOCK) k_scan_down_dev(const int* __restrict__ in, int* __restrict__ out, long long n,
+                                                         const int* __restrict__ d_n, int add,
+                                                         const int* __restrict__ sums) {
+  __shared__ int lds[BLOCK / 64];
+  __shared__ int tile[SCAN_TILE];
+  long long live = d_n ? min(n, (long long)*d_n + add) : n;
+  long long base = (long long)blockIdx.x * SCAN_TILE;
+  if (base >= live) return;
+  for (int i = 0; i < SCAN_ITEMS; i++) {
+    long long idx = base + (long long)i * BLOCK + threadIdx.x;
+    tile[i * BLOCK + threadIdx.x] = idx < live ? in[idx] : 0;
+  }
+  __syncthreads();
+  int loc[SCAN_ITEMS];
+  int s = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; i++) { loc[i] = s; s += tile[threadIdx.x * SCAN_ITEMS + i]; }
+  int tot;
+  int off = block_exclusive_scan(s, lds, tot) + sums[blockIdx.x];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; i++) tile[threadIdx.x * SCAN_ITEMS + i] = off + loc[i];
+  __syncthreads();
+  for (int i = 0; i < SCAN_ITEMS; i++) {
+    long long idx = base + (long long)i * BLOCK + threadIdx.x;
+    if (idx < live) out[idx] = tile[i * BLOCK + threadIdx.x];
+  }
+}
+
 // ------------------------------------------------------------ step 1
 // cusimp_free.cu:869-878 (stale pts_map remap) and :880-889 (table)
 __global__ void k_remap_undo(const int* __restrict__ tmp, const int* __restrict__ pts_map, long long map_n,
