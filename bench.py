@@ -122,7 +122,9 @@ def main():
                   data="synthetic (seeded SDF mesh generated in-process)",
                   config=dict(cfg, faces=F0, vertices=int(v.shape[0]), ratio=0.001, tolerance=4, threshold=1e-3,
                               min_verts=0, target_faces=target, rounds=st["rounds"], final_faces=int(out_f.shape[0]),
-                              exit_reason=st["exit_reason"], parallelism="single GPU"),
+                              exit_reason=st["exit_reason"], parallelism="single GPU",
+                              guard_checks=st["checks"], query_faces=st["query_faces"],
+                              candidate_pairs=st["candidate_pairs"]),
                   latency_ms=dict(p50=round(1e3 * float(np.percentile(times, 50)), 3),
                                   p99=round(1e3 * float(np.percentile(times, 99)), 3)),
                   roofline=roofline)

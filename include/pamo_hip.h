@@ -68,12 +68,57 @@ int pamo_run_stage2(pamo_ctx* ctx, void* stream, const float* d_verts, const int
                     float scale, long long target, float threshold, int tolerance, long long iters, int* out_V,
                     int* out_F, long long* stats);
 
+/* The same loop as pamo_run_stage2 driven round by round from the host over
+ * pamo_free_forward (the exact call sequence of simp_cuda/pamo/__init__.py:
+ * 131-158: forward, then the compaction of lines 140-148).  Same arguments,
+ * outputs and stats; kept as the second implementation the device loop is
+ * tested against. */
+int pamo_run_stage2_rounds(pamo_ctx* ctx, void* stream, const float* d_verts, const int* d_faces, int n_verts,
+                           int n_faces, float scale, long long target, float threshold, int tolerance,
+                           long long iters, int* out_V, int* out_F, long long* stats);
+
 /* Face count after each round of the last pamo_run_stage2 (host array of
  * cap entries); returns the number of rounds. */
 int pamo_run_rounds(pamo_ctx* ctx, int* host_faces, int cap);
 
 /* Copies the mesh of the last pamo_run_stage2 into caller device buffers. */
 int pamo_run_fetch(pamo_ctx* ctx, void* stream, float* d_verts_out, int* d_faces_out);
+
+/* Batched stage 2 (no counterpart in the reference, which simplifies one mesh
+ * per PaMO.run call, simp_cuda/pamo/__init__.py:107-165).  Many meshes share
+ * one packed device state; every round of pamo_batch_step runs the loop body
+ * of PaMO.run for all active meshes in shared launches, and each mesh's
+ * result is bit-identical to running it alone with pamo_run_stage2.
+ *
+ * pamo_batch_submit copies one mesh in (centred vertices, faces, scale and
+ * the run parameters of pamo_run_stage2; job is an opaque caller tag) and
+ * returns its handle (>= 0).  The mesh joins at the next round, also while
+ * others are mid-run. */
+int pamo_batch_submit(pamo_ctx* ctx, void* stream, const float* d_verts, const int* d_faces, int n_verts, int n_faces,
+                      float scale, long long target, float threshold, int tolerance, long long iters, long long job);
+
+/* Runs up to max_rounds rounds of every active mesh; meshes that meet their
+ * exit condition are retired between rounds.  Returns the number retired;
+ * n_active receives the meshes still running. */
+int pamo_batch_step(pamo_ctx* ctx, void* stream, int max_rounds, int* n_active);
+
+/* Finished, not yet taken jobs: writes up to cap (handle, job) entries and
+ * returns how many there are. */
+int pamo_batch_poll(pamo_ctx* ctx, int* handles, long long* jobs, int cap);
+
+/* Result sizes of a job (returns 1 while it still runs).  stats: [0] rounds
+ * [1] exit reason [2] collapses [3] undo passes (max) [4] job tag [5] input
+ * vertices [6] input faces [7] microseconds from admission to retirement;
+ * round_faces receives up to cap per-round face counts. */
+int pamo_batch_result(pamo_ctx* ctx, int handle, int* n_verts, int* n_faces, long long* stats, int* round_faces,
+                      int cap);
+
+/* Copies a finished job's mesh into caller device buffers and frees it. */
+int pamo_batch_take(pamo_ctx* ctx, void* stream, int handle, float* d_verts_out, int* d_faces_out);
+
+/* Engine totals: [0] active meshes [1] packed vertices [2] packed faces
+ * [3] self-intersection checks [4] query faces [5] candidate pairs. */
+int pamo_batch_state(pamo_ctx* ctx, long long out[6]);
 
 /* Kernel timing (no counterpart in the reference, whose only timing is
  * wall-clock prints, simp_cuda/pamo/__init__.py:121,160-163): HIP events

@@ -36,39 +36,21 @@ __device__ __forceinline__ void plane_of(double pl[4], const float t[3][3]) {
   crossd(pl, ba, bc);
   pl[3] = -dotdf(pl, t[1]);
 }
-// tri_tri_3d.cuh:100-129
-__device__ __forceinline__ bool is_coplanar_ref(const float a[3][3], const float b[3][3]) {
-  float scale = 0.0f;
-#pragma unroll
-  for (int i = 0; i < 3; i++)
-#pragma unroll
-    for (int j = 0; j < 3; j++) scale = fmaxf(scale, fabsf(a[i][j]));
-#pragma unroll
-  for (int i = 0; i < 3; i++)
-#pragma unroll
-    for (int j = 0; j < 3; j++) scale = fmaxf(scale, fabsf(b[i][j]));
-  double pl[4];
-  plane_of(pl, a);
-  double s0 = dotdf(pl, b[0]) + pl[3];
-  double s1 = dotdf(pl, b[1]) + pl[3];
-  double s2 = dotdf(pl, b[2]) + pl[3];
-  double eps = (double)scale * 1e-6;
-  return fabs(s0) <= eps && fabs(s1) <= eps && fabs(s2) <= eps;
-}
 __device__ __forceinline__ void interp3(float r[3], const float a[3], const float b[3], float t) {
   const float s = 1.0f - t;
   r[0] = s * a[0] + t * b[0];
   r[1] = s * a[1] + t * b[1];
   r[2] = s * a[2] + t * b[2];
 }
-// tri_tri_3d.cuh:131-276
-__device__ bool isect_tri_tri_ref(const float ta[3][3], const float tb[3][3], float ri1[3], float ri2[3]) {
+// tri_tri_3d.cuh:131-276, with the first plane and the sides of the second
+// triangle passed in (the coplanarity test computes the same values)
+__device__ bool isect_tri_tri_ref(const float ta[3][3], const float tb[3][3], const double pa[4], const double sd[3],
+                                  float ri1[3], float ri2[3]) {
   float mn[2], mx[2], loc[2][2][3];
   float side[2][3];
-  double pa[4], pb[4];
-  plane_of(pa, ta);
+  double pb[4];
 #pragma unroll
-  for (int k = 0; k < 3; k++) side[1][k] = (float)(dotdf(pa, tb[k]) + pa[3]);
+  for (int k = 0; k < 3; k++) side[1][k] = (float)sd[k];
   if (!side[1][0] && !side[1][1] && !side[1][2]) return false;
   if ((side[1][0] && side[1][1] && side[1][2]) && (side[1][0] < 0.0f) == (side[1][1] < 0.0f) &&
       (side[1][0] < 0.0f) == (side[1][2] < 0.0f))
@@ -130,8 +112,10 @@ __device__ bool isect_tri_tri_ref(const float ta[3][3], const float tb[3][3], fl
 }
 
 // One ordered pair (current c, query q): self_intersect.cuh:233-378.
-// cur/qry are the raw (untranslated) vertex positions.
-__device__ bool narrow_hit(const float cur[3][3], const float qry[3][3], I3 ci, I3 qi) {
+// cur/qry are the raw (untranslated) vertex positions; nc the number of
+// equal vertex-index pairs.  Callers skip nc == 2 (both branches of the
+// reference return false for it, :285-286 and :338-339).
+__device__ bool narrow_hit(const float cur[3][3], const float qry[3][3], int nc) {
   float ta[3][3], tb[3][3];
   int mean[3];
 #pragma unroll
@@ -139,6 +123,7 @@ __device__ bool narrow_hit(const float cur[3][3], const float qry[3][3], I3 ci, 
     float m = (cur[0][k] + cur[1][k] + cur[2][k] + qry[0][k] + qry[1][k] + qry[2][k]) / 6.0f;
     mean[k] = (int)floorf(m);
   }
+  float scale = 0.0f;
 #pragma unroll
   for (int v = 0; v < 3; v++)
 #pragma unroll
@@ -146,30 +131,39 @@ __device__ bool narrow_hit(const float cur[3][3], const float qry[3][3], I3 ci, 
       ta[v][k] = cur[v][k] - (float)mean[k];
       tb[v][k] = qry[v][k] - (float)mean[k];
     }
-  int cv[3] = {ci.i, ci.j, ci.k}, qv[3] = {qi.i, qi.j, qi.k};
-  int nc = 0;
+  // is_coplanar (tri_tri_3d.cuh:100-129): the same plane and sides feed the
+  // interval test
 #pragma unroll
-  for (int j = 0; j < 3; j++)
+  for (int i = 0; i < 3; i++)
 #pragma unroll
-    for (int k = 0; k < 3; k++) nc += cv[j] == qv[k];
-  if (is_coplanar_ref(ta, tb)) return false;
+    for (int j = 0; j < 3; j++) scale = fmaxf(scale, fabsf(ta[i][j]));
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) scale = fmaxf(scale, fabsf(tb[i][j]));
+  double pa[4], sd[3];
+  plane_of(pa, ta);
+#pragma unroll
+  for (int k = 0; k < 3; k++) sd[k] = dotdf(pa, tb[k]) + pa[3];
+  const double eps = (double)scale * 1e-6;
+  if (fabs(sd[0]) <= eps && fabs(sd[1]) <= eps && fabs(sd[2]) <= eps) return false;
   if (nc == 2) return false;
   {
-    const float eps = 1e-6f;
+    const float e6 = 1e-6f;
     int sh = 0;
 #pragma unroll
     for (int v = 0; v < 3; v++) {
       bool any = false;
 #pragma unroll
       for (int w = 0; w < 3; w++)
-        any = any || (fabsf(ta[v][0] - tb[w][0]) < eps && fabsf(ta[v][1] - tb[w][1]) < eps &&
-                      fabsf(ta[v][2] - tb[w][2]) < eps);
+        any = any || (fabsf(ta[v][0] - tb[w][0]) < e6 && fabsf(ta[v][1] - tb[w][1]) < e6 &&
+                      fabsf(ta[v][2] - tb[w][2]) < e6);
       sh += any;
     }
     if (sh >= 2) return false;
   }
   float r1[3], r2[3];
-  if (!isect_tri_tri_ref(ta, tb, r1, r2)) return false;
+  if (!isect_tri_tri_ref(ta, tb, pa, sd, r1, r2)) return false;
   float dx = fabsf(r2[0] - r1[0]), dy = fabsf(r2[1] - r1[1]), dz = fabsf(r2[2] - r1[2]);
   float ld = -INFINITY;
   if (dx > ld) ld = dx;
@@ -177,6 +171,21 @@ __device__ bool narrow_hit(const float cur[3][3], const float qry[3][3], I3 ci, 
   if (dz > ld) ld = dz;
   if (ld < 1e-3f && nc == 1) return false;
   return true;
+}
+__device__ __forceinline__ int shared_indices(I3 a, I3 b) {
+  int av[3] = {a.i, a.j, a.k}, bv[3] = {b.i, b.j, b.k};
+  int nc = 0;
+#pragma unroll
+  for (int j = 0; j < 3; j++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) nc += av[j] == bv[k];
+  return nc;
+}
+// both orientations of a candidate pair (query f, other g)
+__device__ __forceinline__ bool pair_hits(const float pf[3][3], const float pg[3][3], I3 tf, I3 tg) {
+  int nc = shared_indices(tg, tf);
+  if (nc == 2) return false;
+  return narrow_hit(pg, pf, nc) || narrow_hit(pf, pg, nc);
 }
 
 // ------------------------------------------------------------- LBVH
@@ -397,37 +406,98 @@ __global__ void k_karras(const uint32_t* __restrict__ key, const int* __restrict
   rlast[i] = hi;
 }
 
-// leaves -> nodes, then bottom-up merge; the second child to arrive merges
-__global__ void __launch_bounds__(BLOCK) k_bottom_up(const int* __restrict__ n_ptr, const float4* __restrict__ lbox_lo,
-                                                     const float4* __restrict__ lbox_hi, const int* __restrict__ sorted_leaf,
-                                                     const int* __restrict__ leaf_face, const int* __restrict__ parent,
-                                                     const int* __restrict__ left, const int* __restrict__ right,
-                                                     const int* __restrict__ rfirst, const int* __restrict__ rlast,
-                                                     int* flags, float4* nodeA, float4* nodeB) {
-  int l = blockIdx.x * blockDim.x + threadIdx.x;
-  int n = *n_ptr;
-  if (l >= n) return;
+// Bottom-up node boxes in two phases.  Phase 1: one block per BLOCK sorted
+// leaves; nodes whose leaf range lies inside the block are merged through
+// LDS flags (workgroup scope: the second child to arrive merges).  Phase 2:
+// the few nodes whose range spans blocks, from their in-block children,
+// through agent-scope acq_rel flags in global memory.
+__device__ __forceinline__ int escape_after(int last, int n, const int* __restrict__ rfirst) {
+  // next node in depth-first order after a subtree ending at sorted leaf last
+  if (last == n - 1) return -1;
+  return (last + 1 <= n - 2 && rfirst[last + 1] == last + 1) ? (last + 1) : (n - 1 + last + 1);
+}
+__device__ __forceinline__ void node_range(int x, int n, const int* __restrict__ rfirst,
+                                           const int* __restrict__ rlast, int& a, int& b) {
+  if (x >= n - 1) { a = b = x - (n - 1); }
+  else { a = rfirst[x]; b = rlast[x]; }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_bottom_up_local(const int* __restrict__ n_ptr,
+                                                           const float4* __restrict__ lbox_lo,
+                                                           const float4* __restrict__ lbox_hi,
+                                                           const int* __restrict__ sorted_leaf,
+                                                           const int* __restrict__ leaf_face,
+                                                           const int* __restrict__ parent, const int* __restrict__ left,
+                                                           const int* __restrict__ right, const int* __restrict__ rfirst,
+                                                           const int* __restrict__ rlast, float4* nodeA, float4* nodeB) {
+  __shared__ int flag[BLOCK];
+  __shared__ float blo[2 * BLOCK][3], bhi[2 * BLOCK][3];   // [0,B): leaves, [B,2B): internal nodes
+  const int n = *n_ptr;
+  const int b0 = blockIdx.x * BLOCK;
+  if (b0 >= n) return;
+  const int b1 = min(b0 + BLOCK, n) - 1;
+  flag[threadIdx.x] = 0;
+  const int l = b0 + threadIdx.x;
+  if (l <= b1) {
+    int src = sorted_leaf[l];
+    float4 lo = lbox_lo[src], hi = lbox_hi[src];
+    blo[threadIdx.x][0] = lo.x; blo[threadIdx.x][1] = lo.y; blo[threadIdx.x][2] = lo.z;
+    bhi[threadIdx.x][0] = hi.x; bhi[threadIdx.x][1] = hi.y; bhi[threadIdx.x][2] = hi.z;
+    nodeA[n - 1 + l] = make_float4(lo.x, lo.y, lo.z, __int_as_float(escape_after(l, n, rfirst)));
+    nodeB[n - 1 + l] = make_float4(hi.x, hi.y, hi.z, __int_as_float(~leaf_face[src]));
+  }
+  __syncthreads();
+  if (l > b1 || n == 1) return;
   int node = n - 1 + l;
-  int src = sorted_leaf[l];
-  float4 lo = lbox_lo[src], hi = lbox_hi[src];
-  // escape of the leaf: next subtree starting at l + 1
-  int esc;
-  if (l == n - 1) esc = -1;
-  else esc = (l + 1 <= n - 2 && rfirst[l + 1] == l + 1) ? (l + 1) : (n - 1 + l + 1);
-  nodeA[node] = make_float4(lo.x, lo.y, lo.z, __int_as_float(esc));
-  nodeB[node] = make_float4(hi.x, hi.y, hi.z, __int_as_float(~leaf_face[src]));
-  if (n == 1) return;
-  int p = parent[node];
+  for (int depth = 0; depth < n; depth++) {
+    int p = parent[node];
+    if (p < 0) return;
+    int pa = rfirst[p], pb = rlast[p];
+    if (pa < b0 || pb > b1) return;   // spans blocks: phase 2
+    int old = __hip_atomic_fetch_add(&flag[p - b0], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (old == 0) return;
+    int lc = left[p], rc = right[p];
+    int li = lc >= n - 1 ? (lc - (n - 1) - b0) : (BLOCK + lc - b0);
+    int ri = rc >= n - 1 ? (rc - (n - 1) - b0) : (BLOCK + rc - b0);
+    float lo[3], hi[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      lo[k] = fminf(blo[li][k], blo[ri][k]);
+      hi[k] = fmaxf(bhi[li][k], bhi[ri][k]);
+    }
+    int pi = BLOCK + p - b0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) { blo[pi][k] = lo[k]; bhi[pi][k] = hi[k]; }
+    nodeA[p] = make_float4(lo[0], lo[1], lo[2], __int_as_float(escape_after(pb, n, rfirst)));
+    nodeB[p] = make_float4(hi[0], hi[1], hi[2], __int_as_float(lc));
+    node = p;
+  }
+}
+
+// phase 2: one thread per node; nodes inside one block whose parent spans
+// blocks walk up; flags of internal nodes start at zero
+__global__ void __launch_bounds__(BLOCK) k_bottom_up_global(const int* __restrict__ n_ptr,
+                                                            const int* __restrict__ parent,
+                                                            const int* __restrict__ left, const int* __restrict__ right,
+                                                            const int* __restrict__ rfirst,
+                                                            const int* __restrict__ rlast, int* flags, float4* nodeA,
+                                                            float4* nodeB) {
+  const int n = *n_ptr;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < 2 || x >= 2 * n - 1) return;
+  int a, b;
+  node_range(x, n, rfirst, rlast, a, b);
+  if (a / BLOCK != b / BLOCK) return;          // itself spans blocks
+  int p = parent[x];
+  if (p < 0) return;
+  if (rfirst[p] / BLOCK == rlast[p] / BLOCK) return;   // parent merged in phase 1
   for (int depth = 0; depth < n; depth++) {
     int old = __hip_atomic_fetch_add(&flags[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     if (old == 0) return;
     int lc = left[p], rc = right[p];
     float4 a0 = nodeA[lc], b0 = nodeB[lc], a1 = nodeA[rc], b1 = nodeB[rc];
-    int last = rlast[p];
-    int pesc;
-    if (last == n - 1) pesc = -1;
-    else pesc = (last + 1 <= n - 2 && rfirst[last + 1] == last + 1) ? (last + 1) : (n - 1 + last + 1);
-    nodeA[p] = make_float4(fminf(a0.x, a1.x), fminf(a0.y, a1.y), fminf(a0.z, a1.z), __int_as_float(pesc));
+    nodeA[p] = make_float4(fminf(a0.x, a1.x), fminf(a0.y, a1.y), fminf(a0.z, a1.z),
+                           __int_as_float(escape_after(rlast[p], n, rfirst)));
     nodeB[p] = make_float4(fmaxf(b0.x, b1.x), fmaxf(b0.y, b1.y), fmaxf(b0.z, b1.z), __int_as_float(lc));
     if (p == 0) return;
     p = parent[p];
@@ -484,7 +554,7 @@ __global__ void __launch_bounds__(BLOCK) k_traverse(const int* __restrict__ qlis
           pg[0][0] = a.x; pg[0][1] = a.y; pg[0][2] = a.z;
           pg[1][0] = b.x; pg[1][1] = b.y; pg[1][2] = b.z;
           pg[2][0] = cc.x; pg[2][1] = cc.y; pg[2][2] = cc.z;
-          if (narrow_hit(pg, pf, tg, tf) || narrow_hit(pf, pg, tf, tg)) { hit = true; break; }
+          if (pair_hits(pf, pg, tf, tg)) { hit = true; break; }
         }
       }
       node = esc;
@@ -548,7 +618,7 @@ __global__ void k_traverse_list(const int* __restrict__ qlist, const int* __rest
           pg[2][0] = cc.x; pg[2][1] = cc.y; pg[2][2] = cc.z;
           pairs[3 * (o + cnt)] = f;
           pairs[3 * (o + cnt) + 1] = g;
-          pairs[3 * (o + cnt) + 2] = narrow_hit(pg, pf, tg, tf) ? 1 : 0;
+          pairs[3 * (o + cnt) + 2] = (shared_indices(tg, tf) != 2 && narrow_hit(pg, pf, shared_indices(tg, tf))) ? 1 : 0;
         }
         cnt++;
       }

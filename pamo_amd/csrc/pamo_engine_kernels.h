@@ -27,6 +27,9 @@ struct MeshParam {
   int stuck;
   long long pmo, eco;     // persistent region offsets (map/table, edge costs)
   long long pmcap;        // map/table region entries (alloc_pts + 1)
+  float box[6];           // vertex bounds at admission (lo xyz, hi xyz): every
+                          // later position is a midpoint of earlier ones or a
+                          // restored original, so the box bounds the mesh
 };
 
 __device__ __forceinline__ int find_seg(const long long* __restrict__ off, int M, long long i) {
@@ -44,10 +47,12 @@ __device__ __forceinline__ int find_seg(const long long* __restrict__ off, int M
 // (cusimp_free.cu:905-908); one block per mesh
 __global__ void k_clear_tables(const MeshParam* __restrict__ mp, const long long* __restrict__ uioff, int M,
                                uint8_t* __restrict__ table) {
-  int k = blockIdx.x;
+  int k = blockIdx.y;
   if (k >= M) return;
   if (mp[k].stuck || uioff[k + 1] == uioff[k]) return;
-  for (long long i = threadIdx.x; i < mp[k].pmcap; i += blockDim.x) table[mp[k].pmo + i] = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < mp[k].pmcap;
+       i += (long long)gridDim.x * blockDim.x)
+    table[mp[k].pmo + i] = 0;
 }
 // every incoming undo entry marks its mesh's table (cusimp_free.cu:910-916);
 // bounds-checked (the reference is not)
@@ -98,11 +103,102 @@ __global__ void k_shuffle_scatter_off(Feistel fb, long long n, long long m, cons
   for (int c = 0; c < 3; c++) T[3LL * o + c] = tin[3 * k + c] + voff;
 }
 
+// ------------------------------------------------------ face geometry
+// per alive face: unit normal and plane offset (tri2plane,
+// cusimp_free.cu:337-346, also the old normal of the flip test :520-521)
+// and the three edge lengths |Pi-Pj|, |Pj-Pk|, |Pk-Pi| (:451-483); computed
+// once per round with the reference's expressions
+__global__ void __launch_bounds__(BLOCK) k_face_geom(const float* __restrict__ P, const int* __restrict__ T, int F,
+                                                     float4* __restrict__ fplane, float4* __restrict__ flen) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  I3 t = ldt(T, f);
+  if (dead3(t)) return;
+  V3 v0 = ldp(P, t.i), v1 = ldp(P, t.j), v2 = ldp(P, t.k);
+  V3 n = vcross(vsub(v1, v0), vsub(v2, v0));
+  n = vdivs(n, vnorm(n));
+  float off = -vdot(n, v0);
+  fplane[f] = make_float4(n.x, n.y, n.z, off);
+  flen[f] = make_float4(vnorm(vsub(v0, v1)), vnorm(vsub(v1, v2)), vnorm(vsub(v2, v0)), 0.0f);
+}
+// fan sort (ascending face index, D0) plus the successor of the fan's vertex
+// in each face (the link-condition vertex, cusimp_free.cu:407-430)
+__global__ void k_fan_sort_succ(const int* __restrict__ fan_off, int* __restrict__ fan, const int* __restrict__ T,
+                                int V, int* __restrict__ fsucc) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  int a = fan_off[v], b = fan_off[v + 1];
+  for (int i = a + 1; i < b; i++) {
+    int x = fan[i];
+    int j = i - 1;
+    while (j >= a && fan[j] > x) { fan[j + 1] = fan[j]; j--; }
+    fan[j + 1] = x;
+  }
+  for (int i = a; i < b; i++) fsucc[i] = succ_of(ldt(T, fan[i]), v);
+}
+// cusimp_free.cu:348-366 from the per-face planes
+__global__ void __launch_bounds__(BLOCK) k_quadric_g(const float4* __restrict__ fplane, const int* __restrict__ fan_off,
+                                                     const int* __restrict__ fan, int V, float* __restrict__ Q) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  float q[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  int a = fan_off[v], b = fan_off[v + 1];
+  for (int s = a; s < b; s++) {
+    float4 p = fplane[fan[s]];
+    q[0] += p.x * p.x; q[1] += p.x * p.y; q[2] += p.x * p.z; q[3] += p.x * p.w;
+    q[4] += p.y * p.y; q[5] += p.y * p.z; q[6] += p.y * p.w;
+    q[7] += p.z * p.z; q[8] += p.z * p.w;
+    q[9] += p.w * p.w;
+  }
+#pragma unroll
+  for (int i = 0; i < 10; i++) Q[(long long)10 * v + i] = q[i];
+}
+// fan edge-length sum from the per-face lengths (cusimp_free.cu:449-484)
+__device__ __forceinline__ float fan_len_g(const int* __restrict__ T, const float4* __restrict__ flen,
+                                           const int* __restrict__ fan, int a, int b, float el, int& ne) {
+  for (int s = a; s < b; s++) {
+    int f = fan[s];
+    I3 t = ldt(T, f);
+    float4 L = flen[f];
+    if (t.i > t.j) { el += L.x; ne++; }
+    if (t.j > t.k) { el += L.y; ne++; }
+    if (t.k > t.i) { el += L.z; ne++; }
+  }
+  return el;
+}
+// skinny/flip pass (cusimp_free.cu:497-575) with the per-face old normals
+__device__ bool skinny_pass_g(const float* __restrict__ P, const int* __restrict__ T,
+                              const float4* __restrict__ fplane, const int* __restrict__ fan, int a, int b, int moved,
+                              int other, V3 m, float& Qa, int& nt) {
+  for (int s = a; s < b; s++) {
+    int f = fan[s];
+    I3 t = ldt(T, f);
+    if (t.i == other || t.j == other || t.k == other) continue;
+    V3 o0 = ldp(P, t.i), o1 = ldp(P, t.j), o2 = ldp(P, t.k);
+    V3 n0 = t.i == moved ? m : o0, n1 = t.j == moved ? m : o1, n2 = t.k == moved ? m : o2;
+    float4 op = fplane[f];
+    V3 on = {op.x, op.y, op.z};
+    V3 nn = vcross(vsub(n1, n0), vsub(n2, n0));
+    nn = vdivs(nn, vnorm(nn));
+    if (vdot(on, nn) < 0) return false;
+    float e01 = vnorm(vsub(n0, n1)), e12 = vnorm(vsub(n1, n2)), e20 = vnorm(vsub(n2, n0));
+    float sh = (e01 + e12 + e20) / 2;
+    float area = sqrtf(sh * (sh - e01) * (sh - e12) * (sh - e20));
+    const float k43 = 4.0f * 1.73205077648162841796875f;   // 4.0f * sqrtf(3.0f)
+    float term = k43 * area / (e01 * e01) + e12 * e12 + e20 * e20 + 0.0000001f;
+    Qa += 1.0f - clamp_ref(term, 0.0f, 1.0f);
+    nt++;
+  }
+  return true;
+}
+
 // ----------------------------------------------------------- edge cost
 // as k_edge_cost with per-mesh parameters; the table and the (persistent)
 // cost array are addressed by local indices in the mesh's regions
 __global__ void __launch_bounds__(BLOCK) k_edge_cost_b(const float* __restrict__ P, const int* __restrict__ T,
                                                        const int* __restrict__ fan_off, const int* __restrict__ fan,
+                                                       const int* __restrict__ fsucc, const float4* __restrict__ fplane,
+                                                       const float4* __restrict__ flen,
                                                        const int* __restrict__ edges, const int* __restrict__ E_ptr,
                                                        const float* __restrict__ Q, const uint8_t* __restrict__ table,
                                                        uint32_t* __restrict__ cost, const int* __restrict__ vmesh,
@@ -113,34 +209,32 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_b(const float* __restrict__
   if (e >= *E_ptr) return;
   int u = edges[2 * e], v = edges[2 * e + 1];
   int k = vmesh[u];
-  const MeshParam prm = mp[k];
+  const long long pmo = mp[k].pmo;
   long long lu = u - voff[k], lv = v - voff[k];
-  uint32_t* c = cost + prm.eco + (e - eoffm[k]);
-  if (table[prm.pmo + lu] && table[prm.pmo + lv]) { *c = 0xFFFFFFFFu; return; }
-  if (prm.stuck) return;
-  V3 v0 = ldp(P, u), v1 = ldp(P, v);
+  uint32_t* c = cost + mp[k].eco + (e - eoffm[k]);
+  if (table[pmo + lu] && table[pmo + lv]) { *c = 0xFFFFFFFFu; return; }
+  if (mp[k].stuck) return;
   int ua = fan_off[u], ub = fan_off[u + 1], va = fan_off[v], vb = fan_off[v + 1];
+  // link condition: count of equal successor pairs, max as soon as it passes 2
   int dup = 0;
   for (int a = ua; a < ub; a++) {
-    int iu = succ_of(ldt(T, fan[a]), u);
-    for (int b = va; b < vb; b++) {
-      int iv = succ_of(ldt(T, fan[b]), v);
-      if (iu == iv) dup++;
-      if (dup > 2) { *c = 0xFFFFFFFFu; return; }
-    }
+    int iu = fsucc[a];
+    for (int b = va; b < vb; b++) dup += fsucc[b] == iu;
+    if (dup > 2) { *c = 0xFFFFFFFFu; return; }
   }
   if (dup != 2) return;
-  const float scale = prm.scale, tau = prm.tau;
+  const float scale = mp[k].scale, tau = mp[k].tau;
+  V3 v0 = ldp(P, u), v1 = ldp(P, v);
   int ne = 0;
-  float el = fan_edge_len(P, T, fan_off, fan, u, 0.0f, ne);
-  el = fan_edge_len(P, T, fan_off, fan, v, el, ne);
+  float el = fan_len_g(T, flen, fan, ua, ub, 0.0f, ne);
+  el = fan_len_g(T, flen, fan, va, vb, el, ne);
   el = el / (float)ne / scale * tau;
   el = el + vnorm(vsub(v0, v1)) / scale * tau;
   V3 m = vdivs(vadd(v0, v1), 2.0f);
   float Qa = 0.0f;
   int nt = 0;
-  if (!skinny_pass(P, T, fan_off, fan, u, v, m, Qa, nt)) { *c = 0xFFFFFFFFu; return; }
-  if (!skinny_pass(P, T, fan_off, fan, v, u, m, Qa, nt)) { *c = 0xFFFFFFFFu; return; }
+  if (!skinny_pass_g(P, T, fplane, fan, ua, ub, u, v, m, Qa, nt)) { *c = 0xFFFFFFFFu; return; }
+  if (!skinny_pass_g(P, T, fplane, fan, va, vb, v, u, m, Qa, nt)) { *c = 0xFFFFFFFFu; return; }
   Qa = 5.0f * Qa;
   const float* qu = Q + (long long)10 * u;
   const float* qv = Q + (long long)10 * v;
@@ -353,53 +447,42 @@ __global__ void k_compact_idx_dev(const int* __restrict__ flags, const int* __re
 }
 
 // -------------------------------------------------------- segmented LBVH
-// leaf boxes and per-mesh scene boxes (wave-uniform fast path)
+// leaf boxes of the alive faces of participating meshes
 __global__ void __launch_bounds__(BLOCK) k_leaf_boxes_b(const int* __restrict__ leaf_face, const int* __restrict__ n_ptr,
                                                         const float* __restrict__ P, const int* __restrict__ T,
-                                                        const int* __restrict__ fmesh, float4* __restrict__ lbox_lo,
-                                                        float4* __restrict__ lbox_hi, int* scene) {
-  int n = *n_ptr;
+                                                        float4* __restrict__ lbox_lo, float4* __restrict__ lbox_hi) {
   int l = blockIdx.x * blockDim.x + threadIdx.x;
-  bool act = l < n;
-  int k = -1;
-  float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-  if (act) {
-    int f = leaf_face[l];
-    k = fmesh[f];
-    I3 t = ldt(T, f);
-    V3 a = ldp(P, t.i), b = ldp(P, t.j), c = ldp(P, t.k);
-    v[0] = fminf(a.x, fminf(b.x, c.x)); v[1] = fminf(a.y, fminf(b.y, c.y)); v[2] = fminf(a.z, fminf(b.z, c.z));
-    v[3] = fmaxf(a.x, fmaxf(b.x, c.x)); v[4] = fmaxf(a.y, fmaxf(b.y, c.y)); v[5] = fmaxf(a.z, fmaxf(b.z, c.z));
-    lbox_lo[l] = make_float4(v[0], v[1], v[2], 0.0f);
-    lbox_hi[l] = make_float4(v[3], v[4], v[5], 0.0f);
+  if (l >= *n_ptr) return;
+  I3 t = ldt(T, leaf_face[l]);
+  V3 a = ldp(P, t.i), b = ldp(P, t.j), c = ldp(P, t.k);
+  lbox_lo[l] = make_float4(fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)), fminf(a.z, fminf(b.z, c.z)), 0.0f);
+  lbox_hi[l] = make_float4(fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)), fmaxf(a.z, fmaxf(b.z, c.z)), 0.0f);
+}
+// vertex bounds of one mesh at admission (ordered-int atomics, 6 slots)
+__global__ void __launch_bounds__(BLOCK) k_mesh_bounds(const float* __restrict__ P, long long n, int* box) {
+  __shared__ float red[6][BLOCK / 64];
+  long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  float x[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  if (v < n) {
+    x[0] = x[3] = P[3 * v]; x[1] = x[4] = P[3 * v + 1]; x[2] = x[5] = P[3 * v + 2];
   }
-  int k0 = __shfl(k, 0, 64);
-  bool uniform = __all(k == k0 || !act);
-  if (uniform && __any(act)) {
-    // first active lane's mesh: all active lanes share it
-    unsigned long long m = __ballot(act);
-    int leader = __ffsll((long long)m) - 1;
-    int kk = __shfl(k, leader, 64);
 #pragma unroll
-    for (int q = 0; q < 6; q++) {
-      float x = v[q];
+  for (int q = 0; q < 6; q++) {
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        float y = __shfl_xor(x, o, 64);
-        x = q < 3 ? fminf(x, y) : fmaxf(x, y);
-      }
-      v[q] = x;
+    for (int o = 32; o > 0; o >>= 1) {
+      float y = __shfl_xor(x[q], o, 64);
+      x[q] = q < 3 ? fminf(x[q], y) : fmaxf(x[q], y);
     }
-    if ((threadIdx.x & 63) == leader) {
-      for (int q = 0; q < 6; q++) {
-        if (q < 3) atomicMin(&scene[6 * kk + q], f2ord(v[q]));
-        else atomicMax(&scene[6 * kk + q], f2ord(v[q]));
-      }
-    }
-  } else if (act) {
+  }
+  if ((threadIdx.x & 63) == 0)
+    for (int q = 0; q < 6; q++) red[q][threadIdx.x >> 6] = x[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
     for (int q = 0; q < 6; q++) {
-      if (q < 3) atomicMin(&scene[6 * k + q], f2ord(v[q]));
-      else atomicMax(&scene[6 * k + q], f2ord(v[q]));
+      float y = red[q][0];
+      for (int w = 1; w < BLOCK / 64; w++) y = q < 3 ? fminf(y, red[q][w]) : fmaxf(y, red[q][w]);
+      if (q < 3) atomicMin(&box[q], f2ord(y));
+      else atomicMax(&box[q], f2ord(y));
     }
   }
 }
@@ -409,28 +492,45 @@ __global__ void k_scene_init_b(int* scene, int M) {
   scene[i] = (i % 6) < 3 ? 0x7FFFFFFF : (int)0x80000000;
 }
 // 64-bit keys: mesh index above the 30-bit Morton code of the box centre in
-// the mesh's own scene box
+// the mesh's admission box
 __global__ void k_morton_b(const float4* __restrict__ lbox_lo, const float4* __restrict__ lbox_hi,
                            const int* __restrict__ n_ptr, const int* __restrict__ leaf_face,
-                           const int* __restrict__ fmesh, const int* __restrict__ scene, u64* __restrict__ keys,
+                           const int* __restrict__ fmesh, const MeshParam* __restrict__ mp, u64* __restrict__ keys,
                            int* __restrict__ vals) {
   int l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= *n_ptr) return;
   int k = fmesh[leaf_face[l]];
-  const int* s = scene + 6 * k;
+  const float* box = mp[k].box;
   float4 lo = lbox_lo[l], hi = lbox_hi[l];
   float c[3] = {0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
   uint32_t q[3];
 #pragma unroll
   for (int a = 0; a < 3; a++) {
-    float slo = ord2f(s[a]), shi = ord2f(s[3 + a]);
-    float ext = shi - slo;
-    float t = ext > 0.0f ? (c[a] - slo) / ext : 0.0f;
+    float ext = box[3 + a] - box[a];
+    float t = ext > 0.0f ? (c[a] - box[a]) / ext : 0.0f;
     t = fminf(fmaxf(t * 1024.0f, 0.0f), 1023.0f);
     q[a] = (uint32_t)t;
   }
   keys[l] = ((u64)(uint32_t)k << 30) | (u64)(expand10(q[0]) * 4 + expand10(q[1]) * 2 + expand10(q[2]));
   vals[l] = l;
+}
+
+// queries in Morton order: flag per sorted leaf position, then compaction
+__global__ void k_query_sorted(const int* __restrict__ sorted_leaf, const int* __restrict__ leaf_face,
+                               const int* __restrict__ n_ptr, const uint8_t* __restrict__ qflag, int* __restrict__ qf) {
+  int l = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = *n_ptr;
+  if (l > n) return;
+  qf[l] = l < n ? (int)qflag[leaf_face[sorted_leaf[l]]] : 0;
+}
+__global__ void k_query_list(const int* __restrict__ sorted_leaf, const int* __restrict__ leaf_face,
+                             const int* __restrict__ n_ptr, const int* __restrict__ qf, const int* __restrict__ qpos,
+                             int* __restrict__ qlist, int* __restrict__ nq) {
+  int l = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = *n_ptr;
+  if (l == n) *nq = qpos[n];
+  if (l >= n || !qf[l]) return;
+  qlist[qpos[l]] = leaf_face[sorted_leaf[l]];
 }
 
 // stable LSD radix sort of (u64 key, int value), 8-bit digits (as
@@ -631,7 +731,7 @@ __global__ void __launch_bounds__(BLOCK) k_traverse_b(const int* __restrict__ ql
           pg[0][0] = a.x; pg[0][1] = a.y; pg[0][2] = a.z;
           pg[1][0] = b.x; pg[1][1] = b.y; pg[1][2] = b.z;
           pg[2][0] = cc.x; pg[2][1] = cc.y; pg[2][2] = cc.z;
-          if (narrow_hit(pg, pf, tg, tf) || narrow_hit(pf, pg, tf, tg)) { hit = true; break; }
+          if (pair_hits(pf, pg, tf, tg)) { hit = true; break; }
         }
       }
       node = esc;

@@ -293,12 +293,16 @@ bool pamo_ctx::build_bvh_and_traverse(int Fn, bool debug_pairs, std::vector<int>
     std::swap(vin, vout);
   }
   if (!fill_i32(flags.p, 0, Fl)) return false;
+  k_fill_i32<<<1, 1, 0, st>>>(ptr<int>(kparent), -1, 1);
   k_karras<<<nblk(Fl), BLOCK, 0, st>>>(kin, n_alive, ptr<int>(kleft), ptr<int>(kright), ptr<int>(kparent),
                                       ptr<int>(rfirst), ptr<int>(rlast));
-  k_bottom_up<<<nblk(Fl), BLOCK, 0, st>>>(n_alive, ptr<float4>(lbox_lo), ptr<float4>(lbox_hi), vin,
-                                         ptr<int>(leaf_face), ptr<int>(kparent), ptr<int>(kleft), ptr<int>(kright),
-                                         ptr<int>(rfirst), ptr<int>(rlast), ptr<int>(flags), ptr<float4>(nodeA),
-                                         ptr<float4>(nodeB));
+  k_bottom_up_local<<<nblk(Fl), BLOCK, 0, st>>>(n_alive, ptr<float4>(lbox_lo), ptr<float4>(lbox_hi), vin,
+                                               ptr<int>(leaf_face), ptr<int>(kparent), ptr<int>(kleft),
+                                               ptr<int>(kright), ptr<int>(rfirst), ptr<int>(rlast), ptr<float4>(nodeA),
+                                               ptr<float4>(nodeB));
+  k_bottom_up_global<<<nblk(2 * Fl), BLOCK, 0, st>>>(n_alive, ptr<int>(kparent), ptr<int>(kleft), ptr<int>(kright),
+                                                    ptr<int>(rfirst), ptr<int>(rlast), ptr<int>(flags),
+                                                    ptr<float4>(nodeA), ptr<float4>(nodeB));
   prof_end(pbv);
   if (!check_launch("lbvh")) return false;
   if (!debug_pairs) {
@@ -545,7 +549,7 @@ static void eng_free(pamo_ctx* c) {
                 &g.undo_out, &g.tmp, &g.marked, &g.pm, &g.table, &g.ecost, &g.mp, &g.voff, &g.foff, &g.eoffm,
                 &g.uioff, &g.uioff2, &g.nustk, &g.h, &g.lcnt, &g.ncoll, &g.cut, &g.cutoff, &g.uobase, &g.hm, &g.part,
                 &g.first, &g.nvu, &g.nvoff, &g.nfoff, &g.seg_a, &g.seg_b, &g.root, &g.root_end, &g.scene,
-                &g.flat_cost, &g.g, &g.keep, &g.kpos, &g.newk, &g.vkeep, &g.vpos};
+                &g.flat_cost, &g.g, &g.keep, &g.kpos, &g.newk, &g.vkeep, &g.vpos, &g.fsucc, &g.fplane, &g.flen};
   for (Buf* b : all)
     if (b->p) (void)hipFree(b->p);
   for (auto& s : g.slots) {
@@ -734,6 +738,7 @@ int pamo_run_stage2(pamo_ctx* ctx, void* stream, const float* d_verts, const int
   int h = pamo_batch_submit(ctx, stream, d_verts, d_faces, n_verts, n_faces, scale, target, threshold, tolerance,
                             iters, -1);
   if (h < 0) return h;
+  const long long s0[4] = {c.eng->st_checks, c.eng->st_queries, c.eng->st_cand, c.eng->st_hits};
   while (!c.eng->slots[h].done) {
     int na = 0;
     if (pamo_batch_step(ctx, stream, 1, &na) < 0) return -1;
@@ -748,8 +753,8 @@ int pamo_run_stage2(pamo_ctx* ctx, void* stream, const float* d_verts, const int
   if (out_F) *out_F = s.rF;
   if (stats) {
     Engine& g = *c.eng;
-    long long v[8] = {s.rounds, s.exit_reason, s.collapses, s.max_passes, g.st_checks, g.st_queries, g.st_cand,
-                      g.st_hits};
+    long long v[8] = {s.rounds,           s.exit_reason,          s.collapses,          s.max_passes,
+                      g.st_checks - s0[0], g.st_queries - s0[1], g.st_cand - s0[2], g.st_hits - s0[3]};
     memcpy(stats, v, sizeof(v));
   }
   return 0;
