@@ -63,7 +63,8 @@ int pamo_free_copy_outputs(pamo_ctx* ctx, void* stream, float* d_points, int* d_
  * stats (may be NULL): [0] rounds [1] exit reason (1 target, 2 <=10 faces,
  * 3 tolerance, 4 iteration cap) [2] collapses [3] undo passes (max)
  * [4] self-intersection checks [5] query faces [6] candidate pairs
- * [7] faces found intersecting. */
+ * [7] intersecting pairs found (engine; the round-by-round loop counts
+ * intersecting query faces). */
 int pamo_run_stage2(pamo_ctx* ctx, void* stream, const float* d_verts, const int* d_faces, int n_verts, int n_faces,
                     float scale, long long target, float threshold, int tolerance, long long iters, int* out_V,
                     int* out_F, long long* stats);
@@ -140,6 +141,12 @@ int pamo_profile_read(pamo_ctx* ctx, int kernel, double out[3]);
  * 12 first check's intersecting-face flags, 13 round-start faces,
  * 14 invalid-vertex table. */
 long long pamo_debug_copy(pamo_ctx* ctx, void* stream, int which, void* host_dst, long long cap_bytes);
+
+/* Test hook (no counterpart in the reference): limits the candidate-pair
+ * buffer of the engine's self-intersection check to cap pairs (-1 restores
+ * the default of 12 per face).  Queries whose pairs do not fit are walked by
+ * the single-kernel fallback; results must not change. */
+int pamo_debug_pair_cap(pamo_ctx* ctx, long long cap);
 
 /* Parity hook: every ordered candidate pair (query, current) of one
  * self-intersection check over all alive faces of a mesh, with the

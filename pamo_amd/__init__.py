@@ -72,7 +72,7 @@ def simplify_stage2(verts, faces, scale, target, threshold=1e-3, tolerance=4, it
     of = torch.empty((oF.value, 3), dtype=torch.int32, device=verts.device)
     ctx.check(L.pamo_run_fetch(ctx.handle, stream, ov.data_ptr(), of.data_ptr()), "pamo_run_fetch")
     stats = dict(rounds=int(st[0]), exit_reason=int(st[1]), collapses=int(st[2]), max_undo_passes=int(st[3]),
-                 checks=int(st[4]), query_faces=int(st[5]), candidate_pairs=int(st[6]), hit_faces=int(st[7]))
+                 checks=int(st[4]), query_faces=int(st[5]), candidate_pairs=int(st[6]), hit_pairs=int(st[7]))
     rf = np.zeros(max(stats["rounds"], 1), np.int32)
     L.pamo_run_rounds(ctx.handle, rf.ctypes.data, rf.size)
     stats["round_faces"] = rf[:stats["rounds"]]

@@ -64,6 +64,8 @@ def lib():
     L.pamo_profile.restype = I
     L.pamo_profile_read.argtypes = [P, I, P]
     L.pamo_profile_read.restype = I
+    L.pamo_debug_pair_cap.argtypes = [P, LL]
+    L.pamo_debug_pair_cap.restype = I
     _lib = L
     return L
 
@@ -106,6 +108,10 @@ class Context:
         out = np.zeros(3, np.float64)
         self.check(self._L.pamo_profile_read(self.handle, int(kernel), out.ctypes.data), "pamo_profile_read")
         return dict(launches=int(out[0]), ms=float(out[1]), bytes=float(out[2]))
+
+    def debug_pair_cap(self, cap):
+        """Limit the engine's candidate-pair buffer (-1: default); test hook."""
+        self.check(self._L.pamo_debug_pair_cap(self.handle, int(cap)), "pamo_debug_pair_cap")
 
     def check(self, rc, what):
         if rc < 0:

@@ -42,11 +42,17 @@ __device__ __forceinline__ void interp3(float r[3], const float a[3], const floa
   r[1] = s * a[1] + t * b[1];
   r[2] = s * a[2] + t * b[2];
 }
+__device__ __forceinline__ float sel3(int r, float a, float b, float c) { return r == 0 ? a : (r == 1 ? b : c); }
 // tri_tri_3d.cuh:131-276, with the first plane and the sides of the second
-// triangle passed in (the coplanarity test computes the same values)
+// triangle passed in (the coplanarity test computes the same values).  The
+// reference picks the vertex order (t0, t1, t2) of each triangle at run
+// time and indexes with it; here the order is one of three rotations r
+// (t0 = r, t1 = r+1, t2 = r+2 mod 3) and the vertices are selected by value,
+// so no array is indexed dynamically and nothing goes to scratch memory.
+// Every arithmetic expression is the reference's, on the same operands.
 __device__ bool isect_tri_tri_ref(const float ta[3][3], const float tb[3][3], const double pa[4], const double sd[3],
                                   float ri1[3], float ri2[3]) {
-  float mn[2], mx[2], loc[2][2][3];
+  float mn[2], mx[2], lo0[2][3], lo1[2][3];
   float side[2][3];
   double pb[4];
 #pragma unroll
@@ -66,46 +72,52 @@ __device__ bool isect_tri_tri_ref(const float ta[3][3], const float tb[3][3], co
 #pragma unroll
   for (int i = 0; i < 2; i++) {
     const float(*tri)[3] = i == 0 ? ta : tb;
-    int t0, t1, t2;
-    if ((side[i][0] && side[i][1]) && (side[i][0] < 0.0f) == (side[i][1] < 0.0f)) { t0 = 1; t1 = 2; t2 = 0; }
-    else if ((side[i][1] && side[i][2]) && (side[i][1] < 0.0f) == (side[i][2] < 0.0f)) { t0 = 2; t1 = 0; t2 = 1; }
-    else if (!side[i][1] && (side[i][2] < 0.0f) != (side[i][0] < 0.0f)) { t0 = 1; t1 = 2; t2 = 0; }
-    else { t0 = 0; t1 = 1; t2 = 2; }
-    double dot_b = dotdf(dir, tri[t1]);
-    float sidec = side[i][t1];
-    if (sidec) {
-      double dot_a = dotdf(dir, tri[t0]);
-      double dot_c = dotdf(dir, tri[t2]);
-      float fac0 = sidec / (sidec - side[i][t0]);
-      float fac1 = sidec / (sidec - side[i][t2]);
+    const float s0 = side[i][0], s1 = side[i][1], s2 = side[i][2];
+    int r;
+    if ((s0 && s1) && (s0 < 0.0f) == (s1 < 0.0f)) r = 1;
+    else if ((s1 && s2) && (s1 < 0.0f) == (s2 < 0.0f)) r = 2;
+    else if (!s1 && (s2 < 0.0f) != (s0 < 0.0f)) r = 1;
+    else r = 0;
+    float A[3], B[3], C[3];   // tri[t0], tri[t1], tri[t2]
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      A[k] = sel3(r, tri[0][k], tri[1][k], tri[2][k]);
+      B[k] = sel3(r, tri[1][k], tri[2][k], tri[0][k]);
+      C[k] = sel3(r, tri[2][k], tri[0][k], tri[1][k]);
+    }
+    const float sA = sel3(r, s0, s1, s2), sB = sel3(r, s1, s2, s0), sC = sel3(r, s2, s0, s1);
+    double dot_b = dotdf(dir, B);
+    if (sB) {
+      double dot_a = dotdf(dir, A);
+      double dot_c = dotdf(dir, C);
+      float fac0 = sB / (sB - sA);
+      float fac1 = sB / (sB - sC);
       double off0 = fac0 * (dot_a - dot_b);
       double off1 = fac1 * (dot_c - dot_b);
       if (off0 > off1) {
         double td = off0; off0 = off1; off1 = td;
         float tf = fac0; fac0 = fac1; fac1 = tf;
-        int ti = t0; t0 = t2; t2 = ti;
+#pragma unroll
+        for (int k = 0; k < 3; k++) { float x = A[k]; A[k] = C[k]; C[k] = x; }
       }
       mn[i] = (float)(dot_b + off0);
       mx[i] = (float)(dot_b + off1);
-      interp3(loc[i][0], tri[t1], tri[t0], fac0);
-      interp3(loc[i][1], tri[t1], tri[t2], fac1);
+      interp3(lo0[i], B, A, fac0);
+      interp3(lo1[i], B, C, fac1);
     } else {
       mn[i] = mx[i] = (float)dot_b;
 #pragma unroll
-      for (int k = 0; k < 3; k++) loc[i][0][k] = loc[i][1][k] = tri[t1][k];
+      for (int k = 0; k < 3; k++) lo0[i][k] = lo1[i][k] = B[k];
     }
   }
   if ((mx[0] > mn[1]) && (mn[0] < mx[1])) {
-    const float *a, *b;
-    if (mn[0] >= mn[1]) {
-      a = loc[0][0];
-      b = (mx[0] <= mx[1]) ? loc[0][1] : loc[1][1];
-    } else {
-      if (mx[0] <= mx[1]) { a = loc[0][1]; b = loc[1][0]; }
-      else { a = loc[1][0]; b = loc[1][1]; }
-    }
+    // the reference's choice of segment ends, by value
+    const bool c1 = mn[0] >= mn[1], c2 = mx[0] <= mx[1];
 #pragma unroll
-    for (int k = 0; k < 3; k++) { ri1[k] = a[k]; ri2[k] = b[k]; }
+    for (int k = 0; k < 3; k++) {
+      ri1[k] = c1 ? lo0[0][k] : (c2 ? lo1[0][k] : lo0[1][k]);
+      ri2[k] = c1 ? (c2 ? lo1[0][k] : lo1[1][k]) : (c2 ? lo0[1][k] : lo1[1][k]);
+    }
     return true;
   }
   return false;
@@ -411,6 +423,51 @@ __global__ void k_karras(const uint32_t* __restrict__ key, const int* __restrict
 // LDS flags (workgroup scope: the second child to arrive merges).  Phase 2:
 // the few nodes whose range spans blocks, from their in-block children,
 // through agent-scope acq_rel flags in global memory.
+__device__ __forceinline__ long long seg_base(int j, long long cap) {
+  long long b = 0;
+  for (int i = 1; i < j; i++) b += (cap + (1LL << i) - 1) >> i;
+  return b;
+}
+inline long long seg_entries(long long cap) {   // host: buffer entries for all levels
+  long long b = 0;
+  for (int i = 1; (1LL << (i - 1)) < cap; i++) b += (cap + (1LL << i) - 1) >> i;
+  return b + 1;
+}
+// block reduction of BLOCK boxes (lo/hi in LDS, NaN padding) into levels
+// j0+1..j0+8; entry index of the block's first element at level j0 is e0
+__device__ __forceinline__ void seg_reduce_block(float (*rlo)[3], float (*rhi)[3], long long e0, int j0, int n,
+                                                 long long cap, float4* __restrict__ seg_lo,
+                                                 float4* __restrict__ seg_hi) {
+  const int t = threadIdx.x;
+  long long base = seg_base(j0 + 1, cap);
+  for (int d = 1; (BLOCK >> d) >= 1; d++) {
+    const int j = j0 + d;
+    const int cnt = BLOCK >> d;
+    float lo[3], hi[3];
+    const bool act = t < cnt;
+    if (act) {
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        lo[k] = fminf(rlo[2 * t][k], rlo[2 * t + 1][k]);
+        hi[k] = fmaxf(rhi[2 * t][k], rhi[2 * t + 1][k]);
+      }
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int k = 0; k < 3; k++) { rlo[t][k] = lo[k]; rhi[t][k] = hi[k]; }
+      long long idx = (e0 >> d) + t;
+      long long size = ((long long)n + (1LL << j) - 1) >> j;
+      if (idx < size && (1LL << (j - 1)) < n) {   // levels a query can use (2^(j-1) < n <= cap)
+        seg_lo[base + idx] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+        seg_hi[base + idx] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+      }
+    }
+    __syncthreads();
+    base += (cap + (1LL << j) - 1) >> j;
+  }
+}
+
 __device__ __forceinline__ int escape_after(int last, int n, const int* __restrict__ rfirst) {
   // next node in depth-first order after a subtree ending at sorted leaf last
   if (last == n - 1) return -1;
@@ -429,9 +486,12 @@ __global__ void __launch_bounds__(BLOCK) k_bottom_up_local(const int* __restrict
                                                            const int* __restrict__ leaf_face,
                                                            const int* __restrict__ parent, const int* __restrict__ left,
                                                            const int* __restrict__ right, const int* __restrict__ rfirst,
-                                                           const int* __restrict__ rlast, float4* nodeA, float4* nodeB) {
+                                                           const int* __restrict__ rlast, float4* nodeA, float4* nodeB,
+                                                           long long cap, float4* __restrict__ seg_lo,
+                                                           float4* __restrict__ seg_hi) {
   __shared__ int flag[BLOCK];
   __shared__ float blo[2 * BLOCK][3], bhi[2 * BLOCK][3];   // [0,B): leaves, [B,2B): internal nodes
+  __shared__ float rlo[BLOCK][3], rhi[BLOCK][3];           // segment-tree reduction
   const int n = *n_ptr;
   const int b0 = blockIdx.x * BLOCK;
   if (b0 >= n) return;
@@ -445,8 +505,15 @@ __global__ void __launch_bounds__(BLOCK) k_bottom_up_local(const int* __restrict
     bhi[threadIdx.x][0] = hi.x; bhi[threadIdx.x][1] = hi.y; bhi[threadIdx.x][2] = hi.z;
     nodeA[n - 1 + l] = make_float4(lo.x, lo.y, lo.z, __int_as_float(escape_after(l, n, rfirst)));
     nodeB[n - 1 + l] = make_float4(hi.x, hi.y, hi.z, __int_as_float(~leaf_face[src]));
+    rlo[threadIdx.x][0] = lo.x; rlo[threadIdx.x][1] = lo.y; rlo[threadIdx.x][2] = lo.z;
+    rhi[threadIdx.x][0] = hi.x; rhi[threadIdx.x][1] = hi.y; rhi[threadIdx.x][2] = hi.z;
+  } else {
+    const float qn = __int_as_float(0x7FC00000);
+#pragma unroll
+    for (int k = 0; k < 3; k++) rlo[threadIdx.x][k] = rhi[threadIdx.x][k] = qn;
   }
   __syncthreads();
+  if (n > BLOCK) seg_reduce_block(rlo, rhi, b0, 0, n, cap, seg_lo, seg_hi);   // block-uniform
   if (l > b1 || n == 1) return;
   int node = n - 1 + l;
   for (int depth = 0; depth < n; depth++) {
@@ -474,34 +541,74 @@ __global__ void __launch_bounds__(BLOCK) k_bottom_up_local(const int* __restrict
   }
 }
 
-// phase 2: one thread per node; nodes inside one block whose parent spans
-// blocks walk up; flags of internal nodes start at zero
-__global__ void __launch_bounds__(BLOCK) k_bottom_up_global(const int* __restrict__ n_ptr,
-                                                            const int* __restrict__ parent,
-                                                            const int* __restrict__ left, const int* __restrict__ right,
-                                                            const int* __restrict__ rfirst,
-                                                            const int* __restrict__ rlast, int* flags, float4* nodeA,
-                                                            float4* nodeB) {
+// Nodes whose leaf range spans blocks.  A node's box is the union of the
+// leaf boxes in its range (fminf/fmaxf ignore NaN and are otherwise exact,
+// so any grouping gives the box of the pairwise bottom-up merge), taken here
+// from an aligned segment tree over the sorted leaves: level j >= 1, entry i
+// is the union of leaves [i 2^j, (i+1) 2^j) clipped to n, stored at
+// seg_base(j) + i with bases laid out for a capacity of cap leaves.
+// Levels 1..8 come from k_bottom_up_local (one block = 256 aligned leaves),
+// higher ones from k_seg_upper, 8 levels per launch.  No node waits for
+// another, so no cross-workgroup flags or cache maintenance are needed.
+__global__ void __launch_bounds__(BLOCK) k_seg_upper(const int* __restrict__ n_ptr, long long cap, int j0,
+                                                     float4* __restrict__ seg_lo, float4* __restrict__ seg_hi) {
+  __shared__ float rlo[BLOCK][3], rhi[BLOCK][3];
   const int n = *n_ptr;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n < 2 || x >= 2 * n - 1) return;
-  int a, b;
-  node_range(x, n, rfirst, rlast, a, b);
-  if (a / BLOCK != b / BLOCK) return;          // itself spans blocks
-  int p = parent[x];
-  if (p < 0) return;
-  if (rfirst[p] / BLOCK == rlast[p] / BLOCK) return;   // parent merged in phase 1
-  for (int depth = 0; depth < n; depth++) {
-    int old = __hip_atomic_fetch_add(&flags[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == 0) return;
-    int lc = left[p], rc = right[p];
-    float4 a0 = nodeA[lc], b0 = nodeB[lc], a1 = nodeA[rc], b1 = nodeB[rc];
-    nodeA[p] = make_float4(fminf(a0.x, a1.x), fminf(a0.y, a1.y), fminf(a0.z, a1.z),
-                           __int_as_float(escape_after(rlast[p], n, rfirst)));
-    nodeB[p] = make_float4(fmaxf(b0.x, b1.x), fmaxf(b0.y, b1.y), fmaxf(b0.z, b1.z), __int_as_float(lc));
-    if (p == 0) return;
-    p = parent[p];
+  const long long size0 = ((long long)n + (1LL << j0) - 1) >> j0;
+  const long long e0 = (long long)blockIdx.x * BLOCK;
+  if (size0 <= 1 || e0 >= size0) return;   // block-uniform
+  const long long e = e0 + threadIdx.x;
+  const float qn = __int_as_float(0x7FC00000);
+  const long long b0 = seg_base(j0, cap);
+#pragma unroll
+  for (int k = 0; k < 3; k++) rlo[threadIdx.x][k] = rhi[threadIdx.x][k] = qn;
+  if (e < size0) {
+    float4 lo = seg_lo[b0 + e], hi = seg_hi[b0 + e];
+    rlo[threadIdx.x][0] = lo.x; rlo[threadIdx.x][1] = lo.y; rlo[threadIdx.x][2] = lo.z;
+    rhi[threadIdx.x][0] = hi.x; rhi[threadIdx.x][1] = hi.y; rhi[threadIdx.x][2] = hi.z;
   }
+  __syncthreads();
+  seg_reduce_block(rlo, rhi, e0, j0, n, cap, seg_lo, seg_hi);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_cross_nodes(const int* __restrict__ n_ptr, long long cap,
+                                                       const int* __restrict__ left, const int* __restrict__ rfirst,
+                                                       const int* __restrict__ rlast,
+                                                       const float4* __restrict__ seg_lo,
+                                                       const float4* __restrict__ seg_hi, float4* nodeA,
+                                                       float4* nodeB) {
+  const int n = *n_ptr;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < 2 || i >= n - 1) return;
+  const int a = rfirst[i], b = rlast[i];
+  if (a / BLOCK == b / BLOCK) return;   // merged in k_bottom_up_local
+  const float qn = __int_as_float(0x7FC00000);
+  float lo[3] = {qn, qn, qn}, hi[3] = {qn, qn, qn};
+  long long l = a, h = (long long)b + 1;
+  long long base = 0;   // base of level j (j >= 1)
+  for (int j = 0; l < h; j++) {
+    if (l & 1) {
+      float4 x, y;
+      if (j == 0) { x = nodeA[n - 1 + l]; y = nodeB[n - 1 + l]; }
+      else { x = seg_lo[base + l]; y = seg_hi[base + l]; }
+      lo[0] = fminf(lo[0], x.x); lo[1] = fminf(lo[1], x.y); lo[2] = fminf(lo[2], x.z);
+      hi[0] = fmaxf(hi[0], y.x); hi[1] = fmaxf(hi[1], y.y); hi[2] = fmaxf(hi[2], y.z);
+      l++;
+    }
+    if (h & 1) {
+      h--;
+      float4 x, y;
+      if (j == 0) { x = nodeA[n - 1 + h]; y = nodeB[n - 1 + h]; }
+      else { x = seg_lo[base + h]; y = seg_hi[base + h]; }
+      lo[0] = fminf(lo[0], x.x); lo[1] = fminf(lo[1], x.y); lo[2] = fminf(lo[2], x.z);
+      hi[0] = fmaxf(hi[0], y.x); hi[1] = fmaxf(hi[1], y.y); hi[2] = fmaxf(hi[2], y.z);
+    }
+    l >>= 1;
+    h >>= 1;
+    if (j >= 1) base += (cap + (1LL << j) - 1) >> j;
+  }
+  nodeA[i] = make_float4(lo[0], lo[1], lo[2], __int_as_float(escape_after(b, n, rfirst)));
+  nodeB[i] = make_float4(hi[0], hi[1], hi[2], __int_as_float(left[i]));
 }
 
 // ------------------------------------------------------------ traversal

@@ -103,28 +103,24 @@ __global__ void k_shuffle_scatter_off(Feistel fb, long long n, long long m, cons
   for (int c = 0; c < 3; c++) T[3LL * o + c] = tin[3 * k + c] + voff;
 }
 
-// ------------------------------------------------------ face geometry
-// per alive face: unit normal and plane offset (tri2plane,
-// cusimp_free.cu:337-346, also the old normal of the flip test :520-521)
-// and the three edge lengths |Pi-Pj|, |Pj-Pk|, |Pk-Pi| (:451-483); computed
-// once per round with the reference's expressions
-__global__ void __launch_bounds__(BLOCK) k_face_geom(const float* __restrict__ P, const int* __restrict__ T, int F,
-                                                     float4* __restrict__ fplane, float4* __restrict__ flen) {
-  int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= F) return;
-  I3 t = ldt(T, f);
-  if (dead3(t)) return;
-  V3 v0 = ldp(P, t.i), v1 = ldp(P, t.j), v2 = ldp(P, t.k);
-  V3 n = vcross(vsub(v1, v0), vsub(v2, v0));
-  n = vdivs(n, vnorm(n));
-  float off = -vdot(n, v0);
-  fplane[f] = make_float4(n.x, n.y, n.z, off);
-  flen[f] = make_float4(vnorm(vsub(v0, v1)), vnorm(vsub(v1, v2)), vnorm(vsub(v2, v0)), 0.0f);
-}
-// fan sort (ascending face index, D0) plus the successor of the fan's vertex
-// in each face (the link-condition vertex, cusimp_free.cu:407-430)
-__global__ void k_fan_sort_succ(const int* __restrict__ fan_off, int* __restrict__ fan, const int* __restrict__ T,
-                                int V, int* __restrict__ fsucc) {
+// ------------------------------------------------------ fan slots
+// One thread per vertex: sorts its fan (ascending face index, D0) and writes,
+// per fan slot, what the edge costs read about that face, so the cost kernel
+// walks contiguous slot arrays instead of gathering face records:
+//   fsucc  successor of the vertex in the face (link condition,
+//          cusimp_free.cu:407-430)
+//   sface  the face's vertex indices and its index
+//   splane unit normal and offset (tri2plane, cusimp_free.cu:337-346; also
+//          the old normal of the flip test :520-521)
+//   slen   |Pi-Pj|, |Pj-Pk|, |Pk-Pi| (:451-483)
+// and the vertex quadric from the same planes in fan order (:348-366).
+// A face's values are computed by each of its three vertices with the same
+// expressions, so all copies are identical.
+__global__ void __launch_bounds__(BLOCK) k_fan_slots(const float* __restrict__ P, const int* __restrict__ T,
+                                                     const int* __restrict__ fan_off, int* __restrict__ fan, int V,
+                                                     int* __restrict__ fsucc, int4* __restrict__ sface,
+                                                     float4* __restrict__ splane, float4* __restrict__ slen,
+                                                     float* __restrict__ Q) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= V) return;
   int a = fan_off[v], b = fan_off[v + 1];
@@ -134,49 +130,74 @@ __global__ void k_fan_sort_succ(const int* __restrict__ fan_off, int* __restrict
     while (j >= a && fan[j] > x) { fan[j + 1] = fan[j]; j--; }
     fan[j + 1] = x;
   }
-  for (int i = a; i < b; i++) fsucc[i] = succ_of(ldt(T, fan[i]), v);
-}
-// cusimp_free.cu:348-366 from the per-face planes
-__global__ void __launch_bounds__(BLOCK) k_quadric_g(const float4* __restrict__ fplane, const int* __restrict__ fan_off,
-                                                     const int* __restrict__ fan, int V, float* __restrict__ Q) {
-  int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= V) return;
   float q[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  int a = fan_off[v], b = fan_off[v + 1];
   for (int s = a; s < b; s++) {
-    float4 p = fplane[fan[s]];
-    q[0] += p.x * p.x; q[1] += p.x * p.y; q[2] += p.x * p.z; q[3] += p.x * p.w;
-    q[4] += p.y * p.y; q[5] += p.y * p.z; q[6] += p.y * p.w;
-    q[7] += p.z * p.z; q[8] += p.z * p.w;
-    q[9] += p.w * p.w;
+    int f = fan[s];
+    I3 t = ldt(T, f);
+    fsucc[s] = succ_of(t, v);
+    sface[s] = make_int4(t.i, t.j, t.k, f);
+    V3 v0 = ldp(P, t.i), v1 = ldp(P, t.j), v2 = ldp(P, t.k);
+    V3 n = vcross(vsub(v1, v0), vsub(v2, v0));
+    n = vdivs(n, vnorm(n));
+    float off = -vdot(n, v0);
+    splane[s] = make_float4(n.x, n.y, n.z, off);
+    slen[s] = make_float4(vnorm(vsub(v0, v1)), vnorm(vsub(v1, v2)), vnorm(vsub(v2, v0)), 0.0f);
+    q[0] += n.x * n.x; q[1] += n.x * n.y; q[2] += n.x * n.z; q[3] += n.x * off;
+    q[4] += n.y * n.y; q[5] += n.y * n.z; q[6] += n.y * off;
+    q[7] += n.z * n.z; q[8] += n.z * off;
+    q[9] += off * off;
   }
 #pragma unroll
   for (int i = 0; i < 10; i++) Q[(long long)10 * v + i] = q[i];
 }
-// fan edge-length sum from the per-face lengths (cusimp_free.cu:449-484)
-__device__ __forceinline__ float fan_len_g(const int* __restrict__ T, const float4* __restrict__ flen,
-                                           const int* __restrict__ fan, int a, int b, float el, int& ne) {
+
+// ------------------------------------------------- vertex-ordered edges
+// Edges keep their index (the claim key's tie-break), but are processed
+// grouped by their first endpoint u, so neighbouring lanes read the same
+// fans.  The emit counts edges per u (the atomic's return value is the
+// edge's rank within u; any rank order gives the same results, since every
+// output depends only on the edge index).
+__global__ void k_edge_emit_u(const int* __restrict__ T, int F, const int* __restrict__ eoff, int* __restrict__ edges,
+                              int* __restrict__ ucnt, int* __restrict__ erank) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  I3 t = ldt(T, f);
+  if (dead3(t)) return;
+  int o = eoff[f];
+  if (t.i > t.j) { edges[2 * o] = t.j; edges[2 * o + 1] = t.i; erank[o] = atomicAdd(&ucnt[t.j], 1); o++; }
+  if (t.j > t.k) { edges[2 * o] = t.k; edges[2 * o + 1] = t.j; erank[o] = atomicAdd(&ucnt[t.k], 1); o++; }
+  if (t.k > t.i) { edges[2 * o] = t.i; edges[2 * o + 1] = t.k; erank[o] = atomicAdd(&ucnt[t.i], 1); o++; }
+}
+__global__ void k_edge_vorder(const int* __restrict__ edges, const int* __restrict__ E_ptr,
+                              const int* __restrict__ uoff, const int* __restrict__ erank, int4* __restrict__ vedge) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= *E_ptr) return;
+  int u = edges[2 * e], v = edges[2 * e + 1];
+  vedge[uoff[u] + erank[e]] = make_int4(e, u, v, 0);
+}
+
+// fan edge-length sum from the slots (cusimp_free.cu:449-484)
+__device__ __forceinline__ float fan_len_s(const int4* __restrict__ sface, const float4* __restrict__ slen, int a,
+                                           int b, float el, int& ne) {
   for (int s = a; s < b; s++) {
-    int f = fan[s];
-    I3 t = ldt(T, f);
-    float4 L = flen[f];
-    if (t.i > t.j) { el += L.x; ne++; }
-    if (t.j > t.k) { el += L.y; ne++; }
-    if (t.k > t.i) { el += L.z; ne++; }
+    int4 t = sface[s];
+    float4 L = slen[s];
+    if (t.x > t.y) { el += L.x; ne++; }
+    if (t.y > t.z) { el += L.y; ne++; }
+    if (t.z > t.x) { el += L.z; ne++; }
   }
   return el;
 }
-// skinny/flip pass (cusimp_free.cu:497-575) with the per-face old normals
-__device__ bool skinny_pass_g(const float* __restrict__ P, const int* __restrict__ T,
-                              const float4* __restrict__ fplane, const int* __restrict__ fan, int a, int b, int moved,
-                              int other, V3 m, float& Qa, int& nt) {
+// skinny/flip pass (cusimp_free.cu:497-575) from the slots
+__device__ bool skinny_pass_s(const float* __restrict__ P, const int4* __restrict__ sface,
+                              const float4* __restrict__ splane, int a, int b, int moved, int other, V3 m, float& Qa,
+                              int& nt) {
   for (int s = a; s < b; s++) {
-    int f = fan[s];
-    I3 t = ldt(T, f);
-    if (t.i == other || t.j == other || t.k == other) continue;
-    V3 o0 = ldp(P, t.i), o1 = ldp(P, t.j), o2 = ldp(P, t.k);
-    V3 n0 = t.i == moved ? m : o0, n1 = t.j == moved ? m : o1, n2 = t.k == moved ? m : o2;
-    float4 op = fplane[f];
+    int4 t = sface[s];
+    if (t.x == other || t.y == other || t.z == other) continue;
+    V3 o0 = ldp(P, t.x), o1 = ldp(P, t.y), o2 = ldp(P, t.z);
+    V3 n0 = t.x == moved ? m : o0, n1 = t.y == moved ? m : o1, n2 = t.z == moved ? m : o2;
+    float4 op = splane[s];
     V3 on = {op.x, op.y, op.z};
     V3 nn = vcross(vsub(n1, n0), vsub(n2, n0));
     nn = vdivs(nn, vnorm(nn));
@@ -193,21 +214,23 @@ __device__ bool skinny_pass_g(const float* __restrict__ P, const int* __restrict
 }
 
 // ----------------------------------------------------------- edge cost
-// as k_edge_cost with per-mesh parameters; the table and the (persistent)
-// cost array are addressed by local indices in the mesh's regions
-__global__ void __launch_bounds__(BLOCK) k_edge_cost_b(const float* __restrict__ P, const int* __restrict__ T,
-                                                       const int* __restrict__ fan_off, const int* __restrict__ fan,
-                                                       const int* __restrict__ fsucc, const float4* __restrict__ fplane,
-                                                       const float4* __restrict__ flen,
-                                                       const int* __restrict__ edges, const int* __restrict__ E_ptr,
-                                                       const float* __restrict__ Q, const uint8_t* __restrict__ table,
-                                                       uint32_t* __restrict__ cost, const int* __restrict__ vmesh,
+// cusimp_free.cu:382-585 with per-mesh parameters, one lane per edge in
+// vertex order; the table and the (persistent) cost array are addressed by
+// local indices in the mesh's regions
+__global__ void __launch_bounds__(BLOCK) k_edge_cost_v(const float* __restrict__ P, const int4* __restrict__ vedge,
+                                                       const int* __restrict__ E_ptr, const int* __restrict__ fan_off,
+                                                       const int* __restrict__ fsucc, const int4* __restrict__ sface,
+                                                       const float4* __restrict__ splane,
+                                                       const float4* __restrict__ slen, const float* __restrict__ Q,
+                                                       const uint8_t* __restrict__ table, uint32_t* __restrict__ cost,
+                                                       const int* __restrict__ vmesh,
                                                        const long long* __restrict__ voff,
                                                        const long long* __restrict__ eoffm,
                                                        const MeshParam* __restrict__ mp) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= *E_ptr) return;
-  int u = edges[2 * e], v = edges[2 * e + 1];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= *E_ptr) return;
+  int4 ev = vedge[i];
+  const int e = ev.x, u = ev.y, v = ev.z;
   int k = vmesh[u];
   const long long pmo = mp[k].pmo;
   long long lu = u - voff[k], lv = v - voff[k];
@@ -226,21 +249,21 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_b(const float* __restrict__
   const float scale = mp[k].scale, tau = mp[k].tau;
   V3 v0 = ldp(P, u), v1 = ldp(P, v);
   int ne = 0;
-  float el = fan_len_g(T, flen, fan, ua, ub, 0.0f, ne);
-  el = fan_len_g(T, flen, fan, va, vb, el, ne);
+  float el = fan_len_s(sface, slen, ua, ub, 0.0f, ne);
+  el = fan_len_s(sface, slen, va, vb, el, ne);
   el = el / (float)ne / scale * tau;
   el = el + vnorm(vsub(v0, v1)) / scale * tau;
   V3 m = vdivs(vadd(v0, v1), 2.0f);
   float Qa = 0.0f;
   int nt = 0;
-  if (!skinny_pass_g(P, T, fplane, fan, ua, ub, u, v, m, Qa, nt)) { *c = 0xFFFFFFFFu; return; }
-  if (!skinny_pass_g(P, T, fplane, fan, va, vb, v, u, m, Qa, nt)) { *c = 0xFFFFFFFFu; return; }
+  if (!skinny_pass_s(P, sface, splane, ua, ub, u, v, m, Qa, nt)) { *c = 0xFFFFFFFFu; return; }
+  if (!skinny_pass_s(P, sface, splane, va, vb, v, u, m, Qa, nt)) { *c = 0xFFFFFFFFu; return; }
   Qa = 5.0f * Qa;
   const float* qu = Q + (long long)10 * u;
   const float* qv = Q + (long long)10 * v;
   float s[10];
 #pragma unroll
-  for (int i = 0; i < 10; i++) s[i] = qu[i] + qv[i];
+  for (int j = 0; j < 10; j++) s[j] = qu[j] + qv[j];
   float t0 = s[0] * m.x + s[1] * m.y + s[2] * m.z + s[3] * 1.0f;
   float t1 = s[1] * m.x + s[4] * m.y + s[5] * m.z + s[6] * 1.0f;
   float t2 = s[2] * m.x + s[5] * m.y + s[7] * m.z + s[8] * 1.0f;
@@ -262,17 +285,31 @@ __global__ void k_gather_cost(const uint32_t* __restrict__ cost_reg, const int* 
   flat[e] = cost_reg[mp[k].eco + (e - eoffm[k])];
 }
 
-__global__ void __launch_bounds__(BLOCK) k_claim_b(const int* __restrict__ edges, const int* __restrict__ E_ptr,
-                                                   const uint32_t* __restrict__ cost, const int* __restrict__ fan_off,
-                                                   const int* __restrict__ fan, u64* tri_min) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= *E_ptr) return;
-  u64 key = ((u64)cost[e] << 32) | (u64)(uint32_t)e;
-  for (int w = 0; w < 2; w++) {
-    int x = edges[2 * e + w];
-    int a = fan_off[x], b = fan_off[x + 1];
-    for (int s = a; s < b; s++) atomicMin(&tri_min[fan[s]], key);
-  }
+// ---------------------------------------------------------------- claim
+// The reference takes, per alive face, the minimum key (cost << 32 | edge)
+// over every edge whose endpoint's fan holds the face (cusimp_free.cu:
+// 587-610).  A face is in the fan of x exactly when x is one of its
+// vertices, so that minimum is the minimum over the face's three vertices
+// of the per-vertex minimum over incident edges: two atomics per edge
+// instead of one per fan face, and min is exact in any order.
+__global__ void __launch_bounds__(BLOCK) k_vertex_min(const int4* __restrict__ vedge, const int* __restrict__ E_ptr,
+                                                      const uint32_t* __restrict__ cost, u64* vmin) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= *E_ptr) return;
+  int4 ev = vedge[i];
+  u64 key = ((u64)cost[ev.x] << 32) | (u64)(uint32_t)ev.x;
+  atomicMin(&vmin[ev.y], key);
+  atomicMin(&vmin[ev.z], key);
+}
+__global__ void __launch_bounds__(BLOCK) k_face_min(const int* __restrict__ T, int F, const u64* __restrict__ vmin,
+                                                    u64* __restrict__ tri_min) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  I3 t = ldt(T, f);
+  if (dead3(t)) return;   // dead faces are in no fan and never read
+  u64 a = vmin[t.i], b = vmin[t.j], c = vmin[t.k];
+  u64 m = a < b ? a : b;
+  tri_min[f] = m < c ? m : c;
 }
 
 __global__ void __launch_bounds__(BLOCK) k_collapse_b(const int* __restrict__ edges, const int* __restrict__ E_ptr,
@@ -750,6 +787,174 @@ __global__ void __launch_bounds__(BLOCK) k_traverse_b(const int* __restrict__ ql
       atomicAdd(&stats[1], h64);
     }
   }
+}
+
+// ------------------------------------------------ traversal in two passes
+// The walk and the narrow phase run in separate kernels.  In one kernel a
+// wave runs the long FP64 pair test in every loop step in which any of its
+// lanes sits on an overlapping leaf, so its time is (steps of the longest
+// walk) x (pair test) whatever the number of queries.  Here the walk only
+// collects, per query, the overlapping faces that do not share an edge with
+// it (pair_hits rejects those outright), and the pair kernel tests one pair
+// per lane.  The verdict per query is the OR over its pairs, the same as the
+// walk that stops at the first hit.
+//
+// A query may be split over 2^kshift lanes, each walking one subtree at
+// depth kshift of the mesh's tree: the subtrees partition the leaves, so the
+// union of the lanes' candidates is the query's candidate set, and the
+// longest dependent chain of node loads shrinks for queries with many
+// candidates (what bounds the kernel when there are few queries).
+//
+// Candidates collect in LDS (slot j of lane t at j*BLOCK+t, conflict-free);
+// a lane whose list fills moves it to the pair buffer with an atomic of its
+// own, and after the walk each wave reserves room for what its lanes still
+// hold with one atomic.  Pairs are (query face, other face).  A query whose
+// pairs do not fit the pair buffer goes to the overflow list, which
+// k_traverse_b walks in full.
+constexpr int CAND_CAP = 32;
+__global__ void __launch_bounds__(BLOCK) k_traverse_emit(
+    const int* __restrict__ qlist, const int* __restrict__ nq_ptr, const int* __restrict__ n_ptr,
+    const int* __restrict__ fmesh, const int* __restrict__ root, const int* __restrict__ root_end,
+    const float4* __restrict__ nodeA, const float4* __restrict__ nodeB, const float* __restrict__ P,
+    const int* __restrict__ T, int2* __restrict__ pairs, long long pair_cap, unsigned long long* __restrict__ npairs,
+    int* __restrict__ ovf, int* __restrict__ novf, unsigned long long* __restrict__ stats, int* err, int kshift) {
+  __shared__ int cl[CAND_CAP * BLOCK];
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const long long gid = (long long)blockIdx.x * blockDim.x + t;
+  const long long qi = gid >> kshift;
+  const int part = (int)(gid & ((1 << kshift) - 1));
+  int nq = *nq_ptr;
+  unsigned cand = 0;
+  int cnt = 0;
+  int f = -1;
+  bool full = false;   // the pair buffer ran out: walk this query in full instead
+  if (qi < nq) {
+    const long long cap = 2LL * (2LL * (*n_ptr) - 1) + 8;
+    f = qlist[qi];
+    int k = fmesh[f];
+    I3 tf = ldt(T, f);
+    float qlo[3], qhi[3];
+    {
+      V3 a = ldp(P, tf.i), b = ldp(P, tf.j), c = ldp(P, tf.k);
+      qlo[0] = fminf(a.x, fminf(b.x, c.x)); qhi[0] = fmaxf(a.x, fmaxf(b.x, c.x));
+      qlo[1] = fminf(a.y, fminf(b.y, c.y)); qhi[1] = fmaxf(a.y, fmaxf(b.y, c.y));
+      qlo[2] = fminf(a.z, fminf(b.z, c.z)); qhi[2] = fmaxf(a.z, fmaxf(b.z, c.z));
+    }
+    // this lane's subtree: 2^kshift parts of the mesh's tree at depth
+    // kshift (the right child of a node is its left child's escape); a
+    // leaf reached early belongs to the part whose remaining bits are zero
+    int node = root[k], end = root_end[k];
+    bool mine_part = true;
+    for (int b = kshift - 1; b >= 0 && mine_part; b--) {
+      int c = __float_as_int(nodeB[node].w);
+      if (c < 0) {
+        mine_part = (part & ((2 << b) - 1)) == 0;
+        break;
+      }
+      node = ((part >> b) & 1) ? __float_as_int(nodeA[c].w) : c;
+    }
+    if (kshift > 0) end = __float_as_int(nodeA[node].w);
+    if (!mine_part) node = end;
+    long long steps = 0;
+    while (node != end && node >= 0) {
+      if (++steps > cap) { atomicAdd(err, 1); break; }
+      float4 A = nodeA[node], B = nodeB[node];
+      bool ov = !(qhi[0] < A.x || B.x < qlo[0]) && !(qhi[1] < A.y || B.y < qlo[1]) && !(qhi[2] < A.z || B.z < qlo[2]);
+      int esc = __float_as_int(A.w);
+      if (ov) {
+        int c = __float_as_int(B.w);
+        if (c >= 0) { node = c; continue; }
+        int g = ~c;
+        if (g != f) {
+          cand++;
+          if (shared_indices(ldt(T, g), tf) != 2) {
+            if (cnt == CAND_CAP) {   // LDS list full: move it to the pair buffer
+              long long b = (long long)atomicAdd(npairs, (unsigned long long)CAND_CAP);
+              for (int j = 0; j < CAND_CAP; j++)
+                if (b + j < pair_cap) pairs[b + j] = make_int2(f, cl[j * BLOCK + t]);
+              cnt = 0;
+              if (b + CAND_CAP > pair_cap) { full = true; break; }
+            }
+            cl[cnt * BLOCK + t] = g;
+            cnt++;
+          }
+        }
+      }
+      node = esc;
+    }
+  }
+  // wave reservation for what is left in LDS
+  const bool small = f >= 0 && !full;
+  int mine = small ? cnt : 0;
+  int incl = mine;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  int total = __shfl(incl, 63, 64);
+  unsigned long long base = 0;
+  if (lane == 63 && total) base = atomicAdd(npairs, (unsigned long long)total);
+  base = __shfl(base, 63, 64);
+  long long at = (long long)base + (incl - mine);
+  // a list that crosses the end of the buffer still fills the slots below
+  // it (every slot under min(npairs, pair_cap) is written by one lane) and
+  // its query is also walked in full, whose verdict then stands
+  const bool fits = small && at + mine <= pair_cap;
+  if (f >= 0 && !fits) ovf[atomicAdd(novf, 1)] = f;
+  for (int j = 0; j < mine; j++)
+    if (at + j < pair_cap) pairs[at + j] = make_int2(f, cl[j * BLOCK + t]);
+  if (stats) {
+    unsigned long long c64 = cand;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c64 += __shfl_xor(c64, o, 64);
+    if (lane == 0 && c64) atomicAdd(&stats[0], c64);
+  }
+}
+
+// One candidate pair per lane (grid-stride over the pairs the walk wrote):
+// both orientations of the reference narrow phase; a hit flags the query.
+__global__ void __launch_bounds__(BLOCK) k_pair_test(const int2* __restrict__ pairs,
+                                                     const unsigned long long* __restrict__ npairs, long long pair_cap,
+                                                     const float* __restrict__ P, const int* __restrict__ T,
+                                                     uint8_t* __restrict__ S, unsigned long long* __restrict__ stats) {
+  long long n = (long long)*npairs;
+  if (n > pair_cap) n = pair_cap;
+  if (stats && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[2], (unsigned long long)n);
+  unsigned hits = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    int2 pr = pairs[i];
+    I3 tf = ldt(T, pr.x), tg = ldt(T, pr.y);
+    float pf[3][3], pg[3][3];
+    {
+      V3 a = ldp(P, tf.i), b = ldp(P, tf.j), c = ldp(P, tf.k);
+      pf[0][0] = a.x; pf[0][1] = a.y; pf[0][2] = a.z;
+      pf[1][0] = b.x; pf[1][1] = b.y; pf[1][2] = b.z;
+      pf[2][0] = c.x; pf[2][1] = c.y; pf[2][2] = c.z;
+    }
+    {
+      V3 a = ldp(P, tg.i), b = ldp(P, tg.j), c = ldp(P, tg.k);
+      pg[0][0] = a.x; pg[0][1] = a.y; pg[0][2] = a.z;
+      pg[1][0] = b.x; pg[1][1] = b.y; pg[1][2] = b.z;
+      pg[2][0] = c.x; pg[2][1] = c.y; pg[2][2] = c.z;
+    }
+    if (pair_hits(pf, pg, tf, tg)) {
+      S[pr.x] = 1;
+      hits++;
+    }
+  }
+  if (stats) {
+    unsigned long long h64 = hits;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) h64 += __shfl_xor(h64, o, 64);
+    if ((threadIdx.x & 63) == 0 && h64) atomicAdd(&stats[1], h64);
+  }
+}
+
+__global__ void k_pairs_reset(int* novf, unsigned long long* npairs) {
+  *novf = 0;
+  *npairs = 0;
 }
 
 // --------------------------------------------------------------- marks

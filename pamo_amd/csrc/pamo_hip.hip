@@ -33,7 +33,7 @@ struct Buf {
 
 enum Counter {
   C_E = 0, C_NFAN, C_NWIN, C_NCOLL, C_NLINE, C_CUT, C_NALIVE, C_NQUERY, C_NMARK, C_ERR, C_NEWV, C_NEWF,
-  C_V, C_F, C_CUTTOT, C_COUNT = 16
+  C_V, C_F, C_CUTTOT, C_NOVF, C_COUNT = 16
 };
 
 inline unsigned nblk(long long n) { return (unsigned)std::max<long long>(1, (n + BLOCK - 1) / BLOCK); }
@@ -68,13 +68,16 @@ struct pamo_ctx {
   // round buffers
   Buf P, P0, occ, T, T0, cnt, fan_off, fan, ecnt, eoff, edges, Q, tri_min, win, winpos, collapsed;
   Buf qflag, qf, qpos, af, apos, qlist, leaf_face, lbox_lo, lbox_hi, keys0, keys1, vals0, vals1, hist;
-  Buf kleft, kright, kparent, rfirst, rlast, flags, nodeA, nodeB, S, S_first, mflag, mpos, marked;
-  Buf scan_sums, counters, cstats, shuf_flags, shuf_pos;
+  Buf pairs, ovf, seg_lo, seg_hi;
+  Buf kleft, kright, kparent, rfirst, rlast, nodeA, nodeB, S, S_first, mflag, mpos, marked;
+  Buf scan_sums, scan_desc, counters, cstats, shuf_flags, shuf_pos;
+  unsigned scan_epoch = 0;   // epoch 0 never names a call, so zeroed words read as unpublished
   // run loop
   Buf RV[2], RF[2], keep, kpos, undo_in;
   int run_cur = 0, run_V = 0, run_F = 0;
   std::vector<int> run_faces;   // faces after each round of the last run
   int run_handle = -1;          // engine slot of the last pamo_run_stage2
+  long long pair_cap_limit = -1;  // test hook: caps the traversal's pair buffer (-1: none)
 
   int* hc = nullptr;   // pinned host mirror of the counters
 
@@ -181,28 +184,33 @@ struct pamo_ctx {
     k_fill_i32<<<1, 1, 0, st>>>(cnt_dev(i), v, 1);
     return check_launch("set counter");
   }
-  // exclusive scan of n ints; in-place allowed; total -> counter slot (or -1)
+  // exclusive scan of n ints (non-negative); in-place allowed; total ->
+  // counter slot (or -1).  One launch (decoupled look-back).
   bool scan(const int* in, int* out, long long n, int total_slot) {
     if (n <= 0) {
       if (total_slot >= 0) return set_counter(total_slot, 0);
       return true;
     }
-    long long nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-    if (!ensure(scan_sums, (size_t)nb * 4)) return false;
-    k_scan_reduce<<<(unsigned)nb, BLOCK, 0, st>>>(in, n, ptr<int>(scan_sums));
-    k_scan_top<<<1, BLOCK, 0, st>>>(ptr<int>(scan_sums), (int)nb, total_slot >= 0 ? cnt_dev(total_slot) : nullptr);
-    k_scan_down<<<(unsigned)nb, BLOCK, 0, st>>>(in, out, n, ptr<int>(scan_sums));
-    return check_launch("scan");
+    return scan_launch(in, out, n, nullptr, 0, total_slot >= 0 ? cnt_dev(total_slot) : nullptr);
   }
   // the same scan over a capacity n whose live length is on the device
-  // (*d_n + add); entries past the live length count as zero
+  // (*d_n + add); entries past the live length are neither read nor written
   bool scan_dev(const int* in, int* out, long long n, const int* d_n, int add) {
     if (n <= 0) return true;
+    return scan_launch(in, out, n, d_n, add, nullptr);
+  }
+  bool scan_launch(const int* in, int* out, long long n, const int* d_n, int add, int* total) {
     long long nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-    if (!ensure(scan_sums, (size_t)nb * 4)) return false;
-    k_scan_reduce_dev<<<(unsigned)nb, BLOCK, 0, st>>>(in, n, d_n, add, ptr<int>(scan_sums));
-    k_scan_top<<<1, BLOCK, 0, st>>>(ptr<int>(scan_sums), (int)nb, nullptr);
-    k_scan_down_dev<<<(unsigned)nb, BLOCK, 0, st>>>(in, out, n, d_n, add, ptr<int>(scan_sums));
+    if (scan_desc.bytes < (size_t)nb * 8) {
+      if (!ensure(scan_desc, (size_t)nb * 8)) return false;
+      if (!hip(hipMemsetAsync(scan_desc.p, 0, scan_desc.bytes, st), "scan desc")) return false;
+    }
+    if (++scan_epoch >= (1u << 22)) {   // epoch wrap: clear the descriptors
+      scan_epoch = 1;
+      if (!hip(hipMemsetAsync(scan_desc.p, 0, scan_desc.bytes, st), "scan desc")) return false;
+    }
+    k_scan_onepass<<<(unsigned)nb, BLOCK, 0, st>>>(in, out, n, d_n, add, ptr<unsigned long long>(scan_desc),
+                                                   scan_epoch, total, cnt_dev(C_ERR));
     return check_launch("scan");
   }
   bool read_counters() {
@@ -231,6 +239,7 @@ struct pamo_ctx {
   }
 
   bool build_bvh_and_traverse(int Fn, bool debug_pairs, std::vector<int>* pairs_out);
+  bool node_boxes(const int* n_alive, const int* sorted_leaf, long long Fl);
   bool check_and_mark(int* n_marked);
   int forward(const float* d_pts, const int* d_tris, const int* d_undo, int n_u, int nV, int nF, float scale,
               float thr, int is_stuck, int init);
@@ -245,9 +254,9 @@ pamo_ctx::~pamo_ctx() {
   eng_free(this);
   Buf* all[] = {&pts_map, &table, &edge_cost, &undo_list, &tmp_undo, &P, &P0, &occ, &T, &T0, &cnt, &fan_off, &fan,
                 &ecnt, &eoff, &edges, &Q, &tri_min, &win, &winpos, &collapsed, &qflag, &qf, &qpos, &af, &apos,
-                &qlist, &leaf_face, &lbox_lo, &lbox_hi, &keys0, &keys1, &vals0, &vals1, &hist, &kleft, &kright,
-                &kparent, &rfirst, &rlast, &flags, &nodeA, &nodeB, &S, &S_first, &mflag, &mpos, &marked,
-                &scan_sums, &counters, &cstats, &shuf_flags, &shuf_pos, &RV[0], &RV[1], &RF[0], &RF[1], &keep,
+                &qlist, &pairs, &ovf, &seg_lo, &seg_hi, &leaf_face, &lbox_lo, &lbox_hi, &keys0, &keys1, &vals0, &vals1, &hist, &kleft, &kright,
+                &kparent, &rfirst, &rlast, &nodeA, &nodeB, &S, &S_first, &mflag, &mpos, &marked,
+                &scan_sums, &scan_desc, &counters, &cstats, &shuf_flags, &shuf_pos, &RV[0], &RV[1], &RF[0], &RF[1], &keep,
                 &kpos, &undo_in};
   for (Buf* b : all)
     if (b->p) (void)hipFree(b->p);
@@ -256,13 +265,32 @@ pamo_ctx::~pamo_ctx() {
   if (hc) (void)hipHostFree(hc);
 }
 
+// Node boxes of the LBVH over n_alive sorted leaves (capacity Fl): in-block
+// merge plus segment levels 1..8, higher segment levels, then the nodes that
+// span blocks.
+bool pamo_ctx::node_boxes(const int* n_alive, const int* sorted_leaf, long long Fl) {
+  if (!ensure(seg_lo, (size_t)seg_entries(Fl) * 16) || !ensure(seg_hi, (size_t)seg_entries(Fl) * 16)) return false;
+  k_bottom_up_local<<<nblk(Fl), BLOCK, 0, st>>>(n_alive, ptr<float4>(lbox_lo), ptr<float4>(lbox_hi), sorted_leaf,
+                                               ptr<int>(leaf_face), ptr<int>(kparent), ptr<int>(kleft),
+                                               ptr<int>(kright), ptr<int>(rfirst), ptr<int>(rlast), ptr<float4>(nodeA),
+                                               ptr<float4>(nodeB), Fl, ptr<float4>(seg_lo), ptr<float4>(seg_hi));
+  for (int j0 = 8; (1LL << j0) < Fl; j0 += 8) {
+    long long entries = (Fl + (1LL << j0) - 1) >> j0;
+    k_seg_upper<<<nblk(entries), BLOCK, 0, st>>>(n_alive, Fl, j0, ptr<float4>(seg_lo), ptr<float4>(seg_hi));
+  }
+  k_cross_nodes<<<nblk(Fl), BLOCK, 0, st>>>(n_alive, Fl, ptr<int>(kleft), ptr<int>(rfirst), ptr<int>(rlast),
+                                           ptr<float4>(seg_lo), ptr<float4>(seg_hi), ptr<float4>(nodeA),
+                                           ptr<float4>(nodeB));
+  return check_launch("node boxes");
+}
+
 // BVH over the alive faces, then the restricted traversal (query list in
 // qlist/C_NQUERY) or, for the debug export, the full pair list.
 bool pamo_ctx::build_bvh_and_traverse(int Fn, bool debug_pairs, std::vector<int>* pairs_out) {
   const long long Fl = std::max(Fn, 1);
   if (!ensure(lbox_lo, Fl * 16) || !ensure(lbox_hi, Fl * 16) || !ensure(keys0, Fl * 4) || !ensure(keys1, Fl * 4) ||
       !ensure(vals0, Fl * 4) || !ensure(vals1, Fl * 4) || !ensure(kleft, Fl * 4) || !ensure(kright, Fl * 4) ||
-      !ensure(kparent, 2 * Fl * 4) || !ensure(rfirst, Fl * 4) || !ensure(rlast, Fl * 4) || !ensure(flags, Fl * 4) ||
+      !ensure(kparent, 2 * Fl * 4) || !ensure(rfirst, Fl * 4) || !ensure(rlast, Fl * 4) ||
       !ensure(nodeA, 2 * Fl * 16) || !ensure(nodeB, 2 * Fl * 16))
     return false;
   int* n_alive = cnt_dev(C_NALIVE);
@@ -292,17 +320,10 @@ bool pamo_ctx::build_bvh_and_traverse(int Fn, bool debug_pairs, std::vector<int>
     std::swap(kin, kout);
     std::swap(vin, vout);
   }
-  if (!fill_i32(flags.p, 0, Fl)) return false;
   k_fill_i32<<<1, 1, 0, st>>>(ptr<int>(kparent), -1, 1);
   k_karras<<<nblk(Fl), BLOCK, 0, st>>>(kin, n_alive, ptr<int>(kleft), ptr<int>(kright), ptr<int>(kparent),
                                       ptr<int>(rfirst), ptr<int>(rlast));
-  k_bottom_up_local<<<nblk(Fl), BLOCK, 0, st>>>(n_alive, ptr<float4>(lbox_lo), ptr<float4>(lbox_hi), vin,
-                                               ptr<int>(leaf_face), ptr<int>(kparent), ptr<int>(kleft),
-                                               ptr<int>(kright), ptr<int>(rfirst), ptr<int>(rlast), ptr<float4>(nodeA),
-                                               ptr<float4>(nodeB));
-  k_bottom_up_global<<<nblk(2 * Fl), BLOCK, 0, st>>>(n_alive, ptr<int>(kparent), ptr<int>(kleft), ptr<int>(kright),
-                                                    ptr<int>(rfirst), ptr<int>(rlast), ptr<int>(flags),
-                                                    ptr<float4>(nodeA), ptr<float4>(nodeB));
+  if (!node_boxes(n_alive, vin, Fl)) return false;
   prof_end(pbv);
   if (!check_launch("lbvh")) return false;
   if (!debug_pairs) {
@@ -549,7 +570,7 @@ static void eng_free(pamo_ctx* c) {
                 &g.undo_out, &g.tmp, &g.marked, &g.pm, &g.table, &g.ecost, &g.mp, &g.voff, &g.foff, &g.eoffm,
                 &g.uioff, &g.uioff2, &g.nustk, &g.h, &g.lcnt, &g.ncoll, &g.cut, &g.cutoff, &g.uobase, &g.hm, &g.part,
                 &g.first, &g.nvu, &g.nvoff, &g.nfoff, &g.seg_a, &g.seg_b, &g.root, &g.root_end, &g.scene,
-                &g.flat_cost, &g.g, &g.keep, &g.kpos, &g.newk, &g.vkeep, &g.vpos, &g.fsucc, &g.fplane, &g.flen};
+                &g.flat_cost, &g.g, &g.keep, &g.kpos, &g.newk, &g.vkeep, &g.vpos, &g.fsucc, &g.sface, &g.splane, &g.slen, &g.erank, &g.vedge, &g.uoff, &g.vmin};
   for (Buf* b : all)
     if (b->p) (void)hipFree(b->p);
   for (auto& s : g.slots) {
@@ -654,6 +675,12 @@ int pamo_batch_take(pamo_ctx* ctx, void* stream, int handle, float* d_verts_out,
   s.outV = Buf();
   s.outF = Buf();
   s.taken = true;
+  return 0;
+}
+
+int pamo_debug_pair_cap(pamo_ctx* ctx, long long cap) {
+  if (!ctx) return -1;
+  ctx->pair_cap_limit = cap;
   return 0;
 }
 

@@ -190,6 +190,84 @@ __global__ void __launch_bounds__(BLOCK) k_scan_down_dev(const int* __restrict__
   }
 }
 
+// ------------------------------------------------ one-pass exclusive scan
+// Decoupled look-back: tile t (one block, SCAN_TILE ints) scans itself,
+// publishes its aggregate, adds the aggregates/prefixes of earlier tiles
+// and publishes its inclusive prefix.  A tile's status and value travel in
+// one 64-bit word written and read with relaxed device-scope atomics, so no
+// ordering between separate locations (and no L2 write-back or invalidate)
+// is needed.  Words carry the call's epoch, so stale words from earlier
+// calls read as "not yet published" and the array never needs clearing
+// (except when the 22-bit epoch wraps, which the host handles).  Blocks are
+// dispatched in index order, so every tile a block waits on is running or
+// done; the spin is capped anyway and reports through err.  Inputs are
+// non-negative (counts and flags); totals stay below 2^40.
+constexpr unsigned long long SCAN_ST_AGG = 1ull, SCAN_ST_PREFIX = 2ull;
+__device__ __forceinline__ unsigned long long scan_word(unsigned epoch, unsigned long long st, long long v) {
+  return ((unsigned long long)epoch << 42) | (st << 40) | ((unsigned long long)v & ((1ull << 40) - 1));
+}
+__global__ void __launch_bounds__(BLOCK) k_scan_onepass(const int* __restrict__ in, int* __restrict__ out, long long n,
+                                                        const int* __restrict__ d_n, int add,
+                                                        unsigned long long* desc, unsigned epoch,
+                                                        int* __restrict__ total, int* err) {
+  __shared__ int lds[BLOCK / 64];
+  __shared__ int tile[SCAN_TILE];
+  __shared__ long long excl_s;
+  const long long live = d_n ? min(n, (long long)*d_n + add) : n;
+  const long long t = blockIdx.x;
+  const long long base = t * SCAN_TILE;
+  if (live <= 0) {
+    if (t == 0 && threadIdx.x == 0 && total) *total = 0;
+    return;
+  }
+  if (base >= live) return;   // tiles past the live length contribute nothing
+  for (int i = 0; i < SCAN_ITEMS; i++) {
+    long long idx = base + (long long)i * BLOCK + threadIdx.x;
+    tile[i * BLOCK + threadIdx.x] = idx < live ? in[idx] : 0;
+  }
+  __syncthreads();
+  int loc[SCAN_ITEMS];
+  int s = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; i++) { loc[i] = s; s += tile[threadIdx.x * SCAN_ITEMS + i]; }
+  int tot;
+  int off = block_exclusive_scan(s, lds, tot);
+  if (threadIdx.x == 0) {
+    long long ex = 0;
+    if (t == 0) {
+      __hip_atomic_store(&desc[0], scan_word(epoch, SCAN_ST_PREFIX, tot), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      __hip_atomic_store(&desc[t], scan_word(epoch, SCAN_ST_AGG, tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      long long j = t - 1;
+      long long spins = 0;
+      while (j >= 0) {
+        unsigned long long w = __hip_atomic_load(&desc[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((unsigned)(w >> 42) != epoch) {
+          if (++spins > (1LL << 20)) { atomicAdd(err, 1); break; }
+          continue;
+        }
+        ex += (long long)(w & ((1ull << 40) - 1));
+        if (((w >> 40) & 3ull) == SCAN_ST_PREFIX) break;
+        j--;
+      }
+      __hip_atomic_store(&desc[t], scan_word(epoch, SCAN_ST_PREFIX, ex + tot), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
+    excl_s = ex;
+    if (total && base + SCAN_TILE >= live) *total = (int)(ex + tot);
+  }
+  __syncthreads();
+  off += (int)excl_s;
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; i++) tile[threadIdx.x * SCAN_ITEMS + i] = off + loc[i];
+  __syncthreads();
+  for (int i = 0; i < SCAN_ITEMS; i++) {
+    long long idx = base + (long long)i * BLOCK + threadIdx.x;
+    if (idx < live) out[idx] = tile[i * BLOCK + threadIdx.x];
+  }
+}
+
 // ------------------------------------------------------------ step 1
 // cusimp_free.cu:869-878 (stale pts_map remap) and :880-889 (table)
 __global__ void k_remap_undo(const int* __restrict__ tmp, const int* __restrict__ pts_map, long long map_n,

@@ -99,3 +99,21 @@ def test_server_results_match(cuda):
     for i, (v, f) in enumerate(meshes):
         ov, of, _ = _oracle_result(v, f, 0.01)
         assert _same(out[i][0], ov) and _same(out[i][1], of), i
+
+
+@pytest.mark.parametrize("cap", [0, 333])
+def test_pair_overflow_fallback(cuda, cap):
+    """A tiny candidate-pair buffer sends most queries (cap 0: all with a
+    candidate) to the single-kernel walk; the result must not change."""
+    from pamo_amd import _lib
+    ctx = _lib.Context(0)
+    ctx.debug_pair_cap(cap)
+    for v, f in (_meshes()[0], _two_spheres()):
+        verts, scale, _ = oracle.preprocess(v, f)
+        target = int(0.01 * f.shape[0])
+        ov, of, st = oracle.run(verts, f, scale, target)
+        gv, gf, gst = simplify_stage2(torch.from_numpy(verts).to(cuda), torch.from_numpy(f).to(cuda), scale, target,
+                                      ctx=ctx)
+        print("cap", cap, "mesh", f.shape, gst["rounds"], st["rounds"], flush=True)
+        assert gst["rounds"] == st["rounds"]
+        assert _same(gv.cpu().numpy(), ov) and _same(gf.cpu().numpy(), of)
