@@ -33,6 +33,34 @@
 //      of collapsed-list position; the reference fills them in atomicAdd order.
 //   collapse threshold conversion saturates like the device conversion (the
 //      host conversion at cusimp_free.cu:923 is undefined only for tau >= 10).
+//
+// What pins each part (tests/test_oracle.py unless noted); no part is left
+// without a check, so nothing here is "parity unpinned" beyond the
+// reference's own undefined behaviour that D0-D9 replace:
+//   shuffle_permutation   rocThrust's own permutations as golden vectors
+//                         (tests/golden/), bijection for many sizes
+//   build_quadrics        closed form on the octahedron; zero QEM on a plane
+//   edge_cost_one         closed form on the octahedron; quantization edge
+//                         cases (collapse_t(1e-3) = 429496, NaN, saturation)
+//   claim                 brute-force per-face minimum over all edges, and the
+//                         winners' fans are disjoint
+//   collapse, restore,    per-round invariants over 12 rounds (closed,
+//   remove_invalid_faces  manifold, Euler characteristic, two faces per net
+//                         collapse, unchanged faces equal the snapshot)
+//   remove_line_edges     sphere with one zero-length edge: the only collapse,
+//                         the two faces it kills, the faces it retargets and
+//                         the invariants of the compacted result
+//   narrow_pair, isect    exact orient3d (checked against rationals) on generic
+//                         pairs; filter outcome codes on constructed coplanar,
+//                         shared-edge and shared-vertex pairs
+//   CpuBVH, check         brute-force O(F^2) AABB candidate sets
+//   mark, undo loop       an early-ending undo loop leaves no kept collapse
+//                         with an intersecting face in its fans (brute force
+//                         over the final mesh), undone vertices are restored;
+//                         interpenetrating input runs to the 5-pass cap
+//   run_loop              termination by target and by tolerance; with no
+//                         collapse possible it stops after exactly `tolerance`
+//                         rounds and returns the mesh unchanged
 
 #include <algorithm>
 #include <atomic>

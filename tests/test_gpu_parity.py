@@ -29,7 +29,18 @@ def _two_spheres(level=4, seed=3, offset=0.9):
     return np.concatenate([v1, v2]).astype(np.float32), np.concatenate([f1, f2 + v1.shape[0]]).astype(np.int32)
 
 
+def _zero_length_edge(level=4, seed=5):
+    """Bumpy sphere with one zero-length edge: exercises the line-edge
+    collapse (cusimp_free.cu:234-274) after the regular winners."""
+    v, f = synth.displaced_icosphere(level, seed)
+    v = v.copy()
+    a, b = sorted(int(x) for x in f[0, :2])
+    v[b] = v[a]
+    return v, f
+
+
 MESHES = {
+    "line_edge": _zero_length_edge,
     "ico3": lambda: synth.displaced_icosphere(3, 7),
     "ico5": lambda: synth.displaced_icosphere(5, 1),
     "torus": lambda: synth.noisy_torus(60, 24, 5),
@@ -38,7 +49,7 @@ MESHES = {
 }
 
 
-@pytest.mark.parametrize("name", ["ico3", "torus", "sdf", "two_spheres"])
+@pytest.mark.parametrize("name", ["ico3", "torus", "sdf", "two_spheres", "line_edge"])
 def test_forward_round_exact(cuda, name):
     v, f = MESHES[name]()
     verts, scale, _ = oracle.preprocess(v, f)
@@ -105,7 +116,7 @@ def test_rounds_exact(cuda, name):
     assert it > 5
 
 
-@pytest.mark.parametrize("name", ["ico5", "two_spheres", "sdf"])
+@pytest.mark.parametrize("name", ["ico5", "two_spheres", "sdf", "line_edge"])
 def test_device_loop_exact(cuda, name):
     """pamo_run_stage2 (device-resident loop) == the oracle's PaMO.run loop."""
     v, f = MESHES[name]()

@@ -286,6 +286,51 @@ def test_round_invariants(mesh):
         assert res["euler"] == chi0
 
 
+def _zero_length_edge(level=2):
+    """Unit icosphere whose first face's first edge has length zero (its
+    larger vertex moved onto the smaller)."""
+    v, f = synth.icosphere(level)
+    v = np.asarray(v, np.float32).copy()
+    f = np.asarray(f, np.int32)
+    a, b = sorted(int(x) for x in f[0, :2])
+    v[b] = v[a]
+    return v, f, a, b
+
+
+def test_line_edge_collapse():
+    """cusimp_free.cu:234-274 on a closed sphere with one zero-length edge
+    (a, b), a < b, and a threshold that admits no regular collapse: the
+    edge is the only collapsed one, b dies, the two faces on (a, b) are
+    removed, every other face of b now uses a, and the compacted result is
+    closed, manifold, has the same Euler characteristic and no alive face
+    with two coincident corners."""
+    v, f, a, b = _zero_length_edge()
+    chi0 = oracle.verify(v, f)["euler"]
+    verts, scale, _ = oracle.preprocess(v, f)
+    assert np.array_equal(verts[a], verts[b])
+    o = oracle.Oracle(shuffle=oracle.SHUFFLE_IDENTITY)
+    P, T, occ, vmap, undo = o.forward(verts, f, np.zeros(0, np.int32), scale, 0.0, False, True)
+    sz = o.sizes()
+    assert sz["n_winners"] == 0 and sz["n_collapsed"] == 1
+    E = o.get("edges")
+    assert sorted(E[o.get("collapsed")[0]].tolist()) == [a, b]
+    assert occ[b, 0] == 0 and np.all(P[b] == 0) and int(occ.sum()) == v.shape[0] - 1
+    both = np.isin(f, [a]).any(1) & np.isin(f, [b]).any(1)
+    assert both.sum() == 2 and np.all(T[both] == -1)
+    alive = T[:, 0] >= 0
+    assert alive.sum() == f.shape[0] - 2 and not np.isin(T[alive], [b]).any()
+    had_b = np.isin(f, [b]).any(1) & ~both
+    assert np.all(np.where(f[had_b] == b, a, f[had_b]) == T[had_b])
+    assert np.all(T[~had_b & ~both] == f[~had_b & ~both])
+    nv = P[occ.ravel() == 1]
+    nf = vmap.ravel()[T[alive]].astype(np.int32)
+    res = oracle.verify(nv, nf)
+    assert res["manifold"] and res["watertight"] and res["euler"] == chi0, res
+    corners = nv[nf]
+    assert not np.any(np.all(corners[:, 0] == corners[:, 1], 1) | np.all(corners[:, 1] == corners[:, 2], 1) |
+                      np.all(corners[:, 2] == corners[:, 0], 1))
+
+
 def test_run_reaches_target_or_tolerance():
     v, f = synth.displaced_icosphere(4, 8)
     verts, scale, _ = oracle.preprocess(v, f)
@@ -298,6 +343,21 @@ def test_run_reaches_target_or_tolerance():
         assert rf[-1] == rf[-2] == rf[-3] == rf[-4] == rf[-5]
     assert np.all(np.diff(rf) <= 0)
     assert oracle.verify(ov, of)["ok"]
+
+
+@pytest.mark.parametrize("tol", [1, 4, 7])
+def test_run_without_collapses_stops_after_tolerance_rounds(tol):
+    """Threshold 0 admits no collapse on a sphere (every quantized cost is
+    above 0): the face count never changes, so PaMO.run's counter reaches
+    `tolerance` after exactly `tolerance` rounds (simp_cuda/pamo/__init__.py:
+    147-158) and the mesh comes back unchanged (identity shuffle)."""
+    v, f = synth.displaced_icosphere(3, 4)
+    verts, scale, _ = oracle.preprocess(v, f)
+    ov, of, st, rf = oracle.run(verts, f, scale, 0, threshold=0.0, tolerance=tol, shuffle=oracle.SHUFFLE_IDENTITY,
+                                return_rounds=True)
+    assert st["rounds"] == tol and st["exit_reason"] == 3
+    assert list(rf) == [f.shape[0]] * tol
+    assert np.array_equal(of, f) and np.array_equal(ov, verts)
 
 
 def test_undo_loop_on_intersecting_input():
@@ -315,6 +375,44 @@ def test_undo_loop_on_intersecting_input():
     und = o.get("undo")
     assert und.shape[0] == s["n_vertices_undo"]
     assert np.all(und == 0)          # first round: the stale map is zero-filled (D1)
+
+
+def test_undo_leaves_no_flagged_fan():
+    """A round whose undo loop ends before its cap (SDF union, round 76):
+    the final geometry has no kept collapse of the scanned prefix whose
+    round-start fans hold a face of an intersecting pair (brute force over
+    all ordered pairs of the final mesh), every undone edge has its far
+    vertex back at its round-start position, and with one pass the undo
+    list holds exactly the undone edges' two vertices."""
+    v, f = synth.sdf_union(60, 100, 5)
+    verts, scale, _ = oracle.preprocess(v, f)
+    o = oracle.Oracle()
+    undo = np.zeros(0, np.int32)
+    for r in range(76):
+        P0 = verts
+        P, T, occ, vmap, undo = o.forward(verts, f, undo, scale, 1e-3, False, r == 0)
+        if r < 75:
+            alive = T[:, 0] >= 0
+            verts = P[occ.ravel() == 1]
+            f = vmap.ravel()[T[alive]].astype(np.int32)
+    s = o.sizes()
+    assert s["undo_passes"] == 1, s
+    E, coll = o.get("edges"), o.get("collapsed")
+    off, fan = o.get("fan_off"), o.get("fan")
+    occ = occ.ravel()
+    scanned = coll[:s["undo_cut"]]
+    kept = [e for e in scanned if occ[E[e][1]] == 0]
+    undone = [e for e in scanned if occ[E[e][1]] == 1]
+    assert len(undone) > 0 and 2 * len(undone) == s["n_vertices_undo"]
+    for e in undone:
+        assert np.array_equal(P[E[e][1]], P0[E[e][1]])
+    _, hits = oracle.full_check(P, T)
+    flagged = np.zeros(T.shape[0], bool)
+    flagged[hits.ravel()] = True
+    for e in kept:
+        a, b = E[e]
+        fs = np.concatenate([fan[off[a]:off[a + 1]], fan[off[b]:off[b + 1]]])
+        assert not flagged[fs].any(), e
 
 
 # ------------------------------------------------------------- verifier
