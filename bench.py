@@ -38,7 +38,11 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", default="sdf2m", choices=["sdf2m", "ico20k"])
+    p.add_argument("--workload", default="sdf2m", choices=["sdf2m", "ico20k", "batch512", "serve"])
+    p.add_argument("--meshes", type=int, default=512, help="batch512: number of meshes")
+    p.add_argument("--jobs", type=int, default=1000, help="serve: number of jobs")
+    p.add_argument("--rate", type=float, default=200.0, help="serve: Poisson arrival rate (jobs/s)")
+    p.add_argument("--seq-sample", type=int, default=8, help="batch/serve: plain-path timing on every n-th mesh")
     p.add_argument("--grid", type=int, default=300, help="lattice for sdf2m (300 gives ~2.0M faces)")
     p.add_argument("--cpu-seconds", type=float, default=20.0)
     p.add_argument("--no-cpu", action="store_true")
@@ -67,6 +71,14 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path (each GPU would simplify its own "
               "meshes independently); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch512":
+        return run_batch(args)
+    if args.workload == "serve":
+        return run_serve(args)
+    return run_plain(args)
+
+
+def run_plain(args):
     import numpy as np
     import torch
 
@@ -151,6 +163,196 @@ def main():
         r = oracle.verify(out_v, out_f)
         result["verify"] = {k: r[k] for k in ("manifold", "watertight", "degenerate_faces", "intersecting_pairs",
                                                "ok")}
+    print(json.dumps(result), flush=True)
+
+
+# ------------------------------------------------------------ many meshes
+def _gen(specs):
+    from pamo_amd import synth
+    t = time.time()
+    workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    meshes = synth.batch_meshes(specs, workers)
+    log("generated %d meshes, %.2fM faces (%.1fs, %d workers)" %
+        (len(meshes), sum(m[1].shape[0] for m in meshes) / 1e6, time.time() - t, workers))
+    return meshes
+
+
+def _edge_cost_roofline(ctx):
+    from pamo_amd import _lib
+    prof = ctx.profile_read(_lib.K_EDGE_COST)
+    avg_ms = prof["ms"] / max(prof["launches"], 1)
+    avg_bytes = prof["bytes"] / max(prof["launches"], 1)
+    achieved = (avg_bytes / (avg_ms / 1e3)) / 1e9 if avg_ms > 0 else 0.0
+    return dict(kernel=_lib.KERNEL_NAMES[_lib.K_EDGE_COST], bound="hbm", achieved=round(achieved, 2),
+                peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                launches=prof["launches"], avg_launch_us=round(1e3 * avg_ms, 3), avg_algorithmic_bytes=round(avg_bytes))
+
+
+def _plain_times(meshes, idx, ctx):
+    """Plain path (pamo_run_stage2, one mesh at a time) on meshes[idx]:
+    per-mesh seconds and outputs."""
+    import torch
+    from pamo_amd import batch, simplify_stage2
+    dev = torch.device("cuda:0")
+    out = {}
+    for i in idx:
+        v, f, r = meshes[i]
+        dv, dt, sc, m = batch.prepare(v, f, dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ov, of, st = simplify_stage2(dv, dt, sc, int(r * len(f)), ctx=ctx)
+        torch.cuda.synchronize()
+        out[i] = (time.perf_counter() - t0, ov, of)
+    return out
+
+
+def _cpu_sample(meshes, idx, seconds):
+    """The oracle on whole meshes of idx in order until the budget is used."""
+    import oracle
+    from pamo_amd import preprocess_numpy
+    ncores = oracle.set_threads(int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0)))
+    faces = 0
+    spent = 0.0
+    n = 0
+    for i in idx:
+        v, f, r = meshes[i]
+        verts, scale, _ = preprocess_numpy(v, f)
+        t0 = time.perf_counter()
+        oracle.run(verts, f, scale, int(r * len(f)))
+        spent += time.perf_counter() - t0
+        faces += len(f)
+        n += 1
+        if spent >= seconds:
+            break
+    return dict(value=round(faces / spent / 1e6, 4) if spent else None, unit="Mfaces/s", cores=ncores, kind="port",
+                sample="oracle ran %d whole meshes (%d faces) of the plain-path sample in %.1fs" % (n, faces, spent))
+
+
+def run_batch(args):
+    import numpy as np
+    import torch
+    from pamo_amd import _lib, batch, synth
+    specs = synth.batch_specs(args.meshes)
+    meshes = _gen(specs)
+    F_total = sum(len(m[1]) for m in meshes)
+    dev = torch.device("cuda:0")
+    prepared = [batch.prepare(v, f, dev) for v, f, r in meshes]
+    torch.cuda.synchronize()
+    ctx = _lib.Context(0)
+    # warm-up: the plain path on a few small meshes loads every kernel
+    sample = list(range(0, len(meshes), max(1, args.seq_sample)))
+    _plain_times(meshes, sample[:2], ctx)
+    steps = args.steps
+    times = []
+    outs = {}
+    for s in range(steps):
+        eng = batch.BatchEngine(ctx=ctx) if s == 0 else eng
+        if s == steps - 1:
+            ctx.profile([_lib.K_EDGE_COST])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        handles = {}
+        for i, ((dv, dt, sc, m), (v, f, r)) in enumerate(zip(prepared, meshes)):
+            handles[eng.submit(dv, dt, sc, int(r * len(f)), job=i)] = i
+        pending = len(meshes)
+        rounds = 0
+        while pending:
+            _, active = eng.step(1)
+            rounds += 1
+            for h, _ in eng.poll():
+                gv, gf, info = eng.take(h)
+                outs[handles[h]] = (gv, gf, info)
+                pending -= 1
+            if active == 0 and pending:
+                raise RuntimeError("engine idle with %d meshes pending" % pending)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        log("batch step %d: %.1f ms, %d engine rounds" % (s, 1e3 * times[-1], rounds))
+    roofline = _edge_cost_roofline(ctx)
+    ctx.profile([])
+    sec = float(np.mean(times))
+    # plain path, one mesh at a time, on every seq_sample-th mesh
+    seq = _plain_times(meshes, sample, ctx)
+    f_sample = sum(len(meshes[i][1]) for i in sample)
+    t_sample = sum(seq[i][0] for i in sample)
+    seq_rate = f_sample / t_sample / 1e6
+    same = all(torch.equal(outs[i][0], seq[i][1]) and torch.equal(outs[i][1], seq[i][2]) for i in sample)
+    result = dict(metric="input Mfaces/s, %d meshes batched (cfg4 mix)" % len(meshes), value=round(F_total / sec / 1e6, 3),
+                  unit="Mfaces/s", n_gpus=1, steps=steps, warmup=1, ms_per_step=round(1e3 * sec, 3),
+                  higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32",
+                  data="synthetic (seeded generators, in-process)",
+                  config=dict(workload="batch512", meshes=len(meshes), faces=F_total,
+                              face_range=[int(min(len(m[1]) for m in meshes)), int(max(len(m[1]) for m in meshes))],
+                              families="displaced icosphere / SDF union / noisy torus in thirds",
+                              ratios=[0.01, 0.005, 0.001], seeds="4000000+i", tolerance=4, threshold=1e-3,
+                              parallelism="single GPU, one packed engine"),
+                  sequential_plain=dict(value=round(seq_rate, 3), unit="Mfaces/s", meshes=len(sample),
+                                        faces=f_sample, seconds=round(t_sample, 3)),
+                  batch_speedup=round((F_total / sec / 1e6) / seq_rate, 2), batch_equals_plain=bool(same),
+                  roofline=roofline)
+    if not args.no_cpu:
+        result["cpu_baseline"] = _cpu_sample(meshes, sample, args.cpu_seconds)
+    if not args.no_verify:
+        import oracle
+        bad = 0
+        for i in sample[:16]:
+            r = oracle.verify(outs[i][0].cpu().numpy(), outs[i][1].cpu().numpy())
+            bad += 0 if (r["manifold"] and r["watertight"] and r["intersecting_pairs"] == 0) else 1
+        result["verify"] = dict(meshes_checked=min(16, len(sample)), failing=bad)
+    print(json.dumps(result), flush=True)
+
+
+def run_serve(args):
+    import numpy as np
+    import torch
+    from pamo_amd import _lib, batch, synth
+    specs = synth.batch_specs(args.jobs, seed_base=5_000_000)
+    meshes = _gen(specs)
+    rng = np.random.default_rng(5)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, size=len(meshes)))
+    jobs = [dict(points=v, faces=f, ratio=r, arrival=float(a)) for (v, f, r), a in zip(meshes, arrivals)]
+    ctx = _lib.Context(0)
+    sample = list(range(0, len(meshes), max(1, args.seq_sample)))
+    _plain_times(meshes, sample[:2], ctx)   # warm-up
+    eng = batch.BatchEngine(ctx=ctx)
+    t0 = time.perf_counter()
+    recs = batch.Server(engine=eng).run(jobs)
+    wall = time.perf_counter() - t0
+    lat = np.array([r["latency"] for r in recs])
+    F_total = sum(len(m[1]) for m in meshes)
+    span = max(r["finished"] for r in recs) - float(arrivals[0])
+    # plain path: measured service time of every seq_sample-th job, a linear
+    # fit in input faces for the rest, and a FIFO single-server queue over the
+    # same arrivals
+    seq = _plain_times(meshes, sample, ctx)
+    xs = np.array([len(meshes[i][1]) for i in sample], float)
+    ys = np.array([seq[i][0] for i in sample])
+    b, a = np.polyfit(xs, ys, 1)
+    svc = np.array([max(a + b * len(m[1]), ys.min()) for m in meshes])
+    for i in sample:
+        svc[i] = seq[i][0]
+    done = 0.0
+    seq_lat = []
+    for t_arr, s_i in zip(arrivals, svc):
+        start = max(done, t_arr)
+        done = start + s_i
+        seq_lat.append(done - t_arr)
+    seq_lat = np.array(seq_lat)
+    result = dict(metric="served input Mfaces/s at %g jobs/s (Poisson)" % args.rate,
+                  value=round(F_total / span / 1e6, 3), unit="Mfaces/s", n_gpus=1, steps=1, warmup=1,
+                  ms_per_step=round(1e3 * wall, 3), higher_is_better=True, scaling="weak", vs_baseline=None,
+                  dtype="f32", data="synthetic (seeded generators, in-process)",
+                  config=dict(workload="serve", jobs=len(jobs), faces=F_total, rate_jobs_s=args.rate,
+                              arrival_seed=5, seeds="5000000+i", hbm_budget_gb=200,
+                              parallelism="single GPU, continuous batching"),
+                  latency_ms=dict(p50=round(1e3 * float(np.percentile(lat, 50)), 2),
+                                  p99=round(1e3 * float(np.percentile(lat, 99)), 2)),
+                  sequential_plain=dict(latency_ms=dict(p50=round(1e3 * float(np.percentile(seq_lat, 50)), 2),
+                                                        p99=round(1e3 * float(np.percentile(seq_lat, 99)), 2)),
+                                        measured_jobs=len(sample),
+                                        model="measured service times on the sample, linear fit in faces "
+                                              "for the rest, FIFO queue over the same arrivals"),
+                  jobs_per_s=round(len(jobs) / span, 2))
     print(json.dumps(result), flush=True)
 
 

@@ -87,6 +87,65 @@ __global__ void k_init_mesh_state(int M, const int* __restrict__ nust, int* __re
   lcnt[k] = 0;
 }
 
+// Round start in one launch: counters (with the live V and F), round
+// statistics, per-mesh undo state, the three per-vertex counters of the fan
+// and edge builds, the occupancy flags (1 per vertex, 0 at V) and the
+// per-vertex claim minima.
+__global__ void k_round_init(long long V, long long F, int M, int* __restrict__ counters, int n_counters,
+                             int c_v, int c_f, unsigned* __restrict__ cstats, int n_cstats,
+                             const int* __restrict__ nust, int* __restrict__ first, int* __restrict__ nvu,
+                             int* __restrict__ lcnt, int* __restrict__ cnt, int* __restrict__ fcur,
+                             int* __restrict__ ucnt, int* __restrict__ occ, u64* __restrict__ vmin) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_counters) counters[i] = i == c_v ? (int)V : (i == c_f ? (int)F : 0);
+  if (i < n_cstats) cstats[i] = 0;
+  if (i < M) {
+    first[i] = nust[i];
+    nvu[i] = nust[i];
+    lcnt[i] = 0;
+  }
+  if (i <= V) {
+    cnt[i] = 0;
+    fcur[i] = 0;
+    ucnt[i] = 0;
+    occ[i] = i < V ? 1 : 0;
+    if (i < V) vmin[i] = ~0ull;
+  }
+}
+// fan counts, plus the round-start face snapshot (the undo restore source)
+__global__ void k_fan_count_snap(const int* __restrict__ T, int F, int* cnt, int* __restrict__ T0) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  I3 t = ldt(T, f);
+  T0[3LL * f] = t.i;
+  T0[3LL * f + 1] = t.j;
+  T0[3LL * f + 2] = t.k;
+  if (dead3(t)) return;
+  atomicAdd(&cnt[t.i], 1); atomicAdd(&cnt[t.j], 1); atomicAdd(&cnt[t.k], 1);
+}
+// Per-check resets in one launch: query and hit flags of every face, the
+// per-mesh leaf ranges, the root's parent, the mark counter and the pair
+// buffer counters.
+__global__ void k_check_init(long long F, int M, uint8_t* __restrict__ qflag, uint8_t* __restrict__ S,
+                             int* __restrict__ seg_a, int* __restrict__ seg_b, int* __restrict__ kparent,
+                             int* __restrict__ nmark, int* __restrict__ novf, unsigned long long* __restrict__ npairs) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < F) {
+    qflag[i] = 0;
+    S[i] = 0;
+  }
+  if (i < M) {
+    seg_a[i] = -1;
+    seg_b[i] = -1;
+  }
+  if (i == 0) {
+    kparent[0] = -1;
+    *nmark = 0;
+    *novf = 0;
+    *npairs = 0;
+  }
+}
+
 // admission: faces with local indices -> global, with or without the shuffle
 __global__ void k_copy_faces_off(const int* __restrict__ tin, int* __restrict__ T, int F, int voff) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -113,16 +172,18 @@ __global__ void k_shuffle_scatter_off(Feistel fb, long long n, long long m, cons
 //   splane unit normal and offset (tri2plane, cusimp_free.cu:337-346; also
 //          the old normal of the flip test :520-521)
 //   slen   |Pi-Pj|, |Pj-Pk|, |Pk-Pi| (:451-483)
-// and the vertex quadric from the same planes in fan order (:348-366).
+// and the vertex quadric from the same planes in fan order (:348-366);
+// it also takes the round-start snapshot of the vertex position.
 // A face's values are computed by each of its three vertices with the same
 // expressions, so all copies are identical.
 __global__ void __launch_bounds__(BLOCK) k_fan_slots(const float* __restrict__ P, const int* __restrict__ T,
                                                      const int* __restrict__ fan_off, int* __restrict__ fan, int V,
                                                      int* __restrict__ fsucc, int4* __restrict__ sface,
                                                      float4* __restrict__ splane, float4* __restrict__ slen,
-                                                     float* __restrict__ Q) {
+                                                     float* __restrict__ Q, float* __restrict__ P0) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= V) return;
+  stp(P0, v, ldp(P, v));   // round-start position snapshot (the undo restore source)
   int a = fan_off[v], b = fan_off[v + 1];
   for (int i = a + 1; i < b; i++) {
     int x = fan[i];
@@ -570,6 +631,83 @@ __global__ void k_query_list(const int* __restrict__ sorted_leaf, const int* __r
   qlist[qpos[l]] = leaf_face[sorted_leaf[l]];
 }
 
+// Stable LSD radix sort of up to SS_MAX (u64 key, int value) pairs in one
+// workgroup and one launch, for small alive-face counts where the
+// multi-launch sort is all launch latency.  Keys and values stay in LDS
+// across passes.  Each pass sorts 4 bits: every thread owns SS_ITEMS
+// consecutive items and one private counter per digit (packed two to a
+// u32: digit r in the low half, digit r+8 in the high half of row r), a
+// block scan over the counters in (row, thread) order gives every item its
+// stable position, and the items are written back in place.
+constexpr int SS_NT = BLOCK1K, SS_ITEMS = 8, SS_MAX = SS_NT * SS_ITEMS;
+__global__ void __launch_bounds__(SS_NT) k_sort_small(const u64* __restrict__ kin, const int* __restrict__ vin,
+                                                      const int* __restrict__ n_ptr, int kbits,
+                                                      u64* __restrict__ kout, int* __restrict__ vout, int* err) {
+  __shared__ u64 sk[SS_MAX];
+  __shared__ int sv[SS_MAX];
+  __shared__ unsigned cnt[8 * SS_NT];
+  __shared__ unsigned wl[SS_NT / 64];
+  const int n = *n_ptr;
+  const int t = threadIdx.x;
+  if (n > SS_MAX) {   // the host launches this only under a bound; never expected
+    if (t == 0) atomicAdd(err, 1);
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < SS_ITEMS; i++) {
+    int idx = t * SS_ITEMS + i;
+    sk[idx] = idx < n ? kin[idx] : ~0ull;   // padding sorts last and stays there
+    sv[idx] = idx < n ? vin[idx] : -1;
+  }
+  __syncthreads();
+  for (int shift = 0; shift < kbits; shift += 4) {
+    u64 k[SS_ITEMS];
+    int v[SS_ITEMS];
+    unsigned lr[SS_ITEMS];
+#pragma unroll
+    for (int r = 0; r < 8; r++) cnt[r * SS_NT + t] = 0;
+#pragma unroll
+    for (int i = 0; i < SS_ITEMS; i++) {
+      k[i] = sk[t * SS_ITEMS + i];
+      v[i] = sv[t * SS_ITEMS + i];
+      int d = (int)((k[i] >> shift) & 15ull);
+      int r = d & 7, h = d >> 3;
+      unsigned c = cnt[r * SS_NT + t];
+      lr[i] = (c >> (16 * h)) & 0xFFFFu;
+      cnt[r * SS_NT + t] = c + (1u << (16 * h));
+    }
+    __syncthreads();
+    // exclusive scan of the 8*SS_NT packed counters in (row, thread) order;
+    // thread t owns entries [8t, 8t+8)
+    unsigned x[8], pre[8], sum = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) { x[j] = cnt[8 * t + j]; pre[j] = sum; sum += x[j]; }
+    unsigned tot;
+    unsigned off = block_exclusive_scan_1024(sum, wl, tot);
+#pragma unroll
+    for (int j = 0; j < 8; j++) cnt[8 * t + j] = off + pre[j];
+    __syncthreads();
+    const unsigned low_total = tot & 0xFFFFu;   // items with digits 0..7
+    int pos[SS_ITEMS];
+#pragma unroll
+    for (int i = 0; i < SS_ITEMS; i++) {
+      int d = (int)((k[i] >> shift) & 15ull);
+      int r = d & 7, h = d >> 3;
+      unsigned base = (cnt[r * SS_NT + t] >> (16 * h)) & 0xFFFFu;
+      pos[i] = (int)(base + lr[i] + (h ? low_total : 0u));
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < SS_ITEMS; i++) { sk[pos[i]] = k[i]; sv[pos[i]] = v[i]; }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < SS_ITEMS; i++) {
+    int idx = t * SS_ITEMS + i;
+    if (idx < n) { kout[idx] = sk[idx]; vout[idx] = sv[idx]; }
+  }
+}
+
 // stable LSD radix sort of (u64 key, int value), 8-bit digits (as
 // k_rs_hist/k_rs_scatter, 64-bit keys)
 __global__ void __launch_bounds__(BLOCK) k_rs_hist64(const u64* __restrict__ keys, const int* __restrict__ n_ptr,
@@ -794,10 +932,10 @@ __global__ void __launch_bounds__(BLOCK) k_traverse_b(const int* __restrict__ ql
 // wave runs the long FP64 pair test in every loop step in which any of its
 // lanes sits on an overlapping leaf, so its time is (steps of the longest
 // walk) x (pair test) whatever the number of queries.  Here the walk only
-// collects, per query, the overlapping faces that do not share an edge with
-// it (pair_hits rejects those outright), and the pair kernel tests one pair
-// per lane.  The verdict per query is the OR over its pairs, the same as the
-// walk that stops at the first hit.
+// collects, per query, the overlapping faces (a chain of node loads and
+// nothing else; pair_hits rejects the pairs that share an edge), and the
+// pair kernel tests one pair per lane.  The verdict per query is the OR over
+// its pairs, the same as the walk that stops at the first hit.
 //
 // A query may be split over 2^kshift lanes, each walking one subtree at
 // depth kshift of the mesh's tree: the subtrees partition the leaves, so the
@@ -812,6 +950,7 @@ __global__ void __launch_bounds__(BLOCK) k_traverse_b(const int* __restrict__ ql
 // pairs do not fit the pair buffer goes to the overflow list, which
 // k_traverse_b walks in full.
 constexpr int CAND_CAP = 32;
+constexpr int N_XCD = 8;   // MI355X: 8 accelerator complex dies, 32 CUs and one L2 each
 __global__ void __launch_bounds__(BLOCK) k_traverse_emit(
     const int* __restrict__ qlist, const int* __restrict__ nq_ptr, const int* __restrict__ n_ptr,
     const int* __restrict__ fmesh, const int* __restrict__ root, const int* __restrict__ root_end,
@@ -822,6 +961,23 @@ __global__ void __launch_bounds__(BLOCK) k_traverse_emit(
   const int t = threadIdx.x;
   const int lane = t & 63;
   const long long gid = (long long)blockIdx.x * blockDim.x + t;
+  {
+    // Warm each XCD's L2 with the node arrays before the walks (a kernel
+    // starts with a cold L2, and a walk is a chain of dependent node loads).
+    // Workgroups go to the XCDs round-robin; the ones on one XCD split the
+    // nodes between them.  Only for trees that fit an L2 comfortably.
+    const long long nn = 2LL * (*n_ptr) - 1;
+    if (nn > 0 && nn * 32 <= (2LL << 20)) {
+      const long long per = (gridDim.x + N_XCD - 1) / N_XCD;           // workgroups per XCD
+      const long long j = blockIdx.x / N_XCD;                            // index within the XCD
+      const long long chunk = (nn + per - 1) / per;
+      const long long a = j * chunk, b = min(nn, a + chunk);
+      float acc = 0.0f;
+      // one load per 128-byte line (8 nodes of 16 bytes)
+      for (long long i = a + 8LL * t; i < b; i += 8LL * BLOCK) acc += nodeA[i].x + nodeB[i].x;
+      if (acc == 1.0e-37f && stats) atomicAdd(&stats[3], 1ull);   // keeps the loads; practically never true
+    }
+  }
   const long long qi = gid >> kshift;
   const int part = (int)(gid & ((1 << kshift) - 1));
   int nq = *nq_ptr;
@@ -868,17 +1024,15 @@ __global__ void __launch_bounds__(BLOCK) k_traverse_emit(
         int g = ~c;
         if (g != f) {
           cand++;
-          if (shared_indices(ldt(T, g), tf) != 2) {
-            if (cnt == CAND_CAP) {   // LDS list full: move it to the pair buffer
-              long long b = (long long)atomicAdd(npairs, (unsigned long long)CAND_CAP);
-              for (int j = 0; j < CAND_CAP; j++)
-                if (b + j < pair_cap) pairs[b + j] = make_int2(f, cl[j * BLOCK + t]);
-              cnt = 0;
-              if (b + CAND_CAP > pair_cap) { full = true; break; }
-            }
-            cl[cnt * BLOCK + t] = g;
-            cnt++;
+          if (cnt == CAND_CAP) {   // LDS list full: move it to the pair buffer
+            long long b = (long long)atomicAdd(npairs, (unsigned long long)CAND_CAP);
+            for (int j = 0; j < CAND_CAP; j++)
+              if (b + j < pair_cap) pairs[b + j] = make_int2(f, cl[j * BLOCK + t]);
+            cnt = 0;
+            if (b + CAND_CAP > pair_cap) { full = true; break; }
           }
+          cl[cnt * BLOCK + t] = g;
+          cnt++;
         }
       }
       node = esc;

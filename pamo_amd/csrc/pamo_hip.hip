@@ -70,7 +70,7 @@ struct pamo_ctx {
   Buf qflag, qf, qpos, af, apos, qlist, leaf_face, lbox_lo, lbox_hi, keys0, keys1, vals0, vals1, hist;
   Buf pairs, ovf, seg_lo, seg_hi;
   Buf kleft, kright, kparent, rfirst, rlast, nodeA, nodeB, S, S_first, mflag, mpos, marked;
-  Buf scan_sums, scan_desc, counters, cstats, shuf_flags, shuf_pos;
+  Buf scan_desc, counters, cstats, shuf_flags, shuf_pos;
   unsigned scan_epoch = 0;   // epoch 0 never names a call, so zeroed words read as unpublished
   // run loop
   Buf RV[2], RF[2], keep, kpos, undo_in;
@@ -200,7 +200,7 @@ struct pamo_ctx {
     return scan_launch(in, out, n, d_n, add, nullptr);
   }
   bool scan_launch(const int* in, int* out, long long n, const int* d_n, int add, int* total) {
-    long long nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    long long nb = (n + SCAN1_TILE - 1) / SCAN1_TILE;
     if (scan_desc.bytes < (size_t)nb * 8) {
       if (!ensure(scan_desc, (size_t)nb * 8)) return false;
       if (!hip(hipMemsetAsync(scan_desc.p, 0, scan_desc.bytes, st), "scan desc")) return false;
@@ -209,7 +209,7 @@ struct pamo_ctx {
       scan_epoch = 1;
       if (!hip(hipMemsetAsync(scan_desc.p, 0, scan_desc.bytes, st), "scan desc")) return false;
     }
-    k_scan_onepass<<<(unsigned)nb, BLOCK, 0, st>>>(in, out, n, d_n, add, ptr<unsigned long long>(scan_desc),
+    k_scan_onepass<<<(unsigned)nb, BLOCK1K, 0, st>>>(in, out, n, d_n, add, ptr<unsigned long long>(scan_desc),
                                                    scan_epoch, total, cnt_dev(C_ERR));
     return check_launch("scan");
   }
@@ -256,7 +256,7 @@ pamo_ctx::~pamo_ctx() {
                 &ecnt, &eoff, &edges, &Q, &tri_min, &win, &winpos, &collapsed, &qflag, &qf, &qpos, &af, &apos,
                 &qlist, &pairs, &ovf, &seg_lo, &seg_hi, &leaf_face, &lbox_lo, &lbox_hi, &keys0, &keys1, &vals0, &vals1, &hist, &kleft, &kright,
                 &kparent, &rfirst, &rlast, &nodeA, &nodeB, &S, &S_first, &mflag, &mpos, &marked,
-                &scan_sums, &scan_desc, &counters, &cstats, &shuf_flags, &shuf_pos, &RV[0], &RV[1], &RF[0], &RF[1], &keep,
+                &scan_desc, &counters, &cstats, &shuf_flags, &shuf_pos, &RV[0], &RV[1], &RF[0], &RF[1], &keep,
                 &kpos, &undo_in};
   for (Buf* b : all)
     if (b->p) (void)hipFree(b->p);
@@ -570,7 +570,7 @@ static void eng_free(pamo_ctx* c) {
                 &g.undo_out, &g.tmp, &g.marked, &g.pm, &g.table, &g.ecost, &g.mp, &g.voff, &g.foff, &g.eoffm,
                 &g.uioff, &g.uioff2, &g.nustk, &g.h, &g.lcnt, &g.ncoll, &g.cut, &g.cutoff, &g.uobase, &g.hm, &g.part,
                 &g.first, &g.nvu, &g.nvoff, &g.nfoff, &g.seg_a, &g.seg_b, &g.root, &g.root_end, &g.scene,
-                &g.flat_cost, &g.g, &g.keep, &g.kpos, &g.newk, &g.vkeep, &g.vpos, &g.fsucc, &g.sface, &g.splane, &g.slen, &g.erank, &g.vedge, &g.uoff, &g.vmin};
+                &g.flat_cost, &g.g, &g.keep, &g.kpos, &g.newk, &g.vkeep, &g.vpos, &g.fsucc, &g.sface, &g.splane, &g.slen, &g.erank, &g.vedge, &g.uoff, &g.vmin, &g.fcur, &g.ucnt};
   for (Buf* b : all)
     if (b->p) (void)hipFree(b->p);
   for (auto& s : g.slots) {

@@ -87,111 +87,33 @@ __device__ __forceinline__ int block_exclusive_scan(int x, int* lds, int& total)
   return off + v - x;
 }
 
-constexpr int SCAN_ITEMS = 16;
-constexpr int SCAN_TILE = BLOCK * SCAN_ITEMS;
-
-// pass 1: per-tile sums
-__global__ void __launch_bounds__(BLOCK) k_scan_reduce(const int* __restrict__ in, long long n, int* __restrict__ sums) {
-  __shared__ int lds[BLOCK / 64];
-  long long base = (long long)blockIdx.x * SCAN_TILE;
-  int s = 0;
+// the same for a block of BLOCK1K threads (unsigned values)
+constexpr int BLOCK1K = 1024;
+__device__ __forceinline__ unsigned block_exclusive_scan_1024(unsigned x, unsigned* lds, unsigned& total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  unsigned v = x;
 #pragma unroll
-  for (int i = 0; i < SCAN_ITEMS; i++) {
-    long long idx = base + (long long)i * BLOCK + threadIdx.x;
-    if (idx < n) s += in[idx];
+  for (int o = 1; o < 64; o <<= 1) {
+    unsigned y = __shfl_up(v, o, 64);
+    if (lane >= o) v += y;
   }
-  int tot;
-  block_exclusive_scan(s, lds, tot);
-  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
-}
-// pass 2: exclusive scan of the tile sums by one block; total -> *total
-__global__ void __launch_bounds__(BLOCK) k_scan_top(int* sums, int nb, int* total) {
-  __shared__ int lds[BLOCK / 64];
-  int carry = 0;
-  for (int base = 0; base < nb; base += BLOCK) {
-    int i = base + threadIdx.x;
-    int x = i < nb ? sums[i] : 0;
-    int tot;
-    int e = block_exclusive_scan(x, lds, tot);
-    if (i < nb) sums[i] = carry + e;
-    carry += tot;
-  }
-  if (threadIdx.x == 0 && total) *total = carry;
-}
-// pass 3: scan within each tile (blocked: thread t owns items t*ITEMS..)
-__global__ void __launch_bounds__(BLOCK) k_scan_down(const int* __restrict__ in, int* __restrict__ out, long long n,
-                                                    const int* __restrict__ sums) {
-  __shared__ int lds[BLOCK / 64];
-  __shared__ int tile[SCAN_TILE];
-  long long base = (long long)blockIdx.x * SCAN_TILE;
-  for (int i = 0; i < SCAN_ITEMS; i++) {
-    long long idx = base + (long long)i * BLOCK + threadIdx.x;
-    tile[i * BLOCK + threadIdx.x] = idx < n ? in[idx] : 0;
+  if (lane == 63) lds[wid] = v;
+  __syncthreads();
+  unsigned off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < BLOCK1K / 64; w++) {
+    unsigned s = lds[w];
+    if (w < wid) off += s;
+    tot += s;
   }
   __syncthreads();
-  int loc[SCAN_ITEMS];
-  int s = 0;
-#pragma unroll
-  for (int i = 0; i < SCAN_ITEMS; i++) { loc[i] = s; s += tile[threadIdx.x * SCAN_ITEMS + i]; }
-  int tot;
-  int off = block_exclusive_scan(s, lds, tot) + sums[blockIdx.x];
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < SCAN_ITEMS; i++) tile[threadIdx.x * SCAN_ITEMS + i] = off + loc[i];
-  __syncthreads();
-  for (int i = 0; i < SCAN_ITEMS; i++) {
-    long long idx = base + (long long)i * BLOCK + threadIdx.x;
-    if (idx < n) out[idx] = tile[i * BLOCK + threadIdx.x];
-  }
+  total = tot;
+  return off + v - x;
 }
 
-// device-count variants: live length min(n, *d_n + add)
-__global__ void __launch_bounds__(BLOCK) k_scan_reduce_dev(const int* __restrict__ in, long long n,
-                                                           const int* __restrict__ d_n, int add, int* __restrict__ sums) {
-  __shared__ int lds[BLOCK / 64];
-  long long live = d_n ? min(n, (long long)*d_n + add) : n;
-  long long base = (long long)blockIdx.x * SCAN_TILE;
-  int s = 0;
-#pragma unroll
-  for (int i = 0; i < SCAN_ITEMS; i++) {
-    long long idx = base + (long long)i * BLOCK + threadIdx.x;
-    if (idx < live) s += in[idx];
-  }
-  int tot;
-  block_exclusive_scan(s, lds, tot);
-  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
-}
-__global__ void __launch_bounds__(BLOCK) k_scan_down_dev(const int* __restrict__ in, int* __restrict__ out, long long n,
-                                                         const int* __restrict__ d_n, int add,
-                                                         const int* __restrict__ sums) {
-  __shared__ int lds[BLOCK / 64];
-  __shared__ int tile[SCAN_TILE];
-  long long live = d_n ? min(n, (long long)*d_n + add) : n;
-  long long base = (long long)blockIdx.x * SCAN_TILE;
-  if (base >= live) return;
-  for (int i = 0; i < SCAN_ITEMS; i++) {
-    long long idx = base + (long long)i * BLOCK + threadIdx.x;
-    tile[i * BLOCK + threadIdx.x] = idx < live ? in[idx] : 0;
-  }
-  __syncthreads();
-  int loc[SCAN_ITEMS];
-  int s = 0;
-#pragma unroll
-  for (int i = 0; i < SCAN_ITEMS; i++) { loc[i] = s; s += tile[threadIdx.x * SCAN_ITEMS + i]; }
-  int tot;
-  int off = block_exclusive_scan(s, lds, tot) + sums[blockIdx.x];
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < SCAN_ITEMS; i++) tile[threadIdx.x * SCAN_ITEMS + i] = off + loc[i];
-  __syncthreads();
-  for (int i = 0; i < SCAN_ITEMS; i++) {
-    long long idx = base + (long long)i * BLOCK + threadIdx.x;
-    if (idx < live) out[idx] = tile[i * BLOCK + threadIdx.x];
-  }
-}
 
 // ------------------------------------------------ one-pass exclusive scan
-// Decoupled look-back: tile t (one block, SCAN_TILE ints) scans itself,
+// Decoupled look-back: tile t (one block, SCAN1_TILE ints) scans itself,
 // publishes its aggregate, adds the aggregates/prefixes of earlier tiles
 // and publishes its inclusive prefix.  A tile's status and value travel in
 // one 64-bit word written and read with relaxed device-scope atomics, so no
@@ -203,68 +125,72 @@ __global__ void __launch_bounds__(BLOCK) k_scan_down_dev(const int* __restrict__
 // done; the spin is capped anyway and reports through err.  Inputs are
 // non-negative (counts and flags); totals stay below 2^40.
 constexpr unsigned long long SCAN_ST_AGG = 1ull, SCAN_ST_PREFIX = 2ull;
+constexpr int SCAN1_ITEMS = 8, SCAN1_TILE = BLOCK1K * SCAN1_ITEMS;   // 8192 ints per tile
 __device__ __forceinline__ unsigned long long scan_word(unsigned epoch, unsigned long long st, long long v) {
   return ((unsigned long long)epoch << 42) | (st << 40) | ((unsigned long long)v & ((1ull << 40) - 1));
 }
-__global__ void __launch_bounds__(BLOCK) k_scan_onepass(const int* __restrict__ in, int* __restrict__ out, long long n,
-                                                        const int* __restrict__ d_n, int add,
-                                                        unsigned long long* desc, unsigned epoch,
-                                                        int* __restrict__ total, int* err) {
-  __shared__ int lds[BLOCK / 64];
-  __shared__ int tile[SCAN_TILE];
+__global__ void __launch_bounds__(BLOCK1K) k_scan_onepass(const int* __restrict__ in, int* __restrict__ out,
+                                                          long long n, const int* __restrict__ d_n, int add,
+                                                          unsigned long long* desc, unsigned epoch,
+                                                          int* __restrict__ total, int* err) {
+  __shared__ unsigned lds[BLOCK1K / 64];
+  __shared__ int tile[SCAN1_TILE];
   __shared__ long long excl_s;
   const long long live = d_n ? min(n, (long long)*d_n + add) : n;
   const long long t = blockIdx.x;
-  const long long base = t * SCAN_TILE;
+  const long long base = t * SCAN1_TILE;
   if (live <= 0) {
     if (t == 0 && threadIdx.x == 0 && total) *total = 0;
     return;
   }
   if (base >= live) return;   // tiles past the live length contribute nothing
-  for (int i = 0; i < SCAN_ITEMS; i++) {
-    long long idx = base + (long long)i * BLOCK + threadIdx.x;
-    tile[i * BLOCK + threadIdx.x] = idx < live ? in[idx] : 0;
+  for (int i = 0; i < SCAN1_ITEMS; i++) {
+    long long idx = base + (long long)i * BLOCK1K + threadIdx.x;
+    tile[i * BLOCK1K + threadIdx.x] = idx < live ? in[idx] : 0;
   }
   __syncthreads();
-  int loc[SCAN_ITEMS];
+  int loc[SCAN1_ITEMS];
   int s = 0;
 #pragma unroll
-  for (int i = 0; i < SCAN_ITEMS; i++) { loc[i] = s; s += tile[threadIdx.x * SCAN_ITEMS + i]; }
-  int tot;
-  int off = block_exclusive_scan(s, lds, tot);
+  for (int i = 0; i < SCAN1_ITEMS; i++) { loc[i] = s; s += tile[threadIdx.x * SCAN1_ITEMS + i]; }
+  unsigned utot;
+  int off = (int)block_exclusive_scan_1024((unsigned)s, lds, utot);
+  const int tot = (int)utot;
   if (threadIdx.x == 0) {
     long long ex = 0;
-    if (t == 0) {
-      __hip_atomic_store(&desc[0], scan_word(epoch, SCAN_ST_PREFIX, tot), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      __hip_atomic_store(&desc[t], scan_word(epoch, SCAN_ST_AGG, tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      long long j = t - 1;
-      long long spins = 0;
-      while (j >= 0) {
-        unsigned long long w = __hip_atomic_load(&desc[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if ((unsigned)(w >> 42) != epoch) {
-          if (++spins > (1LL << 20)) { atomicAdd(err, 1); break; }
-          continue;
+    if (gridDim.x > 1) {   // one tile needs no look-back
+      if (t == 0) {
+        __hip_atomic_store(&desc[0], scan_word(epoch, SCAN_ST_PREFIX, tot), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        __hip_atomic_store(&desc[t], scan_word(epoch, SCAN_ST_AGG, tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        long long j = t - 1;
+        long long spins = 0;
+        while (j >= 0) {
+          unsigned long long w = __hip_atomic_load(&desc[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if ((unsigned)(w >> 42) != epoch) {
+            if (++spins > (1LL << 20)) { atomicAdd(err, 1); break; }
+            continue;
+          }
+          ex += (long long)(w & ((1ull << 40) - 1));
+          if (((w >> 40) & 3ull) == SCAN_ST_PREFIX) break;
+          j--;
         }
-        ex += (long long)(w & ((1ull << 40) - 1));
-        if (((w >> 40) & 3ull) == SCAN_ST_PREFIX) break;
-        j--;
+        __hip_atomic_store(&desc[t], scan_word(epoch, SCAN_ST_PREFIX, ex + tot), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
       }
-      __hip_atomic_store(&desc[t], scan_word(epoch, SCAN_ST_PREFIX, ex + tot), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
     }
     excl_s = ex;
-    if (total && base + SCAN_TILE >= live) *total = (int)(ex + tot);
+    if (total && base + SCAN1_TILE >= live) *total = (int)(ex + tot);
   }
   __syncthreads();
   off += (int)excl_s;
 #pragma unroll
-  for (int i = 0; i < SCAN_ITEMS; i++) tile[threadIdx.x * SCAN_ITEMS + i] = off + loc[i];
+  for (int i = 0; i < SCAN1_ITEMS; i++) tile[threadIdx.x * SCAN1_ITEMS + i] = off + loc[i];
   __syncthreads();
-  for (int i = 0; i < SCAN_ITEMS; i++) {
-    long long idx = base + (long long)i * BLOCK + threadIdx.x;
-    if (idx < live) out[idx] = tile[i * BLOCK + threadIdx.x];
+  for (int i = 0; i < SCAN1_ITEMS; i++) {
+    long long idx = base + (long long)i * BLOCK1K + threadIdx.x;
+    if (idx < live) out[idx] = tile[i * BLOCK1K + threadIdx.x];
   }
 }
 

@@ -176,8 +176,9 @@ def sdf_union(n_spheres=200, grid=700, seed=2, k=0.02, n_slab_pairs=0, threads=0
     return sdf_mesh(spheres, slabs, k, grid, threads)
 
 
-def mesh_for_faces(kind, target_faces, seed):
-    """One mesh of roughly target_faces faces from a generator family."""
+def mesh_for_faces(kind, target_faces, seed, threads=0):
+    """One mesh of roughly target_faces faces from a generator family
+    (threads: OpenMP threads of the SDF mesher, 0 = all)."""
     if kind == "icosphere":
         level = int(np.clip(np.round(np.log(max(target_faces, 20) / 20.0) / np.log(4.0)), 1, 8))
         return displaced_icosphere(level, seed)
@@ -190,5 +191,42 @@ def mesh_for_faces(kind, target_faces, seed):
         ns = int(rng.integers(5, 40))
         # faces grow roughly with the square of the grid resolution
         grid = int(np.clip(np.sqrt(target_faces / (2.2 * max(ns, 1) ** 0.5)) * 1.0, 24, 700))
-        return sdf_union(ns, grid, seed)
+        return sdf_union(ns, grid, seed, threads=threads)
     raise ValueError(kind)
+
+
+def batch_specs(n=512, seed_base=4_000_000, fmin=2_000, fmax=400_000, ratios=(0.01, 0.005, 0.001)):
+    """The many-mesh workload: n meshes with face counts log-uniform in
+    [fmin, fmax], generator families mixed in thirds (displaced icosphere at
+    the matching subdivision level, SDF union, noisy torus) and ratios drawn
+    uniformly from `ratios`; mesh i is seeded seed_base + i.  Returns a list
+    of (kind, target_faces, ratio, seed)."""
+    kinds = ("icosphere", "sdf", "torus")
+    out = []
+    for i in range(n):
+        rng = np.random.default_rng(seed_base + i)
+        faces = int(np.exp(rng.uniform(np.log(fmin), np.log(fmax))))
+        ratio = float(ratios[int(rng.integers(len(ratios)))])
+        out.append((kinds[i % 3], faces, ratio, seed_base + i))
+    return out
+
+
+def _spec_mesh(spec, threads=0):
+    kind, faces, ratio, seed = spec
+    v, f = mesh_for_faces(kind, faces, seed, threads)
+    return v, f, ratio
+
+
+def _spec_mesh_1t(spec):
+    return _spec_mesh(spec, 1)   # pool workers: one thread each
+
+
+def batch_meshes(specs, workers=1):
+    """Generate the meshes of batch_specs: a list of (points, faces, ratio)
+    in spec order; `workers` > 1 generates them in a process pool."""
+    if workers <= 1 or len(specs) < 2:
+        return [_spec_mesh(s) for s in specs]
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    with cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
+        return list(ex.map(_spec_mesh_1t, specs, chunksize=4))
