@@ -48,6 +48,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-verify", action="store_true")
     p.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic.json"))
+    p.add_argument("--copy-probe", type=int, default=0,
+                   help="MiB copied device-to-device once before timing: a copy kernel of known size that "
+                        "calibrates the FETCH_SIZE / WRITE_SIZE counters in a rocprofv3 --pmc run")
     return p.parse_args()
 
 
@@ -93,6 +96,14 @@ def run_plain(args):
     df = torch.from_numpy(f).to(dev)
     torch.cuda.synchronize()
     ctx = _lib.Context(0)
+    if args.copy_probe:
+        src = torch.ones(args.copy_probe << 20, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy probe: %d bytes" % src.numel())
+        del src, dst
     for w in range(args.warmup):
         _, _, st = simplify_stage2(dv, df, scale, target, ctx=ctx)
         torch.cuda.synchronize()

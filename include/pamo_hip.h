@@ -148,6 +148,13 @@ long long pamo_debug_copy(pamo_ctx* ctx, void* stream, int which, void* host_dst
  * the single-kernel fallback; results must not change. */
 int pamo_debug_pair_cap(pamo_ctx* ctx, long long cap);
 
+/* Test hook (no counterpart in the reference): how the engine's
+ * self-intersection check finds candidate pairs.  0 (default) by mesh size:
+ * every query against every alive face of its mesh when all meshes have at
+ * most 32768 faces, else an LBVH walk; 1 always the LBVH; 2 always the
+ * exhaustive search.  Both give the same candidate set. */
+int pamo_debug_guard(pamo_ctx* ctx, int mode);
+
 /* Parity hook: every ordered candidate pair (query, current) of one
  * self-intersection check over all alive faces of a mesh, with the
  * reference narrow-phase verdict (simp_cuda/src/bvh/self_intersect.cuh:151-392).

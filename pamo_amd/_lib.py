@@ -66,6 +66,8 @@ def lib():
     L.pamo_profile_read.restype = I
     L.pamo_debug_pair_cap.argtypes = [P, LL]
     L.pamo_debug_pair_cap.restype = I
+    L.pamo_debug_guard.argtypes = [P, I]
+    L.pamo_debug_guard.restype = I
     _lib = L
     return L
 
@@ -112,6 +114,10 @@ class Context:
     def debug_pair_cap(self, cap):
         """Limit the engine's candidate-pair buffer (-1: default); test hook."""
         self.check(self._L.pamo_debug_pair_cap(self.handle, int(cap)), "pamo_debug_pair_cap")
+
+    def debug_guard(self, mode):
+        """Candidate search of the engine's check: 0 by size, 1 LBVH, 2 exhaustive; test hook."""
+        self.check(self._L.pamo_debug_guard(self.handle, int(mode)), "pamo_debug_guard")
 
     def check(self, rc, what):
         if rc < 0:

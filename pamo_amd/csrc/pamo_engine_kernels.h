@@ -125,14 +125,16 @@ __global__ void k_fan_count_snap(const int* __restrict__ T, int F, int* cnt, int
 }
 // Per-check resets in one launch: query and hit flags of every face, the
 // per-mesh leaf ranges, the root's parent, the mark counter and the pair
-// buffer counters.
+// buffer counters, and the overflow flags of the exhaustive search.
 __global__ void k_check_init(long long F, int M, uint8_t* __restrict__ qflag, uint8_t* __restrict__ S,
                              int* __restrict__ seg_a, int* __restrict__ seg_b, int* __restrict__ kparent,
-                             int* __restrict__ nmark, int* __restrict__ novf, unsigned long long* __restrict__ npairs) {
+                             int* __restrict__ nmark, int* __restrict__ novf, unsigned long long* __restrict__ npairs,
+                             int* __restrict__ oflag) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < F) {
     qflag[i] = 0;
     S[i] = 0;
+    oflag[i] = 0;
   }
   if (i < M) {
     seg_a[i] = -1;
@@ -1016,7 +1018,8 @@ __global__ void __launch_bounds__(BLOCK) k_traverse_emit(
     while (node != end && node >= 0) {
       if (++steps > cap) { atomicAdd(err, 1); break; }
       float4 A = nodeA[node], B = nodeB[node];
-      bool ov = !(qhi[0] < A.x || B.x < qlo[0]) && !(qhi[1] < A.y || B.y < qlo[1]) && !(qhi[2] < A.z || B.z < qlo[2]);
+      const bool ov = !((qhi[0] < A.x) | (B.x < qlo[0]) | (qhi[1] < A.y) | (B.y < qlo[1]) | (qhi[2] < A.z) |
+                        (B.z < qlo[2]));   // no short circuits: the comparisons are pure
       int esc = __float_as_int(A.w);
       if (ov) {
         int c = __float_as_int(B.w);
@@ -1104,6 +1107,168 @@ __global__ void __launch_bounds__(BLOCK) k_pair_test(const int2* __restrict__ pa
     for (int o = 32; o > 0; o >>= 1) h64 += __shfl_xor(h64, o, 64);
     if ((threadIdx.x & 63) == 0 && h64) atomicAdd(&stats[1], h64);
   }
+}
+
+// --------------------------------------------- exhaustive candidate search
+// For small meshes the tree costs more than it saves: a check then tests
+// every query against every alive face of its mesh.  The candidate set is
+// the same (alive faces g != f whose float AABB overlaps the query's,
+// touching included, with the boxes of k_leaf_boxes_b), so the flags are
+// the same.  Queries are listed in face order, which groups them by mesh;
+// each mesh gets ceil(queries/BLOCK) query blocks (bx), and grid row by
+// covers alive faces [by*BF_TILE, (by+1)*BF_TILE) of the block's mesh.  The
+// tile's boxes sit in LDS and every lane of a wave reads the same box at the
+// same time.  Candidates go through the same LDS lists, flushes and pair
+// buffer as the walk.
+constexpr int BF_TILE = 256;
+__global__ void k_bf_blocks(int M, const long long* __restrict__ foff, const int* __restrict__ qpos,
+                            int* __restrict__ nb) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= M) return;
+  int nq = qpos[foff[k + 1]] - qpos[foff[k]];
+  nb[k] = (nq + BLOCK - 1) / BLOCK;
+}
+__global__ void __launch_bounds__(BLOCK) k_bf_emit(
+    const int* __restrict__ qlist, const int* __restrict__ qpos, const int* __restrict__ apos,
+    const long long* __restrict__ foff, const long long* __restrict__ boff, int M,
+    const int* __restrict__ leaf_face, const float4* __restrict__ lbox_lo, const float4* __restrict__ lbox_hi,
+    const float* __restrict__ P, const int* __restrict__ T, int2* __restrict__ pairs, long long pair_cap,
+    unsigned long long* __restrict__ npairs, int* __restrict__ ovf, int* __restrict__ novf, int* __restrict__ oflag,
+    unsigned long long* __restrict__ stats) {
+  __shared__ int cl[CAND_CAP * BLOCK];
+  __shared__ float4 blo[BF_TILE], bhi[BF_TILE];
+  __shared__ int bid[BF_TILE];
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const long long bx = blockIdx.x;
+  if (bx >= boff[M]) return;                     // block-uniform exits
+  const int k = find_seg(boff, M, bx);
+  const long long a0 = apos[foff[k]], a1 = apos[foff[k + 1]];
+  const long long tile0 = a0 + (long long)blockIdx.y * BF_TILE;
+  if (tile0 >= a1) return;
+  const int tn = (int)min((long long)BF_TILE, a1 - tile0);
+  for (int j = t; j < BF_TILE; j += BLOCK) {
+    if (j < tn) {
+      blo[j] = lbox_lo[tile0 + j];
+      bhi[j] = lbox_hi[tile0 + j];
+      bid[j] = leaf_face[tile0 + j];
+    } else {
+      bid[j] = -1;   // padding: never a candidate
+    }
+  }
+  __syncthreads();
+  const long long q1 = qpos[foff[k + 1]];
+  const long long qi = qpos[foff[k]] + (bx - boff[k]) * BLOCK + t;
+  unsigned cand = 0;
+  int cnt = 0;
+  int f = -1;
+  bool full = false;
+  if (qi < q1) {
+    f = qlist[qi];
+    I3 tf = ldt(T, f);
+    V3 a = ldp(P, tf.i), b = ldp(P, tf.j), c = ldp(P, tf.k);
+    const float qlo0 = fminf(a.x, fminf(b.x, c.x)), qhi0 = fmaxf(a.x, fmaxf(b.x, c.x));
+    const float qlo1 = fminf(a.y, fminf(b.y, c.y)), qhi1 = fmaxf(a.y, fmaxf(b.y, c.y));
+    const float qlo2 = fminf(a.z, fminf(b.z, c.z)), qhi2 = fmaxf(a.z, fmaxf(b.z, c.z));
+    // the six comparisons of the overlap test, evaluated without short
+    // circuits (they have no side effects) so the box loads are not
+    // serialised behind branches; four boxes per step
+    const int tn4 = (tn + 3) & ~3;
+    for (int j0 = 0; j0 < tn4 && !full; j0 += 4) {
+      unsigned sep[4];
+      int gg[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        float4 lo = blo[j0 + u], hi = bhi[j0 + u];
+        gg[u] = bid[j0 + u];
+        sep[u] = (unsigned)(qhi0 < lo.x) | (unsigned)(hi.x < qlo0) | (unsigned)(qhi1 < lo.y) |
+                 (unsigned)(hi.y < qlo1) | (unsigned)(qhi2 < lo.z) | (unsigned)(hi.z < qlo2);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        int g = gg[u];
+        if (sep[u] || g < 0 || g == f || full) continue;
+        cand++;
+        if (cnt == CAND_CAP) {   // LDS list full: move it to the pair buffer
+          long long b0 = (long long)atomicAdd(npairs, (unsigned long long)CAND_CAP);
+          for (int q = 0; q < CAND_CAP; q++)
+            if (b0 + q < pair_cap) pairs[b0 + q] = make_int2(f, cl[q * BLOCK + t]);
+          cnt = 0;
+          if (b0 + CAND_CAP > pair_cap) { full = true; continue; }
+        }
+        cl[cnt * BLOCK + t] = g;
+        cnt++;
+      }
+    }
+  }
+  const bool small = f >= 0 && !full;
+  int mine = small ? cnt : 0;
+  int incl = mine;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  int total = __shfl(incl, 63, 64);
+  unsigned long long base = 0;
+  if (lane == 63 && total) base = atomicAdd(npairs, (unsigned long long)total);
+  base = __shfl(base, 63, 64);
+  long long at = (long long)base + (incl - mine);
+  const bool fits = small && at + mine <= pair_cap;
+  // a query may overflow in several tiles; it enters the list once
+  if (f >= 0 && !fits && atomicExch(&oflag[f], 1) == 0) ovf[atomicAdd(novf, 1)] = f;
+  for (int j = 0; j < mine; j++)
+    if (at + j < pair_cap) pairs[at + j] = make_int2(f, cl[j * BLOCK + t]);
+  if (stats) {
+    unsigned long long c64 = cand;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c64 += __shfl_xor(c64, o, 64);
+    if (lane == 0 && c64) atomicAdd(&stats[0], c64);
+  }
+}
+// overflow queries of the exhaustive search: every alive face of the mesh,
+// with the narrow phase inline, S[f] = 1 on the first hit
+__global__ void __launch_bounds__(BLOCK) k_bf_full(const int* __restrict__ ovf, const int* __restrict__ novf,
+                                                   const int* __restrict__ fmesh, const int* __restrict__ apos,
+                                                   const long long* __restrict__ foff,
+                                                   const int* __restrict__ leaf_face,
+                                                   const float4* __restrict__ lbox_lo,
+                                                   const float4* __restrict__ lbox_hi, const float* __restrict__ P,
+                                                   const int* __restrict__ T, uint8_t* __restrict__ S) {
+  int qi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (qi >= *novf) return;
+  int f = ovf[qi];
+  int k = fmesh[f];
+  long long a0 = apos[foff[k]], a1 = apos[foff[k + 1]];
+  I3 tf = ldt(T, f);
+  float pf[3][3];
+  {
+    V3 a = ldp(P, tf.i), b = ldp(P, tf.j), c = ldp(P, tf.k);
+    pf[0][0] = a.x; pf[0][1] = a.y; pf[0][2] = a.z;
+    pf[1][0] = b.x; pf[1][1] = b.y; pf[1][2] = b.z;
+    pf[2][0] = c.x; pf[2][1] = c.y; pf[2][2] = c.z;
+  }
+  float qlo[3], qhi[3];
+#pragma unroll
+  for (int x = 0; x < 3; x++) {
+    qlo[x] = fminf(pf[0][x], fminf(pf[1][x], pf[2][x]));
+    qhi[x] = fmaxf(pf[0][x], fmaxf(pf[1][x], pf[2][x]));
+  }
+  bool hit = false;
+  for (long long l = a0; l < a1 && !hit; l++) {
+    float4 lo = lbox_lo[l], hi = lbox_hi[l];
+    if (qhi[0] < lo.x || hi.x < qlo[0] || qhi[1] < lo.y || hi.y < qlo[1] || qhi[2] < lo.z || hi.z < qlo[2]) continue;
+    int g = leaf_face[l];
+    if (g == f) continue;
+    I3 tg = ldt(T, g);
+    float pg[3][3];
+    V3 a = ldp(P, tg.i), b = ldp(P, tg.j), c = ldp(P, tg.k);
+    pg[0][0] = a.x; pg[0][1] = a.y; pg[0][2] = a.z;
+    pg[1][0] = b.x; pg[1][1] = b.y; pg[1][2] = b.z;
+    pg[2][0] = c.x; pg[2][1] = c.y; pg[2][2] = c.z;
+    hit = pair_hits(pf, pg, tf, tg);
+  }
+  if (hit) S[f] = 1;
 }
 
 __global__ void k_pairs_reset(int* novf, unsigned long long* npairs) {

@@ -78,6 +78,7 @@ struct pamo_ctx {
   std::vector<int> run_faces;   // faces after each round of the last run
   int run_handle = -1;          // engine slot of the last pamo_run_stage2
   long long pair_cap_limit = -1;  // test hook: caps the traversal's pair buffer (-1: none)
+  int guard_mode = 0;             // test hook: 0 by mesh size, 1 tree only, 2 exhaustive only
 
   int* hc = nullptr;   // pinned host mirror of the counters
 
@@ -570,7 +571,7 @@ static void eng_free(pamo_ctx* c) {
                 &g.undo_out, &g.tmp, &g.marked, &g.pm, &g.table, &g.ecost, &g.mp, &g.voff, &g.foff, &g.eoffm,
                 &g.uioff, &g.uioff2, &g.nustk, &g.h, &g.lcnt, &g.ncoll, &g.cut, &g.cutoff, &g.uobase, &g.hm, &g.part,
                 &g.first, &g.nvu, &g.nvoff, &g.nfoff, &g.seg_a, &g.seg_b, &g.root, &g.root_end, &g.scene,
-                &g.flat_cost, &g.g, &g.keep, &g.kpos, &g.newk, &g.vkeep, &g.vpos, &g.fsucc, &g.sface, &g.splane, &g.slen, &g.erank, &g.vedge, &g.uoff, &g.vmin, &g.fcur, &g.ucnt};
+                &g.flat_cost, &g.g, &g.keep, &g.kpos, &g.newk, &g.vkeep, &g.vpos, &g.fsucc, &g.sface, &g.splane, &g.slen, &g.erank, &g.vedge, &g.uoff, &g.vmin, &g.fcur, &g.ucnt, &g.oflag, &g.bfnb, &g.bfoff};
   for (Buf* b : all)
     if (b->p) (void)hipFree(b->p);
   for (auto& s : g.slots) {
@@ -675,6 +676,12 @@ int pamo_batch_take(pamo_ctx* ctx, void* stream, int handle, float* d_verts_out,
   s.outV = Buf();
   s.outF = Buf();
   s.taken = true;
+  return 0;
+}
+
+int pamo_debug_guard(pamo_ctx* ctx, int mode) {
+  if (!ctx || mode < 0 || mode > 2) return -1;
+  ctx->guard_mode = mode;
   return 0;
 }
 

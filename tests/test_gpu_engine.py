@@ -112,13 +112,34 @@ def test_server_results_match(cuda):
         assert _same(out[i][0], ov) and _same(out[i][1], of), i
 
 
+@pytest.mark.parametrize("guard", [1, 2])
+def test_guard_paths_match_oracle(cuda, guard):
+    """The LBVH walk (1) and the exhaustive search (2) each give the
+    oracle's result, on meshes with and without self-intersections."""
+    from pamo_amd import _lib
+    ctx = _lib.Context(0)
+    ctx.debug_guard(guard)
+    for v, f in (_meshes()[1], _two_spheres(), _meshes()[5]):
+        verts, scale, _ = oracle.preprocess(v, f)
+        target = int(0.01 * f.shape[0])
+        ov, of, st = oracle.run(verts, f, scale, target)
+        gv, gf, gst = simplify_stage2(torch.from_numpy(verts).to(cuda), torch.from_numpy(f).to(cuda), scale, target,
+                                      ctx=ctx)
+        print("guard", guard, "mesh", f.shape, gst["rounds"], st["rounds"], flush=True)
+        assert gst["rounds"] == st["rounds"]
+        assert _same(gv.cpu().numpy(), ov) and _same(gf.cpu().numpy(), of)
+
+
+@pytest.mark.parametrize("guard", [1, 2])
 @pytest.mark.parametrize("cap", [0, 333])
-def test_pair_overflow_fallback(cuda, cap):
+def test_pair_overflow_fallback(cuda, cap, guard):
     """A tiny candidate-pair buffer sends most queries (cap 0: all with a
-    candidate) to the single-kernel walk; the result must not change."""
+    candidate) to the fallback (the single-kernel walk, or the per-query
+    exhaustive test); the result must not change."""
     from pamo_amd import _lib
     ctx = _lib.Context(0)
     ctx.debug_pair_cap(cap)
+    ctx.debug_guard(guard)
     for v, f in (_meshes()[0], _two_spheres()):
         verts, scale, _ = oracle.preprocess(v, f)
         target = int(0.01 * f.shape[0])
