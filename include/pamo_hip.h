@@ -44,6 +44,18 @@ const char* pamo_last_error(const pamo_ctx* ctx);
 int pamo_free_forward(pamo_ctx* ctx, void* stream, const float* d_points, const int* d_tris, const int* d_undo,
                       int n_undo, int n_pts, int n_tris, float scale, float threshold, int is_stuck, int init);
 
+/* Replaces CUDSP.forward / CUSimp::forward (simp_cuda/src/pybind.cpp:140-181,
+ * simp_cuda/src/cusimp.cu:587-720), the unguarded round that PaSP runs:
+ * fans, edges, quadrics, costs (no invalid-vertex table, the fan length term
+ * counted twice, no x5 skinny weight), claims and collapses without the
+ * skinny restore, then the vertex map.  No self-intersection check, undo
+ * list, invalid-face removal or line-edge collapse.  Outputs as for
+ * pamo_free_forward (points, tris, occupancy, map; the undo list is empty).
+ * Use one context per simplifier object: the edge costs persist between
+ * calls, as in the reference. */
+int pamo_sp_forward(pamo_ctx* ctx, void* stream, const float* d_points, const int* d_tris, int n_pts, int n_tris,
+                    float scale, float threshold, int init);
+
 /* The five outputs of the last forward (simp_cuda/src/pybind.cpp:74-102):
  * points [n_pts,3], tris [n_tris,3], occupancy [n_pts], vertex map [n_pts],
  * vertex undo list [n_undo].  Device views into the context. */
@@ -60,6 +72,8 @@ int pamo_free_copy_outputs(pamo_ctx* ctx, void* stream, float* d_points, int* d_
  * target (or 10), or stays unchanged `tolerance` times.  d_verts are the
  * centred vertices, scale the largest axis extent.  On return out_V and out_F
  * hold the final sizes; fetch the mesh with pamo_run_fetch.
+ * A negative target disables the size exits (target and <= 10 faces), as in
+ * the Blender add-on's loop.
  * stats (may be NULL): [0] rounds [1] exit reason (1 target, 2 <=10 faces,
  * 3 tolerance, 4 iteration cap) [2] collapses [3] undo passes (max)
  * [4] self-intersection checks [5] query faces [6] candidate pairs
@@ -84,6 +98,16 @@ int pamo_run_rounds(pamo_ctx* ctx, int* host_faces, int cap);
 
 /* Copies the mesh of the last pamo_run_stage2 into caller device buffers. */
 int pamo_run_fetch(pamo_ctx* ctx, void* stream, float* d_verts_out, int* d_faces_out);
+
+/* Replaces pamo_simplify of the Blender add-on (pamo_blender/
+ * cusimpinterface.cu:57-161, bound with ctypes in pamo_blender/__init__.py:
+ * 75-79): host arrays in, host arrays out, one process-wide context on the
+ * current device.  in_out_nv_ni = {vertices, faces} on entry and the result
+ * sizes on return; out_verts/out_tris must hold the input sizes.  Runs the
+ * PaMO.run loop on the raw (uncentred) vertices with scale = the largest
+ * bounding-box extent, tolerance 4, no target and no size exit.  As in the
+ * reference, errors are printed to stderr; the sizes are then set to -1. */
+void pamo_simplify(float* in_verts, int* in_tris, float* out_verts, int* out_tris, int* in_out_nv_ni, float threshold);
 
 /* Batched stage 2 (no counterpart in the reference, which simplifies one mesh
  * per PaMO.run call, simp_cuda/pamo/__init__.py:107-165).  Many meshes share

@@ -47,6 +47,8 @@ def _lib():
         _orc.orc_destroy.argtypes = [P]
         _orc.orc_forward.argtypes = [P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                      ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        _orc.orc_forward_sp.argtypes = [P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                        ctypes.c_int, ctypes.c_int]
         _orc.orc_sizes.argtypes = [P, P]
         _orc.orc_copy.argtypes = [P, ctypes.c_int, P, ctypes.c_longlong]
         _orc.orc_copy.restype = ctypes.c_longlong
@@ -118,6 +120,18 @@ class Oracle:
         if rc != 0:
             raise RuntimeError("oracle forward failed with code %d" % rc)
         return self.outputs()
+
+    def forward_sp(self, points, tris, scale, threshold, init):
+        """The unguarded round of ``CUDSP.forward`` (simp_cuda/src/cusimp.cu:
+        587-720); returns (points, tris, occ, map)."""
+        points = np.ascontiguousarray(points, np.float32)
+        tris = np.ascontiguousarray(tris, np.int32)
+        rc = self._lib.orc_forward_sp(self._ctx, _ptr(points), _ptr(tris), int(points.shape[0]), int(tris.shape[0]),
+                                      float(scale), float(threshold), int(bool(init)), int(self.shuffle))
+        if rc != 0:
+            raise RuntimeError("oracle forward_sp failed with code %d" % rc)
+        return (self.get("points"), self.get("tris"), self.get("occ").reshape(-1, 1),
+                self.get("pts_map").reshape(-1, 1))
 
     def sizes(self):
         out = np.zeros(16, np.int64)

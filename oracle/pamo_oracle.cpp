@@ -461,7 +461,10 @@ struct Ctx {
 
   // One skinny/flip pass (cusimp_free.cu:497-575).  Returns false when a
   // normal flips (cost becomes max).
-  bool skinny_pass(int moved, int other, V3 m, float& Qa, int& nt) const {
+  // sp: the unguarded variant's term (cusimp.cu:423,463): sqrt(3) with an
+  // integer argument is the double overload, so the quotient is evaluated in
+  // double and rounded to float once (D10)
+  bool skinny_pass(int moved, int other, V3 m, float& Qa, int& nt, bool sp = false) const {
     for (int s = fan_off[moved]; s < fan_off[moved + 1]; s++) {
       int f = fan[s];
       int i = T[3 * f], j = T[3 * f + 1], k = T[3 * f + 2];
@@ -477,7 +480,9 @@ struct Ctx {
       nn = vdivs(nn, vnorm(nn));
       if (vdot(on, nn) < 0) return false;
       float e01 = vnorm(vsub(n0, n1)), e12 = vnorm(vsub(n1, n2)), e20 = vnorm(vsub(n2, n0));
-      float term = 4.0f * sqrtf(3.0f) * tri_area(n0, n1, n2) / (e01 * e01) + e12 * e12 + e20 * e20 + 0.0000001f;
+      float term = sp ? (float)(4.0 * sqrt(3.0) * (double)tri_area(n0, n1, n2) / (double)(e01 * e01) +
+                                (double)(e12 * e12) + (double)(e20 * e20) + (double)0.0000001f)
+                      : 4.0f * sqrtf(3.0f) * tri_area(n0, n1, n2) / (e01 * e01) + e12 * e12 + e20 * e20 + 0.0000001f;
       Qa += 1.0f - clamp_ref(term, 0.0f, 1.0f);
       nt++;
     }
@@ -492,11 +497,15 @@ struct Ctx {
   }
 
   // cusimp_free.cu:382-585
-  void edge_cost_one(int e, bool is_stuck) {
+  // sp: CUSimp's cost (cusimp.cu:283-482): no table or stuck mode, the fan
+  // length term counted twice (:378), no x5 skinny weight, D10
+  void edge_cost_one(int e, bool is_stuck, bool sp = false) {
     int u = edges[2 * e], v = edges[2 * e + 1];
     V3 v0 = P[u], v1 = P[v];
-    if (table[u] && table[v]) { edge_cost[e] = 0xFFFFFFFFu; return; }
-    if (is_stuck) return;
+    if (!sp) {
+      if (table[u] && table[v]) { edge_cost[e] = 0xFFFFFFFFu; return; }
+      if (is_stuck) return;
+    }
     int dup = 0;
     for (int a = fan_off[u]; a < fan_off[u + 1]; a++) {
       int iu = succ(&T[3 * fan[a]], u);
@@ -517,12 +526,13 @@ struct Ctx {
       }
     }
     el = el / (float)ne / scale * tau;
-    el = el + vnorm(vsub(v0, v1)) / scale * tau;
+    if (sp) el += el + vnorm(vsub(v0, v1)) / scale * tau;
+    else el = el + vnorm(vsub(v0, v1)) / scale * tau;
     V3 m = vdivs(vadd(v0, v1), 2.0f);
     float Qa = 0; int nt = 0;
-    if (!skinny_pass(u, v, m, Qa, nt)) { edge_cost[e] = 0xFFFFFFFFu; return; }
-    if (!skinny_pass(v, u, m, Qa, nt)) { edge_cost[e] = 0xFFFFFFFFu; return; }
-    Qa = 5.0f * Qa;
+    if (!skinny_pass(u, v, m, Qa, nt, sp)) { edge_cost[e] = 0xFFFFFFFFu; return; }
+    if (!skinny_pass(v, u, m, Qa, nt, sp)) { edge_cost[e] = 0xFFFFFFFFu; return; }
+    if (!sp) Qa = 5.0f * Qa;
     float M[16];
     for (int r = 0; r < 16; r++) M[r] = Q[(size_t)16 * u + r] + Q[(size_t)16 * v + r];
     float w[4] = {m.x, m.y, m.z, 1.0f};
@@ -594,7 +604,7 @@ struct Ctx {
   }
 
   // cusimp_free.cu:639-774 (winners are independent, so order is free)
-  void collapse() {
+  void collapse(bool sp = false) {
     std::vector<uint8_t> win(E, 0);
 #pragma omp parallel for schedule(static)
     for (int e = 0; e < E; e++) win[e] = wins(e) ? 1 : 0;
@@ -608,6 +618,7 @@ struct Ctx {
       V3 m = vdivs(vadd(P[u], P[v]), 2.0f);
       P[u] = m; P[v] = {0, 0, 0}; occ[v] = 0;
       surgery(u, v);
+      if (sp) continue;   // the unguarded variant keeps every winner (cusimp.cu:511-584)
       bool sk = false;
       for (int w : {u, v}) {
         if (sk) break;
@@ -718,6 +729,48 @@ struct Ctx {
       tmp_undo[2 * j + 1] = edges[2 * marked[j] + 1];
     }
     return (int)marked.size();
+  }
+
+  // CUSimp::forward (cusimp.cu:587-720): the unguarded round of PaSP.
+  int forward_sp(const float* pts, const int* tris, int nV, int nF, float s, float threshold, bool init,
+                 int shuffle_mode) {
+    err = 0;
+    n_queries = n_candidates = n_hits = 0; n_checks = 0; undo_passes = 0;
+    for (int f = 0; f < nF; f++) {
+      const int* t = &tris[3 * f];
+      if (t[0] == -1 && t[1] == -1 && t[2] == -1) continue;
+      for (int k = 0; k < 3; k++) if (t[k] < 0 || t[k] >= nV) return -2;
+    }
+    n_vertices_undo = 0;
+    tau = threshold; scale = s;
+    collapse_t = collapse_threshold(threshold);
+    V = nV; F = nF;
+    ensure_pts(V);
+    P.resize(V);
+    for (int v = 0; v < V; v++) P[v] = {pts[3 * v], pts[3 * v + 1], pts[3 * v + 2]};
+    occ.assign(V + 1, 1); occ[V] = 0;
+    if ((size_t)F > alloc_tris) alloc_tris = F + F / 5;
+    T.assign(tris, tris + 3 * (size_t)F);
+    if (init && shuffle_mode == 1) {
+      std::vector<int64_t> perm;
+      shuffle_permutation(F, perm);
+      std::vector<int> src(T);
+      for (int j = 0; j < F; j++) for (int k = 0; k < 3; k++) T[3 * j + k] = src[3 * perm[j] + k];
+    }
+    P0 = P; T0 = T;
+    build_fans();
+    build_edges();
+    build_quadrics();
+    ensure_edge_cost(E);
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int e = 0; e < E; e++) edge_cost_one(e, false, true);
+    claim();
+    collapse(true);
+    undo_cut = 0;
+    // pts_map = exclusive scan of occ over V + 1 entries (cusimp.cu:713-716)
+    int run = 0;
+    for (int v = 0; v <= V; v++) { int o = occ[v]; pts_map[v] = run; run += o; }
+    return 0;
   }
 
   int forward(const float* pts, const int* tris, const int* undo_in, int n_u, int nV, int nF,
@@ -859,7 +912,8 @@ static int run_loop(Ctx& c, const float* verts, const int* faces, int V, int F, 
     double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (round_secs) round_secs->push_back(el);
     if (max_seconds > 0 && el > max_seconds) { st.exit_reason = 5; break; }
-    if (ncur <= target || ncur <= 10) { st.exit_reason = ncur <= target ? 1 : 2; break; }
+    // a negative target: no size exit (the pamo_blender loop, cusimpinterface.cu:95-157)
+    if (target >= 0 && (ncur <= target || ncur <= 10)) { st.exit_reason = ncur <= target ? 1 : 2; break; }
     if (ncur == nprev) count++;
     else { count = 0; is_stuck = 0; }
     if (count >= 2) is_stuck = 1;
@@ -893,6 +947,12 @@ int orc_forward(void* ctx, const float* pts, const int* tris, const int* undo, i
                 float scale, float threshold, int is_stuck, int init, int shuffle_mode) {
   return ((orc::Ctx*)ctx)->forward(pts, tris, undo, n_undo, n_pts, n_tris, scale, threshold, is_stuck != 0,
                                    init != 0, shuffle_mode);
+}
+
+// CUSimp::forward (the unguarded round); outputs through orc_copy 0-3.
+int orc_forward_sp(void* ctx, const float* pts, const int* tris, int nV, int nF, float scale, float threshold,
+                   int init, int shuffle_mode) {
+  return ((orc::Ctx*)ctx)->forward_sp(pts, tris, nV, nF, scale, threshold, init != 0, shuffle_mode);
 }
 
 // sizes: V, F, E, n_vertices_undo, n_collapsed(total), n_winners, undo_passes, n_checks,

@@ -87,7 +87,37 @@ class CUDSP_Free:
 
 
 class CUDSP:
-    """Unguarded variant (``pamo._C.CUDSP``, simp_cuda/src/cusimp.cu)."""
+    """Unguarded simplifier context (``pamo._C.CUDSP``, simp_cuda/src/pybind.cpp:
+    119-183 over CUSimp::forward, simp_cuda/src/cusimp.cu:587-720): one
+    collapse round without the self-intersection guard, the skinny-face
+    restore, invalid-face removal or line-edge collapse."""
 
-    def __init__(self):
-        raise NotImplementedError("CUDSP (the unguarded PaSP variant) is not built yet; use CUDSP_Free")
+    def __init__(self, shuffle=_lib.SHUFFLE_FEISTEL):
+        self._ctx = _lib.Context(torch.cuda.current_device(), shuffle)
+
+    def forward(self, points, triangles, scale, threshold, init):
+        _check_input(points, "points")
+        _check_input(triangles, "triangles")
+        if points.dtype != torch.float32:
+            raise RuntimeError("points type must match the sp class")
+        if triangles.dtype != torch.int32:
+            raise RuntimeError("triangles type must match the sp class")
+        n_pts = int(points.shape[0])
+        n_tris = int(triangles.shape[0])
+        L = self._ctx._L
+        stream = _stream_handle()
+        rc = L.pamo_sp_forward(self._ctx.handle, stream, points.data_ptr(), triangles.data_ptr(), n_pts, n_tris,
+                               float(scale), float(threshold), int(bool(init)))
+        self._ctx.check(rc, "CUDSP.forward")
+        import ctypes
+        nv, nf, nu = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._ctx.check(L.pamo_free_outputs(self._ctx.handle, None, None, None, None, None, ctypes.byref(nv),
+                                            ctypes.byref(nf), ctypes.byref(nu)), "outputs")
+        dev = points.device
+        verts = torch.empty((nv.value, 3), dtype=torch.float32, device=dev)
+        tris = torch.empty((nf.value, 3), dtype=torch.int32, device=dev)
+        occ = torch.empty((nv.value, 1), dtype=torch.int32, device=dev)
+        vmap = torch.empty((nv.value, 1), dtype=torch.int32, device=dev)
+        self._ctx.check(L.pamo_free_copy_outputs(self._ctx.handle, stream, verts.data_ptr(), tris.data_ptr(),
+                                                 occ.data_ptr(), vmap.data_ptr(), None), "copy outputs")
+        return verts, tris, occ, vmap

@@ -176,8 +176,49 @@ class PaMO(torch.nn.Module):
 
 
 class PaSP(torch.nn.Module):
-    """Unguarded simplifier (simp_cuda/pamo/__init__.py:181-227)."""
+    """Unguarded simplifier (simp_cuda/pamo/__init__.py:180-227): rounds of
+    _C.CUDSP.forward with the same compaction as PaMO.run, until a round
+    removes no face or `iter` rounds have run.  No centring; scale is the
+    largest axis extent of the input."""
 
     def __init__(self):
         super().__init__()
-        raise NotImplementedError("PaSP (unguarded variant) is not built yet; use PaMO")
+        self.sp = _C.CUDSP()
+
+    def run(self, points, triangles, threshold=0.001, iter=1000):
+        verts = points
+        faces = triangles
+        scale = max(max(verts[:, 0].max() - verts[:, 0].min(), verts[:, 1].max() - verts[:, 1].min()),
+                    verts[:, 2].max() - verts[:, 2].min())
+        init = True
+        self.rounds = 0
+        for it in range(iter):
+            num_faces = faces.shape[0]
+            verts, faces, verts_occ, verts_map = self.sp.forward(verts, faces, float(scale), threshold, init)
+            verts = verts[verts_occ.view(-1).bool()]
+            faces = faces[faces[:, 0] >= 0]
+            faces[:, 0] = verts_map[faces[:, 0].long()].view(-1)
+            faces[:, 1] = verts_map[faces[:, 1].long()].view(-1)
+            faces[:, 2] = verts_map[faces[:, 2].long()].view(-1)
+            init = False
+            self.rounds = it + 1
+            if faces.shape[0] == num_faces:
+                break
+        return verts, faces
+
+
+def simplify_host(points, faces, threshold=1e-3):
+    """The Blender add-on's call (pamo_blender/__init__.py:55-60 over
+    pamo_simplify): host arrays in and out, the guarded loop on the raw
+    vertices until the face count stalls 4 times.  Returns (verts, faces)."""
+    import numpy as np
+    v = np.ascontiguousarray(points, np.float32)
+    t = np.ascontiguousarray(faces, np.int32)
+    vo = np.empty_like(v)
+    to = np.empty_like(t)
+    sh = np.array([v.shape[0], t.shape[0]], np.int32)
+    _lib.lib().pamo_simplify(v.ctypes.data, t.ctypes.data, vo.ctypes.data, to.ctypes.data, sh.ctypes.data,
+                             float(threshold))
+    if sh[0] < 0:
+        raise _lib.PamoError("pamo_simplify failed (see stderr)")
+    return vo[:sh[0]].copy(), to[:sh[1]].copy()

@@ -46,6 +46,8 @@ def lib():
     L.pamo_last_error.restype = ctypes.c_char_p
     L.pamo_free_forward.argtypes = [P, P, P, P, P, I, I, I, F, F, I, I]
     L.pamo_free_forward.restype = I
+    L.pamo_sp_forward.argtypes = [P, P, P, P, I, I, F, F, I]
+    L.pamo_sp_forward.restype = I
     L.pamo_free_outputs.argtypes = [P] + [P] * 5 + [ctypes.POINTER(I)] * 3
     L.pamo_free_outputs.restype = I
     L.pamo_free_copy_outputs.argtypes = [P, P, P, P, P, P, P]
@@ -68,6 +70,8 @@ def lib():
     L.pamo_debug_pair_cap.restype = I
     L.pamo_debug_guard.argtypes = [P, I]
     L.pamo_debug_guard.restype = I
+    L.pamo_simplify.argtypes = [P, P, P, P, P, F]
+    L.pamo_simplify.restype = None
     _lib = L
     return L
 

@@ -1201,6 +1201,12 @@ __global__ void __launch_bounds__(BLOCK) k_bf_emit(
       }
     }
   }
+  // one reservation per workgroup: lane prefix within the wave, wave
+  // prefix within the block, one atomic
+  __shared__ int wtot[BLOCK / 64];
+  __shared__ unsigned long long wbase;
+  __shared__ unsigned wcand[BLOCK / 64];
+  const int wid = t >> 6;
   const bool small = f >= 0 && !full;
   int mine = small ? cnt : 0;
   int incl = mine;
@@ -1209,22 +1215,31 @@ __global__ void __launch_bounds__(BLOCK) k_bf_emit(
     int y = __shfl_up(incl, o, 64);
     if (lane >= o) incl += y;
   }
-  int total = __shfl(incl, 63, 64);
-  unsigned long long base = 0;
-  if (lane == 63 && total) base = atomicAdd(npairs, (unsigned long long)total);
-  base = __shfl(base, 63, 64);
-  long long at = (long long)base + (incl - mine);
+  unsigned c32 = cand;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c32 += __shfl_xor(c32, o, 64);
+  if (lane == 63) wtot[wid] = incl;
+  if (lane == 0) wcand[wid] = c32;
+  __syncthreads();
+  int wpre = 0, btot = 0;
+  unsigned bcand = 0;
+#pragma unroll
+  for (int w = 0; w < BLOCK / 64; w++) {
+    if (w < wid) wpre += wtot[w];
+    btot += wtot[w];
+    bcand += wcand[w];
+  }
+  if (t == 0) {
+    wbase = btot ? atomicAdd(npairs, (unsigned long long)btot) : 0ull;
+    if (stats && bcand) atomicAdd(&stats[0], (unsigned long long)bcand);
+  }
+  __syncthreads();
+  long long at = (long long)wbase + wpre + (incl - mine);
   const bool fits = small && at + mine <= pair_cap;
   // a query may overflow in several tiles; it enters the list once
   if (f >= 0 && !fits && atomicExch(&oflag[f], 1) == 0) ovf[atomicAdd(novf, 1)] = f;
   for (int j = 0; j < mine; j++)
     if (at + j < pair_cap) pairs[at + j] = make_int2(f, cl[j * BLOCK + t]);
-  if (stats) {
-    unsigned long long c64 = cand;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) c64 += __shfl_xor(c64, o, 64);
-    if (lane == 0 && c64) atomicAdd(&stats[0], c64);
-  }
 }
 // overflow queries of the exhaustive search: every alive face of the mesh,
 // with the narrow phase inline, S[f] = 1 on the first hit

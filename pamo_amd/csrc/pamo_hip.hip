@@ -16,6 +16,7 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <mutex>
 
 #include "../../include/pamo_hip.h"
 #include "pamo_bvh.h"
@@ -243,7 +244,7 @@ struct pamo_ctx {
   bool node_boxes(const int* n_alive, const int* sorted_leaf, long long Fl);
   bool check_and_mark(int* n_marked);
   int forward(const float* d_pts, const int* d_tris, const int* d_undo, int n_u, int nV, int nF, float scale,
-              float thr, int is_stuck, int init);
+              float thr, int is_stuck, int init, int sp = 0);
   ~pamo_ctx();
 };
 
@@ -395,8 +396,9 @@ bool pamo_ctx::check_and_mark(int* n_marked) {
 }
 
 int pamo_ctx::forward(const float* d_pts, const int* d_tris, const int* d_undo, int n_u, int nV, int nF, float sc,
-                      float thr, int is_stuck, int init) {
+                      float thr, int is_stuck, int init, int sp) {
   err.clear();
+  if (sp && (n_u != 0 || is_stuck)) { fail("the unguarded round takes no undo list"); return -1; }
   if (nV < 0 || nF < 0 || n_u < 0) { fail("negative size"); return -1; }
   if (!hip(hipSetDevice(device), "hipSetDevice")) return -1;
   if (!ensure(counters, C_COUNT * 4) || !ensure(cstats, 64 * 4)) return -1;
@@ -491,7 +493,7 @@ int pamo_ctx::forward(const float* d_pts, const int* d_tris, const int* d_undo, 
                                         4.0 * E);
   k_edge_cost<<<nblk(E), BLOCK, 0, st>>>(ptr<float>(P), ptr<int>(T), ptr<int>(fan_off), ptr<int>(fan), ptr<int>(edges),
                                         E, ptr<float>(Q), ptr<uint8_t>(table), ptr<uint32_t>(edge_cost), sc, tau,
-                                        is_stuck);
+                                        is_stuck, sp);
   prof_end(ph);
   k_fill_u64<<<nblk(F), BLOCK, 0, st>>>(ptr<u64>(tri_min), ~0ull, F);
   ph = prof_begin(PAMO_K_CLAIM, 4.0 * E + 8.0 * E + 4.0 * (V + 1) + 12.0 * F + 8.0 * F);
@@ -500,8 +502,15 @@ int pamo_ctx::forward(const float* d_pts, const int* d_tris, const int* d_undo, 
   prof_end(ph);
   k_collapse<<<nblk(E), BLOCK, 0, st>>>(ptr<int>(edges), E, ptr<uint32_t>(edge_cost), collapse_t_, ptr<int>(fan_off),
                                        ptr<int>(fan), ptr<u64>(tri_min), ptr<float>(P), ptr<float>(P0), ptr<int>(T),
-                                       ptr<int>(T0), ptr<int>(occ), ptr<int>(win));
+                                       ptr<int>(T0), ptr<int>(occ), ptr<int>(win), sp);
   if (!check_launch("cost/claim/collapse")) return -1;
+  if (sp) {   // the unguarded round ends here (cusimp.cu:713-716): the vertex map
+    n_winners = n_collapsed = cut = 0;
+    undo_passes = 0;
+    if (!scan(ptr<int>(occ), ptr<int>(pts_map), V + 1, C_NEWV) || !read_counters()) return -1;
+    n_cand = n_hitfaces = 0;
+    return 0;
+  }
   if (!scan(ptr<int>(win), ptr<int>(winpos), E, C_NWIN)) return -1;
   k_compact_idx<<<nblk(E), BLOCK, 0, st>>>(ptr<int>(win), ptr<int>(winpos), E, ptr<int>(collapsed), 0);
   k_remove_invalid_faces<<<nblk(F), BLOCK, 0, st>>>(ptr<int>(T), F);
@@ -679,6 +688,60 @@ int pamo_batch_take(pamo_ctx* ctx, void* stream, int handle, float* d_verts_out,
   return 0;
 }
 
+// The pamo_blender entry point (pamo_blender/cusimpinterface.cu:57-161):
+// host arrays in and out, one process-wide context, the PaMO.run loop with
+// tolerance 4 and no size exit, scale = the largest bounding-box extent of
+// the raw (uncentred) input.  Errors are printed, as there; the counts are
+// then set to -1.
+static int pamo_simplify_run(pamo_ctx* ctx, const float* in_verts, const int* in_tris, float* out_verts, int* out_tris,
+                      int* in_out_nv_ni, float threshold) {
+  const int nv = in_out_nv_ni[0], nt = in_out_nv_ni[1];
+  if (nv < 0 || nt < 0) { ctx->fail("negative sizes"); return -1; }
+  float bbox[6] = {-1e30f, -1e30f, -1e30f, 1e30f, 1e30f, 1e30f};
+  for (int i = 0; i < nv; i++)
+    for (int a = 0; a < 3; a++) {
+      bbox[a] = std::max(bbox[a], in_verts[3 * i + a]);
+      bbox[3 + a] = std::min(bbox[3 + a], in_verts[3 * i + a]);
+    }
+  const float scale = std::max({bbox[0] - bbox[3], bbox[1] - bbox[4], bbox[2] - bbox[5]});
+  pamo_ctx& c = *ctx;
+  if (!c.hip(hipSetDevice(c.device), "hipSetDevice") || !c.ensure(c.RV[1], (size_t)std::max(nv, 1) * 12) ||
+      !c.ensure(c.RF[1], (size_t)std::max(nt, 1) * 12))
+    return -1;
+  c.st = nullptr;
+  if (nv && !c.hip(hipMemcpy(c.RV[1].p, in_verts, (size_t)nv * 12, hipMemcpyHostToDevice), "copy in")) return -1;
+  if (nt && !c.hip(hipMemcpy(c.RF[1].p, in_tris, (size_t)nt * 12, hipMemcpyHostToDevice), "copy in")) return -1;
+  int oV = 0, oF = 0;
+  int rc = pamo_run_stage2(ctx, nullptr, c.ptr<float>(c.RV[1]), c.ptr<int>(c.RF[1]), nv, nt, scale, -1, threshold, 4,
+                           1000000, &oV, &oF, nullptr);
+  if (rc) return rc;
+  if (!c.ensure(c.RV[0], (size_t)std::max(oV, 1) * 12) || !c.ensure(c.RF[0], (size_t)std::max(oF, 1) * 12)) return -1;
+  rc = pamo_run_fetch(ctx, nullptr, c.ptr<float>(c.RV[0]), c.ptr<int>(c.RF[0]));
+  if (rc) return rc;
+  if (oV && !c.hip(hipMemcpy(out_verts, c.RV[0].p, (size_t)oV * 12, hipMemcpyDeviceToHost), "copy out")) return -1;
+  if (oF && !c.hip(hipMemcpy(out_tris, c.RF[0].p, (size_t)oF * 12, hipMemcpyDeviceToHost), "copy out")) return -1;
+  in_out_nv_ni[0] = oV;
+  in_out_nv_ni[1] = oF;
+  return 0;
+}
+
+void pamo_simplify(float* in_verts, int* in_tris, float* out_verts, int* out_tris, int* in_out_nv_ni, float threshold) {
+  static std::mutex mu;
+  static pamo_ctx* ctx = nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  int dev = 0;
+  if (!ctx && (hipGetDevice(&dev) != hipSuccess || pamo_ctx_create(&ctx, dev, PAMO_SHUFFLE_FEISTEL) != 0)) {
+    fprintf(stderr, "pamo_simplify: no usable GPU\n");
+    ctx = nullptr;
+    in_out_nv_ni[0] = in_out_nv_ni[1] = -1;
+    return;
+  }
+  if (pamo_simplify_run(ctx, in_verts, in_tris, out_verts, out_tris, in_out_nv_ni, threshold) != 0) {
+    fprintf(stderr, "pamo_simplify: %s\n", pamo_last_error(ctx));
+    in_out_nv_ni[0] = in_out_nv_ni[1] = -1;
+  }
+}
+
 int pamo_debug_guard(pamo_ctx* ctx, int mode) {
   if (!ctx || mode < 0 || mode > 2) return -1;
   ctx->guard_mode = mode;
@@ -734,6 +797,14 @@ int pamo_free_forward(pamo_ctx* ctx, void* stream, const float* d_points, const 
   if (!ctx) return -1;
   ctx->st = (hipStream_t)stream;
   return ctx->forward(d_points, d_tris, d_undo, n_undo, n_pts, n_tris, scale, threshold, is_stuck, init);
+}
+
+int pamo_sp_forward(pamo_ctx* ctx, void* stream, const float* d_points, const int* d_tris, int n_pts, int n_tris,
+                    float scale, float threshold, int init) {
+  if (!ctx) return -1;
+  ctx->st = (hipStream_t)stream;
+  ctx->run_handle = -1;
+  return ctx->forward(d_points, d_tris, nullptr, 0, n_pts, n_tris, scale, threshold, 0, init, 1);
 }
 
 int pamo_free_outputs(pamo_ctx* ctx, float** d_points, int** d_tris, int** d_occ, int** d_map, int** d_undo,
@@ -852,7 +923,7 @@ int pamo_run_stage2_rounds(pamo_ctx* ctx, void* stream, const float* d_verts, co
     F = c.hc[C_NEWF];
     c.run_faces.push_back(F);
     cur = nxt;
-    if (F <= target || F <= 10) { st[1] = F <= target ? 1 : 2; break; }
+    if (target >= 0 && (F <= target || F <= 10)) { st[1] = F <= target ? 1 : 2; break; }
     if (F == nprev) count++;
     else { count = 0; is_stuck = 0; }
     if (count >= 2) is_stuck = 1;

@@ -365,9 +365,12 @@ __device__ __forceinline__ float tri_area(V3 p0, V3 p1, V3 p2) {   // cusimp_fre
   float s = (a + b + c) / 2;
   return sqrtf(s * (s - a) * (s - b) * (s - c));
 }
-// one skinny/flip pass, cusimp_free.cu:497-575; false when a normal flips
+// one skinny/flip pass, cusimp_free.cu:497-575; false when a normal flips.
+// sp: the unguarded variant's term (cusimp.cu:423,463), where sqrt(3) takes
+// an integer and so is the double overload: the whole quotient is evaluated
+// in double and rounded to float once (D10).
 __device__ bool skinny_pass(const float* __restrict__ P, const int* __restrict__ T, const int* __restrict__ fan_off,
-                            const int* __restrict__ fan, int moved, int other, V3 m, float& Qa, int& nt) {
+                            const int* __restrict__ fan, int moved, int other, V3 m, float& Qa, int& nt, int sp = 0) {
   int a = fan_off[moved], b = fan_off[moved + 1];
   for (int s = a; s < b; s++) {
     I3 t = ldt(T, fan[s]);
@@ -380,8 +383,15 @@ __device__ bool skinny_pass(const float* __restrict__ P, const int* __restrict__
     nn = vdivs(nn, vnorm(nn));
     if (vdot(on, nn) < 0) return false;
     float e01 = vnorm(vsub(n0, n1)), e12 = vnorm(vsub(n1, n2)), e20 = vnorm(vsub(n2, n0));
-    const float k43 = 4.0f * 1.73205077648162841796875f;   // 4.0f * sqrtf(3.0f)
-    float term = k43 * tri_area(n0, n1, n2) / (e01 * e01) + e12 * e12 + e20 * e20 + 0.0000001f;
+    float term;
+    if (sp) {
+      const double k43d = 4.0 * 1.7320508075688772;   // (double)4.0f * sqrt(3.0), exact product
+      term = (float)(k43d * (double)tri_area(n0, n1, n2) / (double)(e01 * e01) + (double)(e12 * e12) +
+                     (double)(e20 * e20) + (double)0.0000001f);
+    } else {
+      const float k43 = 4.0f * 1.73205077648162841796875f;   // 4.0f * sqrtf(3.0f)
+      term = k43 * tri_area(n0, n1, n2) / (e01 * e01) + e12 * e12 + e20 * e20 + 0.0000001f;
+    }
     Qa += 1.0f - clamp_ref(term, 0.0f, 1.0f);
     nt++;
   }
@@ -400,17 +410,22 @@ __device__ __forceinline__ float fan_edge_len(const float* __restrict__ P, const
   return el;
 }
 
-// cusimp_free.cu:382-585
+// cusimp_free.cu:382-585; sp: the unguarded variant (cusimp.cu:283-482):
+// no invalid-vertex table or stuck mode, the fan length term counted twice
+// (:378), no x5 weight on the skinny term, the skinny term in double
 __global__ void __launch_bounds__(BLOCK) k_edge_cost(const float* __restrict__ P, const int* __restrict__ T,
                                                      const int* __restrict__ fan_off, const int* __restrict__ fan,
                                                      const int* __restrict__ edges, int E,
                                                      const float* __restrict__ Q, const uint8_t* __restrict__ table,
-                                                     uint32_t* __restrict__ cost, float scale, float tau, int is_stuck) {
+                                                     uint32_t* __restrict__ cost, float scale, float tau, int is_stuck,
+                                                     int sp = 0) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
   int u = edges[2 * e], v = edges[2 * e + 1];
-  if (table[u] && table[v]) { cost[e] = 0xFFFFFFFFu; return; }
-  if (is_stuck) return;
+  if (!sp) {
+    if (table[u] && table[v]) { cost[e] = 0xFFFFFFFFu; return; }
+    if (is_stuck) return;
+  }
   V3 v0 = ldp(P, u), v1 = ldp(P, v);
   int ua = fan_off[u], ub = fan_off[u + 1], va = fan_off[v], vb = fan_off[v + 1];
   int dup = 0;
@@ -427,13 +442,14 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost(const float* __restrict__ P
   float el = fan_edge_len(P, T, fan_off, fan, u, 0.0f, ne);
   el = fan_edge_len(P, T, fan_off, fan, v, el, ne);
   el = el / (float)ne / scale * tau;
-  el = el + vnorm(vsub(v0, v1)) / scale * tau;
+  if (sp) el = el + (el + vnorm(vsub(v0, v1)) / scale * tau);   // edge_len += edge_len + ...
+  else el = el + vnorm(vsub(v0, v1)) / scale * tau;
   V3 m = vdivs(vadd(v0, v1), 2.0f);
   float Qa = 0.0f;
   int nt = 0;
-  if (!skinny_pass(P, T, fan_off, fan, u, v, m, Qa, nt)) { cost[e] = 0xFFFFFFFFu; return; }
-  if (!skinny_pass(P, T, fan_off, fan, v, u, m, Qa, nt)) { cost[e] = 0xFFFFFFFFu; return; }
-  Qa = 5.0f * Qa;
+  if (!skinny_pass(P, T, fan_off, fan, u, v, m, Qa, nt, sp)) { cost[e] = 0xFFFFFFFFu; return; }
+  if (!skinny_pass(P, T, fan_off, fan, v, u, m, Qa, nt, sp)) { cost[e] = 0xFFFFFFFFu; return; }
+  if (!sp) Qa = 5.0f * Qa;
   const float* qu = Q + (long long)10 * u;
   const float* qv = Q + (long long)10 * v;
   float s[10];
@@ -510,7 +526,8 @@ __global__ void __launch_bounds__(BLOCK) k_collapse(const int* __restrict__ edge
                                                     uint32_t collapse_t, const int* __restrict__ fan_off,
                                                     const int* __restrict__ fan, const u64* __restrict__ tri_min,
                                                     float* P, const float* __restrict__ P0, int* T,
-                                                    const int* __restrict__ T0, int* occ, int* __restrict__ win) {
+                                                    const int* __restrict__ T0, int* occ, int* __restrict__ win,
+                                                    int sp = 0) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
   uint32_t c = cost[e];
@@ -531,6 +548,7 @@ __global__ void __launch_bounds__(BLOCK) k_collapse(const int* __restrict__ edge
   stp(P, v, V3{0.0f, 0.0f, 0.0f});
   occ[v] = 0;
   surgery(T, fan_off, fan, u, v);
+  if (sp) return;   // the unguarded variant keeps every winner (cusimp.cu:511-584)
   bool sk = false;
   for (int w = 0; w < 2 && !sk; w++) {
     int x = w == 0 ? u : v;

@@ -149,3 +149,18 @@ def test_pair_overflow_fallback(cuda, cap, guard):
         print("cap", cap, "mesh", f.shape, gst["rounds"], st["rounds"], flush=True)
         assert gst["rounds"] == st["rounds"]
         assert _same(gv.cpu().numpy(), ov) and _same(gf.cpu().numpy(), of)
+
+
+def test_pamo_simplify_host_abi(cuda):
+    """pamo_simplify (the Blender add-on's host-pointer entry point) equals
+    the oracle's loop on the raw vertices with the bounding-box scale, no
+    size exit and tolerance 4."""
+    from pamo_amd import simplify_host
+    for v, f in (_meshes()[0], _two_spheres()):
+        v = np.ascontiguousarray(v, np.float32)
+        scale = float(np.max(v.max(0) - v.min(0)))
+        ov, of, st = oracle.run(v, f, scale, -1, 1e-3, 4)
+        hv, hf = simplify_host(v, f, 1e-3)
+        print("pamo_simplify", f.shape, st, flush=True)
+        assert st["exit_reason"] in (3, 4)
+        assert _same(hv, ov) and _same(hf, of)

@@ -161,3 +161,39 @@ def test_candidate_pairs_exact(cuda, name):
     assert np.array_equal(rows[:, :2], cand)
     gh = rows[rows[:, 2] == 1][:, :2]
     assert np.array_equal(gh, hits)
+
+
+@pytest.mark.parametrize("name", ["ico3", "torus", "two_spheres", "line_edge"])
+def test_sp_forward_exact(cuda, name):
+    """CUDSP.forward (the unguarded round) against the oracle: the four
+    outputs bit for bit, and the edge costs."""
+    v, f = MESHES[name]()
+    verts = np.asarray(v, np.float32)
+    scale = float(np.max(verts.max(0) - verts.min(0)))
+    print("case", name, f.shape, flush=True)
+    g = _C.CUDSP()
+    o = oracle.Oracle()
+    gout = g.forward(torch.from_numpy(verts).to(cuda), torch.from_numpy(f).to(cuda), scale, 1e-3, True)
+    oout = o.forward_sp(verts, f, scale, 1e-3, True)
+    for n, a, b in zip(NAMES[:4], gout, oout):
+        assert _same(a.cpu().numpy(), b), n
+
+
+def test_pasp_run_matches_oracle_loop(cuda):
+    """PaSP.run against the same loop over the oracle's unguarded round."""
+    from pamo_amd import PaSP
+    v, f = MESHES["ico5"]()
+    verts = np.asarray(v, np.float32)
+    gv, gf = PaSP().run(torch.from_numpy(verts).to(cuda), torch.from_numpy(f).to(cuda), threshold=1e-3)
+    scale = float(np.max(verts.max(0) - verts.min(0)))
+    o = oracle.Oracle()
+    ov, of = verts, f
+    for it in range(1000):
+        n0 = of.shape[0]
+        P, T, occ, vmap = o.forward_sp(ov, of, scale, 1e-3, it == 0)
+        ov = P[occ.ravel() == 1]
+        of = vmap.ravel()[T[T[:, 0] >= 0]].astype(np.int32)
+        if of.shape[0] == n0:
+            break
+    print("PaSP", f.shape, "->", of.shape, it + 1, flush=True)
+    assert _same(gv.cpu().numpy(), ov) and _same(gf.cpu().numpy(), of)

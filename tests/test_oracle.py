@@ -167,6 +167,54 @@ def test_edge_cost_closed_form_octahedron():
     assert len(set(cost.tolist())) == 1
 
 
+def test_unguarded_edge_cost_closed_form_octahedron():
+    """The unguarded cost (cusimp.cu:283-482) of the same octahedron edge: the
+    fan length term is counted twice (edge_len += edge_len + ..., :378) and the
+    skinny term carries no x5 weight (it is 0 here anyway), so
+    c = (4/3)/s^2 + (2 sqrt2/s + sqrt2/s) tau with s = 2."""
+    v, f = _octahedron()
+    tau = 1e-3
+    o = oracle.Oracle(shuffle=oracle.SHUFFLE_IDENTITY)
+    o.forward_sp(v, f, 2.0, tau, True)
+    edges = o.get("edges")
+    cost = o.get("edge_cost")
+    idx = [i for i, (a, b) in enumerate(edges) if (a, b) == (0, 2)]
+    assert len(idx) == 1
+    c = (4.0 / 3.0) / 4.0 + (2 * np.sqrt(2) / 2 + np.sqrt(2) / 2) * tau
+    want = c / 10.0 * 2.0 ** 32
+    assert abs(int(cost[idx[0]]) - want) < 2e-6 * want
+    assert len(set(cost.tolist())) == 1
+
+
+def test_unguarded_rounds_keep_topology():
+    """Rounds of the unguarded variant (no guard, no skinny restore, no
+    invalid-face removal): winners collapse independently, each removes its
+    two faces, no face gets a repeated corner, and the compacted mesh stays
+    closed and manifold with the same Euler characteristic; the rounds end
+    when one removes no face (PaSP.run, simp_cuda/pamo/__init__.py:196-222)."""
+    v, f = synth.displaced_icosphere(3, 9)
+    verts = np.asarray(v, np.float32)
+    scale = float(np.max(verts.max(0) - verts.min(0)))
+    chi0 = oracle.verify(verts, f)["euler"]
+    o = oracle.Oracle()
+    rounds = 0
+    for r in range(200):
+        n0 = f.shape[0]
+        P, T, occ, vmap = o.forward_sp(verts, f, scale, 1e-3, r == 0)
+        rounds += 1
+        alive = T[:, 0] >= 0
+        assert T.shape[0] - alive.sum() == 2 * int((occ == 0).sum())
+        At = T[alive]
+        assert not np.any((At[:, 0] == At[:, 1]) | (At[:, 1] == At[:, 2]) | (At[:, 2] == At[:, 0]))
+        verts = P[occ.ravel() == 1]
+        f = vmap.ravel()[At].astype(np.int32)
+        res = oracle.verify(verts, f)
+        assert res["manifold"] and res["watertight"] and res["euler"] == chi0, res
+        if f.shape[0] == n0:
+            break
+    assert 1 < rounds < 200
+
+
 def test_cost_quantization_edges():
     """collapse_t for tau=1e-3 is 429496 (float32 arithmetic of
     cusimp_free.cu:923)."""
