@@ -170,8 +170,8 @@ def run_plain(args):
                                              (n, len(round_faces), 100 * frac, crs[n - 1] if n else 0.0),
                                       rounds_match_gpu=bool(match))
     if not args.no_verify:
-        import oracle
-        r = oracle.verify(out_v, out_f)
+        from pamo_amd.verify import verify
+        r = verify(out_v, out_f)
         result["verify"] = {k: r[k] for k in ("manifold", "watertight", "degenerate_faces", "intersecting_pairs",
                                                "ok")}
     print(json.dumps(result), flush=True)
@@ -304,10 +304,10 @@ def run_batch(args):
     if not args.no_cpu:
         result["cpu_baseline"] = _cpu_sample(meshes, sample, args.cpu_seconds)
     if not args.no_verify:
-        import oracle
+        from pamo_amd.verify import verify
         bad = 0
         for i in sample[:16]:
-            r = oracle.verify(outs[i][0].cpu().numpy(), outs[i][1].cpu().numpy())
+            r = verify(outs[i][0].cpu().numpy(), outs[i][1].cpu().numpy())
             bad += 0 if (r["manifold"] and r["watertight"] and r["intersecting_pairs"] == 0) else 1
         result["verify"] = dict(meshes_checked=min(16, len(sample)), failing=bad)
     print(json.dumps(result), flush=True)

@@ -17,14 +17,13 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _ORC = os.path.join(_HERE, "libpamo_oracle.so")
-_VF = os.path.join(_HERE, "libpamo_verify.so")
 
 SHUFFLE_IDENTITY = 0
 SHUFFLE_FEISTEL = 1
 
 
 def build():
-    """Compile the oracle and verifier (make in this directory)."""
+    """Compile the oracle (make in this directory)."""
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
 
@@ -35,7 +34,6 @@ def _load(path):
 
 
 _orc = None
-_vf = None
 
 
 def _lib():
@@ -64,18 +62,6 @@ def _lib():
                                        ctypes.c_float, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, ctypes.c_double,
                                        P, P, P, ctypes.c_int]
     return _orc
-
-
-def _vlib():
-    global _vf
-    if _vf is None:
-        _vf = _load(_VF)
-        P = ctypes.c_void_p
-        _vf.vf_check.argtypes = [P, P, ctypes.c_int, ctypes.c_int, P, P, ctypes.c_longlong]
-        _vf.vf_orient3d.argtypes = [P, P, P, P]
-        _vf.vf_orient3d_exact.argtypes = [P, P, P, P]
-        _vf.vf_tri_pair.argtypes = [P, P, P, P]
-    return _vf
 
 
 def _ptr(a):
@@ -243,42 +229,6 @@ def full_check(points, tris):
     _lib().orc_full_check(_ptr(points), _ptr(tris), points.shape[0], tris.shape[0], _ptr(cand), cand.size + 1,
                           _ptr(hits), hits.size + 1, _ptr(cnt))
     return cand, hits
-
-
-VERIFY_KEYS = ["faces", "vertices", "edges", "boundary_edges", "nonmanifold_edges", "misoriented_edges",
-               "nonmanifold_vertices", "unreferenced_vertices", "degenerate_faces", "intersecting_pairs",
-               "euler", "bad_indices"]
-
-
-def verify(points, tris, max_pairs=64):
-    """Exact check: manifold, watertight, degenerate faces, self-intersections."""
-    points = np.ascontiguousarray(points, np.float32)
-    tris = np.ascontiguousarray(tris, np.int32)
-    st = np.zeros(12, np.int64)
-    pairs = np.zeros((max_pairs, 2), np.int32)
-    _vlib().vf_check(_ptr(points), _ptr(tris), points.shape[0], tris.shape[0], _ptr(st), _ptr(pairs), max_pairs)
-    res = dict(zip(VERIFY_KEYS, [int(x) for x in st]))
-    res["pairs"] = pairs[:min(max_pairs, res["intersecting_pairs"])]
-    res["manifold"] = (res["nonmanifold_edges"] == 0 and res["nonmanifold_vertices"] == 0
-                       and res["misoriented_edges"] == 0 and res["bad_indices"] == 0)
-    res["watertight"] = res["boundary_edges"] == 0 and res["nonmanifold_edges"] == 0
-    res["ok"] = (res["manifold"] and res["watertight"] and res["degenerate_faces"] == 0
-                 and res["intersecting_pairs"] == 0)
-    return res
-
-
-def orient3d(a, b, c, d, exact_only=False):
-    f = _vlib().vf_orient3d_exact if exact_only else _vlib().vf_orient3d
-    arrs = [np.ascontiguousarray(x, np.float64).reshape(3) for x in (a, b, c, d)]
-    return int(f(*[_ptr(x) for x in arrs]))
-
-
-def tri_pair_intersects(a, b, ia, ib):
-    a = np.ascontiguousarray(a, np.float32).reshape(9)
-    b = np.ascontiguousarray(b, np.float32).reshape(9)
-    ia = np.ascontiguousarray(ia, np.int32).reshape(3)
-    ib = np.ascontiguousarray(ib, np.int32).reshape(3)
-    return bool(_vlib().vf_tri_pair(_ptr(a), _ptr(b), _ptr(ia), _ptr(ib)))
 
 
 def preprocess(points, faces):

@@ -2,6 +2,7 @@
 
 libpamo_hip.so   the gfx950 HIP kernels + C ABI (hipcc)
 libpamo_synth.so the seeded SDF mesh generator used by benchmarks (g++)
+libpamo_verify.so the exact mesh verifier (g++)
 
 Float flags matter for parity with the reference: no FMA contraction
 (the reference is built with nvcc --fmad=false, simp_cuda/setup.py:34),
@@ -47,9 +48,19 @@ def build_synth():
     return out
 
 
+def build_verify():
+    """The exact verifier (FP64 filtered predicates with an exact fallback)."""
+    src = os.path.join(HERE, "csrc", "verify.cpp")
+    out = os.path.join(HERE, "libpamo_verify.so")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math", "-fopenmp",
+          "-o", out, src])
+    return out
+
+
 def build_all():
     build_hip()
     build_synth()
+    build_verify()
 
 
 if __name__ == "__main__":

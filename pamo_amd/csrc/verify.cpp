@@ -1,4 +1,4 @@
-// EXACT MESH VERIFIER (test infrastructure only).
+// EXACT MESH VERIFIER (libpamo_verify.so; CLI: python -m pamo_amd.verify).
 //
 // Checks a triangle mesh for (1) manifoldness and watertightness,
 // (2) degenerate faces and (3) self-intersections, with exact predicates.

@@ -1,5 +1,5 @@
 """Pins the CPU oracle (oracle/pamo_oracle.cpp) and the exact verifier
-(oracle/verify.cpp) with checks that do not restate their formulas:
+(pamo_amd/csrc/verify.cpp) with checks that do not restate their formulas:
 golden vectors from rocThrust, closed forms, exact rational arithmetic,
 brute force on tiny inputs and invariants of the collapse."""
 import json
@@ -11,6 +11,7 @@ import pytest
 
 import oracle
 from pamo_amd import synth
+from pamo_amd import verify as pv
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -52,15 +53,15 @@ def test_orient3d_exact_vs_rationals():
         d = d + rng.integers(-2, 3, 3).astype(np.float32) * np.spacing(np.abs(d).astype(np.float32))
         for pts in ((a, b, c, d), (a, b, c, rng.uniform(-1, 1, 3).astype(np.float32))):
             want = _orient_frac(*pts)
-            assert oracle.orient3d(*[p.astype(np.float64) for p in pts]) == want
-            assert oracle.orient3d(*[p.astype(np.float64) for p in pts], exact_only=True) == want
+            assert pv.orient3d(*[p.astype(np.float64) for p in pts]) == want
+            assert pv.orient3d(*[p.astype(np.float64) for p in pts], exact_only=True) == want
             n += want == 0
     # exact zeros: integer-grid points on a common plane
     for _ in range(100):
         a, b, c = rng.integers(-50, 50, (3, 3))
         d = a + 3 * (b - a) - 2 * (c - a)
         pts = [np.asarray(p, np.float64) * 2.0 ** -20 for p in (a, b, c, d)]
-        assert oracle.orient3d(*pts) == 0 == _orient_frac(*pts)
+        assert pv.orient3d(*pts) == 0 == _orient_frac(*pts)
 
 
 def _tri(rng):
@@ -83,7 +84,7 @@ def test_reference_tritri_agrees_with_exact_on_generic_pairs():
         if da.min() < 1e-3 or db.min() < 1e-3 or np.linalg.norm(np.cross(na, nb)) < 1e-3 * np.linalg.norm(na) * np.linalg.norm(nb):
             continue
         hit, r1, r2 = oracle.isect_raw(a, b)
-        exact = oracle.tri_pair_intersects(a, b, [0, 1, 2], [3, 4, 5])
+        exact = pv.tri_pair_intersects(a, b, [0, 1, 2], [3, 4, 5])
         if hit and np.abs(r2 - r1).max() < 1e-3:
             continue
         checked += 1
@@ -106,7 +107,7 @@ def test_narrow_phase_filters():
     # a crossing pair -> hit
     x = np.array([[0.2, 0.2, -0.5], [0.3, 0.3, 0.5], [0.9, -0.5, 0.2]], np.float32)
     assert oracle.narrow(t, x, [0, 1, 2], [3, 4, 5]) == 1
-    assert oracle.tri_pair_intersects(t, x, [0, 1, 2], [3, 4, 5])
+    assert pv.tri_pair_intersects(t, x, [0, 1, 2], [3, 4, 5])
     # far apart -> miss
     assert oracle.narrow(t, x + 5, [0, 1, 2], [3, 4, 5]) == 0
     # shared vertex with a short contact -> filtered (code 5)
@@ -195,7 +196,7 @@ def test_unguarded_rounds_keep_topology():
     v, f = synth.displaced_icosphere(3, 9)
     verts = np.asarray(v, np.float32)
     scale = float(np.max(verts.max(0) - verts.min(0)))
-    chi0 = oracle.verify(verts, f)["euler"]
+    chi0 = pv.verify(verts, f)["euler"]
     o = oracle.Oracle()
     rounds = 0
     for r in range(200):
@@ -208,7 +209,7 @@ def test_unguarded_rounds_keep_topology():
         assert not np.any((At[:, 0] == At[:, 1]) | (At[:, 1] == At[:, 2]) | (At[:, 2] == At[:, 0]))
         verts = P[occ.ravel() == 1]
         f = vmap.ravel()[At].astype(np.int32)
-        res = oracle.verify(verts, f)
+        res = pv.verify(verts, f)
         assert res["manifold"] and res["watertight"] and res["euler"] == chi0, res
         if f.shape[0] == n0:
             break
@@ -316,7 +317,7 @@ def test_round_invariants(mesh):
     else:
         v, f = synth.noisy_torus(40, 16, 2)
     verts, scale, _ = oracle.preprocess(v, f)
-    chi0 = oracle.verify(verts, f)["euler"]
+    chi0 = pv.verify(verts, f)["euler"]
     o = oracle.Oracle()
     undo = np.zeros(0, np.int32)
     for r in range(12):
@@ -329,7 +330,7 @@ def test_round_invariants(mesh):
         assert np.all(changed <= (~alive | np.isin(T, np.nonzero(occ.ravel() == 1)[0]).all(1)))
         verts = P[occ.ravel() == 1]
         f = vmap.ravel()[T[alive]].astype(np.int32)
-        res = oracle.verify(verts, f)
+        res = pv.verify(verts, f)
         assert res["manifold"] and res["watertight"], res
         assert res["euler"] == chi0
 
@@ -353,7 +354,7 @@ def test_line_edge_collapse():
     closed, manifold, has the same Euler characteristic and no alive face
     with two coincident corners."""
     v, f, a, b = _zero_length_edge()
-    chi0 = oracle.verify(v, f)["euler"]
+    chi0 = pv.verify(v, f)["euler"]
     verts, scale, _ = oracle.preprocess(v, f)
     assert np.array_equal(verts[a], verts[b])
     o = oracle.Oracle(shuffle=oracle.SHUFFLE_IDENTITY)
@@ -372,7 +373,7 @@ def test_line_edge_collapse():
     assert np.all(T[~had_b & ~both] == f[~had_b & ~both])
     nv = P[occ.ravel() == 1]
     nf = vmap.ravel()[T[alive]].astype(np.int32)
-    res = oracle.verify(nv, nf)
+    res = pv.verify(nv, nf)
     assert res["manifold"] and res["watertight"] and res["euler"] == chi0, res
     corners = nv[nf]
     assert not np.any(np.all(corners[:, 0] == corners[:, 1], 1) | np.all(corners[:, 1] == corners[:, 2], 1) |
@@ -390,7 +391,7 @@ def test_run_reaches_target_or_tolerance():
     else:
         assert rf[-1] == rf[-2] == rf[-3] == rf[-4] == rf[-5]
     assert np.all(np.diff(rf) <= 0)
-    assert oracle.verify(ov, of)["ok"]
+    assert pv.verify(ov, of)["ok"]
 
 
 @pytest.mark.parametrize("tol", [1, 4, 7])
@@ -466,21 +467,21 @@ def test_undo_leaves_no_flagged_fan():
 # ------------------------------------------------------------- verifier
 def test_verifier_detects_defects():
     v, f = synth.displaced_icosphere(2, 1)
-    assert oracle.verify(v, f)["ok"]
-    r = oracle.verify(v, f[1:])
+    assert pv.verify(v, f)["ok"]
+    r = pv.verify(v, f[1:])
     assert r["boundary_edges"] == 3 and not r["watertight"]
     g = f.copy()
     g[0] = g[0][[0, 2, 1]]
-    assert oracle.verify(v, g)["misoriented_edges"] > 0
+    assert pv.verify(v, g)["misoriented_edges"] > 0
     w = v.copy()
     w[0] = -w[0] * 1.5           # pull a vertex out through the opposite side
-    assert oracle.verify(w, f)["intersecting_pairs"] > 0
+    assert pv.verify(w, f)["intersecting_pairs"] > 0
 
 
 def test_sdf_generator_is_closed_manifold():
     v, f = synth.sdf_union(40, 80, seed=9)
-    r = oracle.verify(v, f)
+    r = pv.verify(v, f)
     assert r["ok"], r
     v, f = synth.sdf_union(30, 100, seed=5, n_slab_pairs=4)
-    r = oracle.verify(v, f)
+    r = pv.verify(v, f)
     assert r["manifold"] and r["watertight"] and r["degenerate_faces"] == 0
