@@ -239,48 +239,69 @@ __global__ void k_edge_vorder(const int* __restrict__ edges, const int* __restri
   vedge[uoff[u] + erank[e]] = make_int4(e, u, v, 0);
 }
 
-// fan edge-length sum from the slots (cusimp_free.cu:449-484)
-__device__ __forceinline__ float fan_len_s(const int4* __restrict__ sface, const float4* __restrict__ slen, int a,
-                                           int b, float el, int& ne) {
-  for (int s = a; s < b; s++) {
-    int4 t = sface[s];
-    float4 L = slen[s];
-    if (t.x > t.y) { el += L.x; ne++; }
-    if (t.y > t.z) { el += L.y; ne++; }
-    if (t.z > t.x) { el += L.z; ne++; }
-  }
-  return el;
-}
-// skinny/flip pass (cusimp_free.cu:497-575) from the slots
-__device__ bool skinny_pass_s(const float* __restrict__ P, const int4* __restrict__ sface,
-                              const float4* __restrict__ splane, int a, int b, int moved, int other, V3 m, float& Qa,
-                              int& nt) {
-  for (int s = a; s < b; s++) {
-    int4 t = sface[s];
-    if (t.x == other || t.y == other || t.z == other) continue;
-    V3 o0 = ldp(P, t.x), o1 = ldp(P, t.y), o2 = ldp(P, t.z);
-    V3 n0 = t.x == moved ? m : o0, n1 = t.y == moved ? m : o1, n2 = t.z == moved ? m : o2;
-    float4 op = splane[s];
-    V3 on = {op.x, op.y, op.z};
-    V3 nn = vcross(vsub(n1, n0), vsub(n2, n0));
-    nn = vdivs(nn, vnorm(nn));
-    if (vdot(on, nn) < 0) return false;
-    float e01 = vnorm(vsub(n0, n1)), e12 = vnorm(vsub(n1, n2)), e20 = vnorm(vsub(n2, n0));
-    float sh = (e01 + e12 + e20) / 2;
-    float area = sqrtf(sh * (sh - e01) * (sh - e12) * (sh - e20));
-    const float k43 = 4.0f * 1.73205077648162841796875f;   // 4.0f * sqrtf(3.0f)
-    float term = k43 * area / (e01 * e01) + e12 * e12 + e20 * e20 + 0.0000001f;
-    Qa += 1.0f - clamp_ref(term, 0.0f, 1.0f);
-    nt++;
-  }
-  return true;
+// ----------------------------------------------------------- edge cost
+// cusimp_free.cu:382-585 with per-mesh parameters.  A group of EC_G lanes
+// takes one edge (edges in vertex order); lane g loads slots g, g+EC_G, ...
+// of each fan, so the loads of a fan issue together instead of as a chain
+// of dependent loads per lane.  What depends on order stays serial: the
+// edge-length sum and the skinny sum are accumulated term by term in fan
+// order (fan u, then fan v) by every lane of the group from shuffled
+// per-slot terms.  A term that the reference does not add is added as +0,
+// which leaves the sum unchanged: both sums start at +0 and every term is
+// +0 or more (a norm, or 1 - clamp(x, 0, 1)) or NaN.  The link count and the
+// flip test are an integer sum and an OR, exact in any order; a flip
+// anywhere makes the cost the maximum, as the reference's early return.
+constexpr int EC_G = 8;
+
+__device__ __forceinline__ int ec_group_sum(int x) {
+#pragma unroll
+  for (int o = 1; o < EC_G; o <<= 1) x += __shfl_xor(x, o, EC_G);
+  return x;
 }
 
-// ----------------------------------------------------------- edge cost
-// cusimp_free.cu:382-585 with per-mesh parameters, one lane per edge in
-// vertex order; the table and the (persistent) cost array are addressed by
-// local indices in the mesh's regions
-__global__ void __launch_bounds__(BLOCK) k_edge_cost_v(const float* __restrict__ P, const int4* __restrict__ vedge,
+// one fan's slots for the edge: serial length and skinny sums, counts, flip
+__device__ __forceinline__ void ec_fan(const float* __restrict__ P, const int4* __restrict__ sface,
+                                       const float4* __restrict__ splane, const float4* __restrict__ slen, int a,
+                                       int b, int g, int moved, int other, V3 m, float& el, float& Qa, int& ne,
+                                       int& nt, int& flip) {
+  for (int base = a; base < b; base += EC_G) {
+    int s = base + g;
+    float tx = 0.0f, ty = 0.0f, tz = 0.0f, tq = 0.0f;
+    if (s < b) {
+      int4 t = sface[s];
+      float4 L = slen[s];
+      if (t.x > t.y) { tx = L.x; ne++; }
+      if (t.y > t.z) { ty = L.y; ne++; }
+      if (t.z > t.x) { tz = L.z; ne++; }
+      if (!(t.x == other || t.y == other || t.z == other)) {
+        V3 n0 = t.x == moved ? m : ldp(P, t.x);
+        V3 n1 = t.y == moved ? m : ldp(P, t.y);
+        V3 n2 = t.z == moved ? m : ldp(P, t.z);
+        float4 op = splane[s];
+        V3 on = {op.x, op.y, op.z};
+        V3 nn = vcross(vsub(n1, n0), vsub(n2, n0));
+        nn = vdivs(nn, vnorm(nn));
+        if (vdot(on, nn) < 0) flip = 1;
+        float e01 = vnorm(vsub(n0, n1)), e12 = vnorm(vsub(n1, n2)), e20 = vnorm(vsub(n2, n0));
+        float sh = (e01 + e12 + e20) / 2;
+        float area = sqrtf(sh * (sh - e01) * (sh - e12) * (sh - e20));
+        const float k43 = 4.0f * 1.73205077648162841796875f;   // 4.0f * sqrtf(3.0f)
+        float term = k43 * area / (e01 * e01) + e12 * e12 + e20 * e20 + 0.0000001f;
+        tq = 1.0f - clamp_ref(term, 0.0f, 1.0f);
+        nt++;
+      }
+    }
+    const int n = b - base < EC_G ? b - base : EC_G;
+    for (int j = 0; j < n; j++) {
+      el += __shfl(tx, j, EC_G);
+      el += __shfl(ty, j, EC_G);
+      el += __shfl(tz, j, EC_G);
+      Qa += __shfl(tq, j, EC_G);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__ P, const int4* __restrict__ vedge,
                                                        const int* __restrict__ E_ptr, const int* __restrict__ fan_off,
                                                        const int* __restrict__ fsucc, const int4* __restrict__ sface,
                                                        const float4* __restrict__ splane,
@@ -290,7 +311,10 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_v(const float* __restrict__
                                                        const long long* __restrict__ voff,
                                                        const long long* __restrict__ eoffm,
                                                        const MeshParam* __restrict__ mp) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // every lane of a group has the same i, so each return below is taken by
+  // the whole group and the width-EC_G shuffles always see all its lanes
+  const int i = (blockIdx.x * blockDim.x + threadIdx.x) / EC_G;
+  const int g = threadIdx.x & (EC_G - 1);
   if (i >= *E_ptr) return;
   int4 ev = vedge[i];
   const int e = ev.x, u = ev.y, v = ev.z;
@@ -298,39 +322,48 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_v(const float* __restrict__
   const long long pmo = mp[k].pmo;
   long long lu = u - voff[k], lv = v - voff[k];
   uint32_t* c = cost + mp[k].eco + (e - eoffm[k]);
-  if (table[pmo + lu] && table[pmo + lv]) { *c = 0xFFFFFFFFu; return; }
+  if (table[pmo + lu] && table[pmo + lv]) {
+    if (g == 0) *c = 0xFFFFFFFFu;
+    return;
+  }
   if (mp[k].stuck) return;
   int ua = fan_off[u], ub = fan_off[u + 1], va = fan_off[v], vb = fan_off[v + 1];
-  // link condition: count of equal successor pairs, max as soon as it passes 2
+  // link condition: equal successor pairs; more than 2 -> maximum, not 2 -> unchanged
   int dup = 0;
-  for (int a = ua; a < ub; a++) {
+  for (int a = ua + g; a < ub; a += EC_G) {
     int iu = fsucc[a];
     for (int b = va; b < vb; b++) dup += fsucc[b] == iu;
-    if (dup > 2) { *c = 0xFFFFFFFFu; return; }
+  }
+  dup = ec_group_sum(dup);
+  if (dup > 2) {
+    if (g == 0) *c = 0xFFFFFFFFu;
+    return;
   }
   if (dup != 2) return;
   const float scale = mp[k].scale, tau = mp[k].tau;
   V3 v0 = ldp(P, u), v1 = ldp(P, v);
-  int ne = 0;
-  float el = fan_len_s(sface, slen, ua, ub, 0.0f, ne);
-  el = fan_len_s(sface, slen, va, vb, el, ne);
+  V3 m = vdivs(vadd(v0, v1), 2.0f);
+  float el = 0.0f, Qa = 0.0f;
+  int ne = 0, nt = 0, flip = 0;
+  ec_fan(P, sface, splane, slen, ua, ub, g, u, v, m, el, Qa, ne, nt, flip);
+  ec_fan(P, sface, splane, slen, va, vb, g, v, u, m, el, Qa, ne, nt, flip);
+  ne = ec_group_sum(ne);
+  nt = ec_group_sum(nt);
+  flip = ec_group_sum(flip);
+  if (g != 0) return;
+  if (flip) { *c = 0xFFFFFFFFu; return; }
   el = el / (float)ne / scale * tau;
   el = el + vnorm(vsub(v0, v1)) / scale * tau;
-  V3 m = vdivs(vadd(v0, v1), 2.0f);
-  float Qa = 0.0f;
-  int nt = 0;
-  if (!skinny_pass_s(P, sface, splane, ua, ub, u, v, m, Qa, nt)) { *c = 0xFFFFFFFFu; return; }
-  if (!skinny_pass_s(P, sface, splane, va, vb, v, u, m, Qa, nt)) { *c = 0xFFFFFFFFu; return; }
   Qa = 5.0f * Qa;
   const float* qu = Q + (long long)10 * u;
   const float* qv = Q + (long long)10 * v;
-  float s[10];
+  float q[10];
 #pragma unroll
-  for (int j = 0; j < 10; j++) s[j] = qu[j] + qv[j];
-  float t0 = s[0] * m.x + s[1] * m.y + s[2] * m.z + s[3] * 1.0f;
-  float t1 = s[1] * m.x + s[4] * m.y + s[5] * m.z + s[6] * 1.0f;
-  float t2 = s[2] * m.x + s[5] * m.y + s[7] * m.z + s[8] * 1.0f;
-  float t3 = s[3] * m.x + s[6] * m.y + s[8] * m.z + s[9] * 1.0f;
+  for (int j = 0; j < 10; j++) q[j] = qu[j] + qv[j];
+  float t0 = q[0] * m.x + q[1] * m.y + q[2] * m.z + q[3] * 1.0f;
+  float t1 = q[1] * m.x + q[4] * m.y + q[5] * m.z + q[6] * 1.0f;
+  float t2 = q[2] * m.x + q[5] * m.y + q[7] * m.z + q[8] * 1.0f;
+  float t3 = q[3] * m.x + q[6] * m.y + q[8] * m.z + q[9] * 1.0f;
   float qe = t0 * m.x + t1 * m.y + t2 * m.z + t3 * 1.0f;
   float cc = qe / (scale * scale);
   *c = u32sat(clamp_ref(cc + el + Qa / (float)nt * tau, 0.0f, COST_RANGE) / COST_RANGE * U32F);
