@@ -164,9 +164,24 @@ __global__ void k_shuffle_scatter_off(Feistel fb, long long n, long long m, cons
   for (int c = 0; c < 3; c++) T[3LL * o + c] = tin[3 * k + c] + voff;
 }
 
+// ------------------------------------------------------ packed transfers
+// up to COPY_SEGS word ranges copied in one launch (grid.y = range)
+constexpr int COPY_SEGS = 8;
+struct CopySegs {
+  const int* src[COPY_SEGS];
+  int* dst[COPY_SEGS];
+  int nw[COPY_SEGS];
+  int n;
+};
+__global__ void k_copy_segs(CopySegs cs) {
+  const int q = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < cs.n && i < cs.nw[q]) cs.dst[q][i] = cs.src[q][i];
+}
+
 // ------------------------------------------------------ fan slots
-// One thread per vertex: sorts its fan (ascending face index, D0) and writes,
-// per fan slot, what the edge costs read about that face, so the cost kernel
+// Per vertex: sorts its fan (ascending face index, D0) and writes, per fan
+// slot, what the edge costs read about that face, so the cost kernel
 // walks contiguous slot arrays instead of gathering face records:
 //   fsucc  successor of the vertex in the face (link condition,
 //          cusimp_free.cu:407-430)
@@ -178,37 +193,88 @@ __global__ void k_shuffle_scatter_off(Feistel fb, long long n, long long m, cons
 // it also takes the round-start snapshot of the vertex position.
 // A face's values are computed by each of its three vertices with the same
 // expressions, so all copies are identical.
+// the data of one slot: the face's corners and index, the successor of w,
+// the plane and the edge lengths; returns the plane for the quadric
+__device__ __forceinline__ float4 fan_slot_fill(const float* __restrict__ P, const int* __restrict__ T, int w, int f,
+                                               int s, int* __restrict__ fsucc, int4* __restrict__ sface,
+                                               float4* __restrict__ splane, float4* __restrict__ slen) {
+  I3 t = ldt(T, f);
+  fsucc[s] = succ_of(t, w);
+  sface[s] = make_int4(t.i, t.j, t.k, f);
+  V3 v0 = ldp(P, t.i), v1 = ldp(P, t.j), v2 = ldp(P, t.k);
+  V3 n = vcross(vsub(v1, v0), vsub(v2, v0));
+  n = vdivs(n, vnorm(n));
+  float off = -vdot(n, v0);
+  float4 pl = make_float4(n.x, n.y, n.z, off);
+  splane[s] = pl;
+  slen[s] = make_float4(vnorm(vsub(v0, v1)), vnorm(vsub(v1, v2)), vnorm(vsub(v2, v0)), 0.0f);
+  return pl;
+}
+// Q = sum over the fan, in fan order, of p p^T (upper triangle)
+__device__ __forceinline__ void quadric_add(float* q, float4 p) {
+  q[0] += p.x * p.x; q[1] += p.x * p.y; q[2] += p.x * p.z; q[3] += p.x * p.w;
+  q[4] += p.y * p.y; q[5] += p.y * p.z; q[6] += p.y * p.w;
+  q[7] += p.z * p.z; q[8] += p.z * p.w;
+  q[9] += p.w * p.w;
+}
+
+// A group of FS_G lanes takes one vertex.  The fan is sorted by rank in LDS
+// (face ids in a fan are distinct, so the rank of x is the count of smaller
+// entries), the lanes fill the slots in parallel, and one lane adds the
+// planes into the quadric serially in fan order.  Fans longer than FS_MAX
+// take the serial path on one lane.
+constexpr int FS_G = 8;
+constexpr int FS_MAX = 32;
+constexpr int FS_GROUPS = BLOCK / FS_G;
+
 __global__ void __launch_bounds__(BLOCK) k_fan_slots(const float* __restrict__ P, const int* __restrict__ T,
                                                      const int* __restrict__ fan_off, int* __restrict__ fan, int V,
                                                      int* __restrict__ fsucc, int4* __restrict__ sface,
                                                      float4* __restrict__ splane, float4* __restrict__ slen,
                                                      float* __restrict__ Q, float* __restrict__ P0) {
-  int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= V) return;
-  stp(P0, v, ldp(P, v));   // round-start position snapshot (the undo restore source)
-  int a = fan_off[v], b = fan_off[v + 1];
-  for (int i = a + 1; i < b; i++) {
-    int x = fan[i];
-    int j = i - 1;
-    while (j >= a && fan[j] > x) { fan[j + 1] = fan[j]; j--; }
-    fan[j + 1] = x;
+  __shared__ int s_raw[FS_GROUPS][FS_MAX];
+  __shared__ int s_sorted[FS_GROUPS][FS_MAX];
+  __shared__ float4 s_plane[FS_GROUPS][FS_MAX];
+  const int grp = threadIdx.x / FS_G, g = threadIdx.x & (FS_G - 1);
+  const int v = blockIdx.x * FS_GROUPS + grp;
+  const bool valid = v < V;
+  int a = 0, deg = 0;
+  if (valid) {
+    a = fan_off[v];
+    deg = fan_off[v + 1] - a;
   }
+  const bool small = deg <= FS_MAX;
+  if (valid && small)
+    for (int s = g; s < deg; s += FS_G) s_raw[grp][s] = fan[a + s];
+  __syncthreads();
+  if (valid && small)
+    for (int s = g; s < deg; s += FS_G) {
+      int x = s_raw[grp][s], r = 0;
+      for (int j = 0; j < deg; j++) r += s_raw[grp][j] < x;
+      s_sorted[grp][r] = x;
+    }
+  __syncthreads();
+  if (valid && small)
+    for (int s = g; s < deg; s += FS_G) {
+      int f = s_sorted[grp][s];
+      fan[a + s] = f;
+      s_plane[grp][s] = fan_slot_fill(P, T, v, f, a + s, fsucc, sface, splane, slen);
+    }
+  __syncthreads();
+  if (!valid || g != 0) return;
+  stp(P0, v, ldp(P, v));   // round-start position snapshot (the undo restore source)
   float q[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (int s = a; s < b; s++) {
-    int f = fan[s];
-    I3 t = ldt(T, f);
-    fsucc[s] = succ_of(t, v);
-    sface[s] = make_int4(t.i, t.j, t.k, f);
-    V3 v0 = ldp(P, t.i), v1 = ldp(P, t.j), v2 = ldp(P, t.k);
-    V3 n = vcross(vsub(v1, v0), vsub(v2, v0));
-    n = vdivs(n, vnorm(n));
-    float off = -vdot(n, v0);
-    splane[s] = make_float4(n.x, n.y, n.z, off);
-    slen[s] = make_float4(vnorm(vsub(v0, v1)), vnorm(vsub(v1, v2)), vnorm(vsub(v2, v0)), 0.0f);
-    q[0] += n.x * n.x; q[1] += n.x * n.y; q[2] += n.x * n.z; q[3] += n.x * off;
-    q[4] += n.y * n.y; q[5] += n.y * n.z; q[6] += n.y * off;
-    q[7] += n.z * n.z; q[8] += n.z * off;
-    q[9] += off * off;
+  if (small) {
+    for (int s = 0; s < deg; s++) quadric_add(q, s_plane[grp][s]);
+  } else {
+    const int b = a + deg;
+    for (int i = a + 1; i < b; i++) {
+      int x = fan[i];
+      int j = i - 1;
+      while (j >= a && fan[j] > x) { fan[j + 1] = fan[j]; j--; }
+      fan[j + 1] = x;
+    }
+    for (int s = a; s < b; s++) quadric_add(q, fan_slot_fill(P, T, v, fan[s], s, fsucc, sface, splane, slen));
   }
 #pragma unroll
   for (int i = 0; i < 10; i++) Q[(long long)10 * v + i] = q[i];
@@ -310,7 +376,8 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__
                                                        const int* __restrict__ vmesh,
                                                        const long long* __restrict__ voff,
                                                        const long long* __restrict__ eoffm,
-                                                       const MeshParam* __restrict__ mp) {
+                                                       const MeshParam* __restrict__ mp,
+                                                       uint32_t* __restrict__ flat) {
   // every lane of a group has the same i, so each return below is taken by
   // the whole group and the width-EC_G shuffles always see all its lanes
   const int i = (blockIdx.x * blockDim.x + threadIdx.x) / EC_G;
@@ -322,11 +389,17 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__
   const long long pmo = mp[k].pmo;
   long long lu = u - voff[k], lv = v - voff[k];
   uint32_t* c = cost + mp[k].eco + (e - eoffm[k]);
+  // the cost lands in the mesh's persistent region and, by global edge
+  // index, in the flat array the claim and collapse read; where the
+  // reference leaves the cost unchanged the flat entry takes the kept value
   if (table[pmo + lu] && table[pmo + lv]) {
-    if (g == 0) *c = 0xFFFFFFFFu;
+    if (g == 0) *c = flat[e] = 0xFFFFFFFFu;
     return;
   }
-  if (mp[k].stuck) return;
+  if (mp[k].stuck) {
+    if (g == 0) flat[e] = *c;
+    return;
+  }
   int ua = fan_off[u], ub = fan_off[u + 1], va = fan_off[v], vb = fan_off[v + 1];
   // link condition: equal successor pairs; more than 2 -> maximum, not 2 -> unchanged
   int dup = 0;
@@ -336,10 +409,13 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__
   }
   dup = ec_group_sum(dup);
   if (dup > 2) {
-    if (g == 0) *c = 0xFFFFFFFFu;
+    if (g == 0) *c = flat[e] = 0xFFFFFFFFu;
     return;
   }
-  if (dup != 2) return;
+  if (dup != 2) {
+    if (g == 0) flat[e] = *c;
+    return;
+  }
   const float scale = mp[k].scale, tau = mp[k].tau;
   V3 v0 = ldp(P, u), v1 = ldp(P, v);
   V3 m = vdivs(vadd(v0, v1), 2.0f);
@@ -351,7 +427,7 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__
   nt = ec_group_sum(nt);
   flip = ec_group_sum(flip);
   if (g != 0) return;
-  if (flip) { *c = 0xFFFFFFFFu; return; }
+  if (flip) { *c = flat[e] = 0xFFFFFFFFu; return; }
   el = el / (float)ne / scale * tau;
   el = el + vnorm(vsub(v0, v1)) / scale * tau;
   Qa = 5.0f * Qa;
@@ -366,19 +442,7 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__
   float t3 = q[3] * m.x + q[6] * m.y + q[8] * m.z + q[9] * 1.0f;
   float qe = t0 * m.x + t1 * m.y + t2 * m.z + t3 * 1.0f;
   float cc = qe / (scale * scale);
-  *c = u32sat(clamp_ref(cc + el + Qa / (float)nt * tau, 0.0f, COST_RANGE) / COST_RANGE * U32F);
-}
-
-// gather of the per-edge cost from the mesh regions into a flat array (the
-// claim and collapse kernels read costs by global edge index)
-__global__ void k_gather_cost(const uint32_t* __restrict__ cost_reg, const int* __restrict__ edges,
-                              const int* __restrict__ E_ptr, const int* __restrict__ vmesh,
-                              const long long* __restrict__ eoffm, const MeshParam* __restrict__ mp,
-                              uint32_t* __restrict__ flat) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= *E_ptr) return;
-  int k = vmesh[edges[2 * e]];
-  flat[e] = cost_reg[mp[k].eco + (e - eoffm[k])];
+  *c = flat[e] = u32sat(clamp_ref(cc + el + Qa / (float)nt * tau, 0.0f, COST_RANGE) / COST_RANGE * U32F);
 }
 
 // ---------------------------------------------------------------- claim
