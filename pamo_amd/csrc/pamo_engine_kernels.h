@@ -280,23 +280,76 @@ __global__ void __launch_bounds__(BLOCK) k_fan_slots(const float* __restrict__ P
   for (int i = 0; i < 10; i++) Q[(long long)10 * v + i] = q[i];
 }
 
+// ------------------------------------------- scans with their producers
+// Edge list (cusimp_free.cu:306-335): per face the count of its descending
+// half-edges, scanned to the face's first edge; the emit writes the edges in
+// face order and ranks each within its first endpoint (the rank places the
+// edge in the vertex-ordered list, k_edge_vorder).
+struct EdgeScanOp {
+  const int* T;
+  int F;
+  int* eoff;
+  int* edges;
+  int* ucnt;
+  int* erank;
+  __device__ int count(long long f) const {
+    if (f >= F) return 0;
+    I3 t = ldt(T, (int)f);
+    return dead3(t) ? 0 : (int)(t.i > t.j) + (int)(t.j > t.k) + (int)(t.k > t.i);
+  }
+  __device__ void emit(long long f, int o) const {
+    eoff[f] = o;
+    if (f >= F) return;
+    I3 t = ldt(T, (int)f);
+    if (dead3(t)) return;
+    if (t.i > t.j) { edges[2 * o] = t.j; edges[2 * o + 1] = t.i; erank[o] = atomicAdd(&ucnt[t.j], 1); o++; }
+    if (t.j > t.k) { edges[2 * o] = t.k; edges[2 * o + 1] = t.j; erank[o] = atomicAdd(&ucnt[t.k], 1); o++; }
+    if (t.k > t.i) { edges[2 * o] = t.i; edges[2 * o + 1] = t.k; erank[o] = atomicAdd(&ucnt[t.i], 1); o++; }
+  }
+};
+// End-of-round vertex compaction: the scan of occupancy is the new index g,
+// and occupied vertices move to it with their mesh id.
+struct VertCompactOp {
+  const int* occ;
+  int* g;
+  const int* V_ptr;
+  const float* P;
+  const int* vmesh;
+  float* outP;
+  int* outm;
+  __device__ int count(long long v) const { return occ[v]; }
+  __device__ void emit(long long v, int o) const {
+    g[v] = o;
+    if (v >= *V_ptr || !occ[v]) return;
+    outP[3 * o] = P[3 * v]; outP[3 * o + 1] = P[3 * v + 1]; outP[3 * o + 2] = P[3 * v + 2];
+    outm[o] = vmesh[v];
+  }
+};
+// End-of-round face compaction: kept faces (not removed) move to the scan
+// position with their corners renamed through g.
+struct FaceCompactOp {
+  const int* T;
+  const int* F_ptr;
+  int* kpos;
+  const int* g;
+  const int* fmesh;
+  int* outT;
+  int* outm;
+  __device__ int count(long long f) const { return f < *F_ptr && T[3 * f] >= 0 ? 1 : 0; }
+  __device__ void emit(long long f, int o) const {
+    kpos[f] = o;
+    if (f >= *F_ptr || T[3 * f] < 0) return;
+    outT[3 * o] = g[T[3 * f]]; outT[3 * o + 1] = g[T[3 * f + 1]]; outT[3 * o + 2] = g[T[3 * f + 2]];
+    outm[o] = fmesh[f];
+  }
+};
+
 // ------------------------------------------------- vertex-ordered edges
 // Edges keep their index (the claim key's tie-break), but are processed
 // grouped by their first endpoint u, so neighbouring lanes read the same
-// fans.  The emit counts edges per u (the atomic's return value is the
-// edge's rank within u; any rank order gives the same results, since every
-// output depends only on the edge index).
-__global__ void k_edge_emit_u(const int* __restrict__ T, int F, const int* __restrict__ eoff, int* __restrict__ edges,
-                              int* __restrict__ ucnt, int* __restrict__ erank) {
-  int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= F) return;
-  I3 t = ldt(T, f);
-  if (dead3(t)) return;
-  int o = eoff[f];
-  if (t.i > t.j) { edges[2 * o] = t.j; edges[2 * o + 1] = t.i; erank[o] = atomicAdd(&ucnt[t.j], 1); o++; }
-  if (t.j > t.k) { edges[2 * o] = t.k; edges[2 * o + 1] = t.j; erank[o] = atomicAdd(&ucnt[t.k], 1); o++; }
-  if (t.k > t.i) { edges[2 * o] = t.i; edges[2 * o + 1] = t.k; erank[o] = atomicAdd(&ucnt[t.i], 1); o++; }
-}
+// fans.  The edge emit counts edges per u (the atomic's return value is
+// the edge's rank within u; any rank order gives the same results, since
+// every output depends only on the edge index).
 __global__ void k_edge_vorder(const int* __restrict__ edges, const int* __restrict__ E_ptr,
                               const int* __restrict__ uoff, const int* __restrict__ erank, int4* __restrict__ vedge) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -451,7 +504,8 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__
 // 587-610).  A face is in the fan of x exactly when x is one of its
 // vertices, so that minimum is the minimum over the face's three vertices
 // of the per-vertex minimum over incident edges: two atomics per edge
-// instead of one per fan face, and min is exact in any order.
+// instead of one per fan face, and min is exact in any order.  The collapse
+// reads the face minimum from vmin and the round-start faces directly.
 __global__ void __launch_bounds__(BLOCK) k_vertex_min(const int4* __restrict__ vedge, const int* __restrict__ E_ptr,
                                                       const uint32_t* __restrict__ cost, u64* vmin) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -461,69 +515,126 @@ __global__ void __launch_bounds__(BLOCK) k_vertex_min(const int4* __restrict__ v
   atomicMin(&vmin[ev.y], key);
   atomicMin(&vmin[ev.z], key);
 }
-__global__ void __launch_bounds__(BLOCK) k_face_min(const int* __restrict__ T, int F, const u64* __restrict__ vmin,
-                                                    u64* __restrict__ tri_min) {
-  int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= F) return;
-  I3 t = ldt(T, f);
-  if (dead3(t)) return;   // dead faces are in no fan and never read
-  u64 a = vmin[t.i], b = vmin[t.j], c = vmin[t.k];
-  u64 m = a < b ? a : b;
-  tri_min[f] = m < c ? m : c;
-}
 
-__global__ void __launch_bounds__(BLOCK) k_collapse_b(const int* __restrict__ edges, const int* __restrict__ E_ptr,
-                                                      const uint32_t* __restrict__ cost, const int* __restrict__ vmesh,
-                                                      const MeshParam* __restrict__ mp,
-                                                      const int* __restrict__ fan_off, const int* __restrict__ fan,
-                                                      const u64* __restrict__ tri_min, float* P,
-                                                      const float* __restrict__ P0, int* T, const int* __restrict__ T0,
-                                                      int* occ, int* __restrict__ win) {
+// Winner decision, one lane per edge: an edge wins when it passes the
+// threshold (cusimp_free.cu:650) and holds the minimum key of every face in
+// both fans (:658-681).  A face's minimum is the least vertex minimum over
+// its round-start corners (the fan slots' sface, written before any
+// collapse of this round).  u and v have non-empty fans (the edge's own
+// face), so vmin[u] == vmin[v] == key is necessary; it rejects almost every
+// edge with two loads issued with the first ones.  Given it, a face holds
+// key as its minimum exactly when no corner has a smaller vertex minimum.
+// Winners' fans are disjoint, so applying them later, in any order, gives
+// the reference's result.
+__global__ void __launch_bounds__(BLOCK) k_winners_b(const int* __restrict__ edges, const int* __restrict__ E_ptr,
+                                                     const uint32_t* __restrict__ cost, const int* __restrict__ vmesh,
+                                                     const MeshParam* __restrict__ mp,
+                                                     const int* __restrict__ fan_off, const u64* __restrict__ vmin,
+                                                     const int4* __restrict__ sface, int* __restrict__ win) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   int E = *E_ptr;
   if (e > E) return;
   if (e == E) { win[E] = 0; return; }
   uint32_t c = cost[e];
   int u = edges[2 * e], v = edges[2 * e + 1];
-  bool ok = c <= mp[vmesh[u]].ct;
-  if (ok) {
-    u64 key = ((u64)c << 32) | (u64)(uint32_t)e;
-    for (int w = 0; w < 2 && ok; w++) {
-      int x = w == 0 ? u : v;
-      for (int s = fan_off[x]; s < fan_off[x + 1]; s++)
-        if (tri_min[fan[s]] != key) { ok = false; break; }
+  const u64 key = ((u64)c << 32) | (u64)(uint32_t)e;
+  const u64 mu = vmin[u], mv = vmin[v];
+  const int k = vmesh[u];
+  bool ok = mu == key && mv == key;
+  ok = ok && c <= mp[k].ct;
+  for (int w = 0; w < 2 && ok; w++) {
+    int x = w == 0 ? u : v;
+    for (int s = fan_off[x]; s < fan_off[x + 1] && ok; s++) {
+      int4 t = sface[s];
+      ok = vmin[t.x] >= key && vmin[t.y] >= key && vmin[t.z] >= key;
     }
   }
   win[e] = ok ? 1 : 0;
-  if (!ok) return;
-  V3 m = vdivs(vadd(ldp(P, u), ldp(P, v)), 2.0f);
-  stp(P, u, m);
-  stp(P, v, V3{0.0f, 0.0f, 0.0f});
-  occ[v] = 0;
-  surgery(T, fan_off, fan, u, v);
-  bool sk = false;
-  for (int w = 0; w < 2 && !sk; w++) {
-    int x = w == 0 ? u : v;
-    for (int s = fan_off[x]; s < fan_off[x + 1]; s++) {
-      I3 t = ldt(T, fan[s]);
-      if (t.i == -1) continue;
-      if (is_skinny(ldp(P, t.i), ldp(P, t.j), ldp(P, t.k))) { sk = true; break; }
-    }
-  }
-  if (sk) restore_edge(u, v, P, P0, T, T0, occ, fan_off, fan);
 }
-
-// winners -> the mesh's collapsed region, in ascending edge order
-// (region of mesh k starts at cbase[k] = 2 * edge offset, capacity 2 E_k)
-__global__ void k_winners_to_regions(const int* __restrict__ win, const int* __restrict__ winpos,
-                                     const int* __restrict__ E_ptr, const int* __restrict__ edges,
-                                     const int* __restrict__ vmesh, const long long* __restrict__ eoffm,
-                                     int* __restrict__ coll) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= *E_ptr || !win[e]) return;
-  int k = vmesh[edges[2 * e]];
-  long long base = eoffm[k];
-  coll[2 * base + (winpos[e] - winpos[base])] = e;
+// The winners from the dense list (ascending edge order), one group of
+// CL_G lanes each: the entry in the mesh's collapsed region, then the
+// collapse with the skinny restore (cusimp_free.cu:683-774).  The faces of
+// a winner's fans belong to no other winner and still hold their round-start
+// corners (sface), so the group's lanes take the fan slots in parallel:
+//   surgery    faces with both u and v are removed (by the lane of their
+//              fan-u slot only), the other fan-v faces get u for v;
+//   skinny     any remaining face of either fan skinny with u at the
+//              midpoint (an OR over the lanes: the reference stops at the
+//              first, and only "any" is used);
+//   restore    positions, occupancy and every fan face from the snapshots.
+// Each face is written by one lane, in program order.
+constexpr int CL_G = 8;
+__device__ void collapse_one(int i, int g, const int* __restrict__ wlist, const int* __restrict__ winpos,
+                             const int* __restrict__ edges, const int* __restrict__ vmesh,
+                             const long long* __restrict__ eoffm, int* __restrict__ coll,
+                             const int* __restrict__ fan_off, const int4* __restrict__ sface, float* P,
+                             const float* __restrict__ P0, int* T, const int* __restrict__ T0, int* occ) {
+  const int e = wlist[i];
+  const int u = edges[2 * e], v = edges[2 * e + 1];
+  const V3 pu = ldp(P, u), pv = ldp(P, v);
+  const V3 m = vdivs(vadd(pu, pv), 2.0f);
+  const int ua = fan_off[u], ub = fan_off[u + 1], va = fan_off[v], vb = fan_off[v + 1];
+  if (g == 0) {
+    int k = vmesh[u];
+    long long base = eoffm[k];
+    coll[2 * base + (i - winpos[base])] = e;
+    stp(P, u, m);
+    stp(P, v, V3{0.0f, 0.0f, 0.0f});
+    occ[v] = 0;
+  }
+  int sk = 0;
+  const int du = ub - ua, n = du + (vb - va);
+  for (int j = g; j < n; j += CL_G) {
+    const bool in_u = j < du;
+    const int s = in_u ? ua + j : va + (j - du);
+    const int4 t = sface[s];
+    const bool has_u = t.x == u || t.y == u || t.z == u;
+    const bool has_v = t.x == v || t.y == v || t.z == v;
+    if (has_u && has_v) {   // removed; written once, from its fan-u slot
+      if (in_u) { T[3 * t.w] = -1; T[3 * t.w + 1] = -1; T[3 * t.w + 2] = -1; }
+      continue;
+    }
+    I3 c = {t.x, t.y, t.z};
+    if (!in_u) {            // fan-v face without u: v becomes u
+      if (c.i == v) { c.i = u; T[3 * t.w] = u; }
+      else if (c.j == v) { c.j = u; T[3 * t.w + 1] = u; }
+      else { c.k = u; T[3 * t.w + 2] = u; }
+    }
+    const V3 a = c.i == u ? m : ldp(P, c.i), b = c.j == u ? m : ldp(P, c.j), d = c.k == u ? m : ldp(P, c.k);
+    sk |= is_skinny(a, b, d) ? 1 : 0;
+  }
+#pragma unroll
+  for (int o = 1; o < CL_G; o <<= 1) sk |= __shfl_xor(sk, o, CL_G);
+  if (!sk) return;
+  if (g == 0) {
+    occ[v] = 1;
+    stp(P, u, ldp(P0, u));
+    stp(P, v, ldp(P0, v));
+  }
+  for (int j = g; j < n; j += CL_G) {
+    const bool in_u = j < du;
+    const int s = in_u ? ua + j : va + (j - du);
+    const int4 t = sface[s];
+    if (!in_u && (t.x == u || t.y == u || t.z == u)) continue;   // restored from its fan-u slot
+    const int f = t.w;
+    T[3 * f] = T0[3 * f]; T[3 * f + 1] = T0[3 * f + 1]; T[3 * f + 2] = T0[3 * f + 2];
+  }
+}
+__global__ void __launch_bounds__(BLOCK) k_collapse_b(const int* __restrict__ wlist, const int* __restrict__ W_ptr,
+                                                      const int* __restrict__ winpos, const int* __restrict__ edges,
+                                                      const int* __restrict__ vmesh,
+                                                      const long long* __restrict__ eoffm, int* __restrict__ coll,
+                                                      const int* __restrict__ fan_off,
+                                                      const int4* __restrict__ sface, float* P,
+                                                      const float* __restrict__ P0, int* T,
+                                                      const int* __restrict__ T0, int* occ) {
+  // grid-stride over winners; every lane of a group has the same i, so the
+  // loop exits, returns and shuffles are group-wide
+  const int g = threadIdx.x & (CL_G - 1);
+  const int W = *W_ptr;
+  const int stride = gridDim.x * (BLOCK / CL_G);
+  for (int i = (blockIdx.x * blockDim.x + threadIdx.x) / CL_G; i < W; i += stride)
+    collapse_one(i, g, wlist, winpos, edges, vmesh, eoffm, coll, fan_off, sface, P, P0, T, T0, occ);
 }
 
 // per-mesh edge offsets (faces emit their edges in order)
@@ -533,58 +644,32 @@ __global__ void k_edge_offsets(int M, const long long* __restrict__ foff, const 
   if (k > M) return;
   eoffm[k] = eoff_face[foff[k]];
 }
-// per-mesh winner counts
-__global__ void k_win_counts(int M, const long long* __restrict__ eoffm, const int* __restrict__ winpos,
-                             int* __restrict__ h) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= M) return;
-  h[k] = winpos[eoffm[k + 1]] - winpos[eoffm[k]];
-}
-__global__ void k_part_from_cut(int M, const int* __restrict__ cut, int* __restrict__ part) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < M) part[k] = cut[k] > 0 ? 1 : 0;
-}
-__global__ void k_flat_total(const long long* __restrict__ off, int M, int* out) { *out = (int)off[M]; }
 
-__global__ void k_line_detect_b(const float* __restrict__ P, const int* __restrict__ edges,
-                                const int* __restrict__ E_ptr, const int* __restrict__ vmesh, int* lcnt) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+// After the collapse, in one launch: blocks below nbf remove invalid faces
+// (cusimp_free.cu:218-232), the rest count zero-length edges per mesh (the
+// line-edge pass runs only for meshes that have one).  The two read and
+// write disjoint data.
+__global__ void k_invalid_line_b(int* T, int F, int nbf, const float* __restrict__ P, const int* __restrict__ edges,
+                                 const int* __restrict__ E_ptr, const int* __restrict__ vmesh, int* lcnt) {
+  if ((int)blockIdx.x < nbf) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    I3 t = ldt(T, f);
+    if (dead3(t)) return;
+    if (t.i == t.j || t.j == t.k || t.k == t.i) { T[3 * f] = -1; T[3 * f + 1] = -1; T[3 * f + 2] = -1; }
+    return;
+  }
+  int e = (blockIdx.x - nbf) * blockDim.x + threadIdx.x;
   if (e >= *E_ptr) return;
   int u = edges[2 * e];
   V3 p = ldp(P, u), q = ldp(P, edges[2 * e + 1]);
   if (p.x == q.x && p.y == q.y && p.z == q.z) atomicAdd(&lcnt[vmesh[u]], 1);
 }
-// sequential per mesh (D5), one thread per mesh with candidates; appends to
-// the mesh's collapsed region; then the scan cut (cusimp_free.cu:1016)
-__global__ void k_line_apply_cut(int M, const int* __restrict__ lcnt, float* P, int* T, int* occ,
-                                 const int* __restrict__ edges, const long long* __restrict__ eoffm,
-                                 const int* __restrict__ fan_off, const int* __restrict__ fan, int* coll,
-                                 const int* __restrict__ h, int* __restrict__ ncoll, int* __restrict__ cut) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= M) return;
-  int n = h[k];
-  long long base = 2 * eoffm[k];
-  if (lcnt[k] > 0) {
-    for (long long e = eoffm[k]; e < eoffm[k + 1]; e++) {
-      int u = edges[2 * e], v = edges[2 * e + 1];
-      V3 p = ldp(P, u), q = ldp(P, v);
-      if (!(p.x == q.x && p.y == q.y && p.z == q.z)) continue;
-      coll[base + n++] = (int)e;
-      stp(P, v, V3{0.0f, 0.0f, 0.0f});
-      occ[v] = 0;
-      surgery(T, fan_off, fan, u, v);
-    }
-  }
-  ncoll[k] = n;
-  int hk = h[k];
-  cut[k] = hk == 0 ? 0 : min(n, ((hk + 511) / 512) * 512);
-}
-
-// ------------------------------------------------- flat views over meshes
-// exclusive scan of a small per-mesh int array into long long offsets (one
-// block; M may exceed the block)
-__global__ void k_small_scan(const int* __restrict__ x, int M, long long* __restrict__ off, long long mult,
-                             const int* __restrict__ extra, long long extra_mult) {
+// exclusive scan of a small per-mesh int array (x * mult + extra *
+// extra_mult) into long long offsets, off[M] = total; all threads of one
+// block take part (M may exceed the block)
+__device__ void block_small_scan(const int* __restrict__ x, int M, long long* __restrict__ off, long long mult,
+                                 const int* __restrict__ extra, long long extra_mult) {
   __shared__ long long carry;
   __shared__ long long part[BLOCK];
   if (threadIdx.x == 0) carry = 0;
@@ -607,8 +692,50 @@ __global__ void k_small_scan(const int* __restrict__ x, int M, long long* __rest
     __syncthreads();
   }
   if (threadIdx.x == 0) off[M] = carry;
+  __syncthreads();
+}
+// After the collapse, per mesh, in one block: the winner count; line-edge
+// collapse, sequential per mesh (D5), appending to the mesh's collapsed
+// region; the scan cut
+// (cusimp_free.cu:1016); the flat offsets of the cut entries and of the undo
+// outputs (12 per cut entry plus the incoming list of stuck meshes); and
+// which meshes take part in the check.
+__global__ void __launch_bounds__(BLOCK) k_line_cut_tail(int M, const int* __restrict__ lcnt, float* P, int* T,
+                                                         int* occ, const int* __restrict__ edges,
+                                                         const long long* __restrict__ eoffm,
+                                                         const int* __restrict__ fan_off, const int* __restrict__ fan,
+                                                         int* coll, const int* __restrict__ winpos,
+                                                         int* __restrict__ h, int* __restrict__ ncoll,
+                                                         int* __restrict__ cut, long long* __restrict__ cutoff,
+                                                         long long* __restrict__ uobase,
+                                                         const int* __restrict__ nustk, int* __restrict__ part) {
+  for (int k = threadIdx.x; k < M; k += BLOCK) {
+    const int hk = winpos[eoffm[k + 1]] - winpos[eoffm[k]];   // the mesh's winners
+    h[k] = hk;
+    int n = hk;
+    long long base = 2 * eoffm[k];
+    if (lcnt[k] > 0) {
+      for (long long e = eoffm[k]; e < eoffm[k + 1]; e++) {
+        int u = edges[2 * e], v = edges[2 * e + 1];
+        V3 p = ldp(P, u), q = ldp(P, v);
+        if (!(p.x == q.x && p.y == q.y && p.z == q.z)) continue;
+        coll[base + n++] = (int)e;
+        stp(P, v, V3{0.0f, 0.0f, 0.0f});
+        occ[v] = 0;
+        surgery(T, fan_off, fan, u, v);
+      }
+    }
+    ncoll[k] = n;
+    int ck = hk == 0 ? 0 : min(n, ((hk + 511) / 512) * 512);
+    cut[k] = ck;
+    part[k] = ck > 0 ? 1 : 0;
+  }
+  __syncthreads();
+  block_small_scan(cut, M, cutoff, 1, nullptr, 0);
+  block_small_scan(cut, M, uobase, 12, nustk, 1);
 }
 
+// ------------------------------------------------- flat views over meshes
 // faces of the fans of the scanned collapsed entries (flat list over meshes)
 __global__ void k_qflag_b(const long long* __restrict__ cutoff, int M, const long long* __restrict__ eoffm,
                           const int* __restrict__ coll, const int* __restrict__ edges,
@@ -635,12 +762,6 @@ __global__ void k_query_flags_b(const uint8_t* __restrict__ qflag, const int* __
   bool p = alive && part[fmesh[f]];
   lf[f] = p ? 1 : 0;
   qf[f] = (p && qflag[f]) ? 1 : 0;
-}
-__global__ void k_compact_idx_dev(const int* __restrict__ flags, const int* __restrict__ pos,
-                                  const int* __restrict__ n_ptr, int* __restrict__ out) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= *n_ptr) return;
-  if (flags[i]) out[pos[i]] = i;
 }
 
 // -------------------------------------------------------- segmented LBVH
@@ -1169,15 +1290,16 @@ __global__ void __launch_bounds__(BLOCK) k_traverse_emit(
 
 // One candidate pair per lane (grid-stride over the pairs the walk wrote):
 // both orientations of the reference narrow phase; a hit flags the query.
-__global__ void __launch_bounds__(BLOCK) k_pair_test(const int2* __restrict__ pairs,
-                                                     const unsigned long long* __restrict__ npairs, long long pair_cap,
-                                                     const float* __restrict__ P, const int* __restrict__ T,
-                                                     uint8_t* __restrict__ S, unsigned long long* __restrict__ stats) {
+// pairs handled by blocks bid of nb (grid-stride)
+__device__ void pair_test_blocks(long long bid, long long nb, const int2* __restrict__ pairs,
+                                 const unsigned long long* __restrict__ npairs, long long pair_cap,
+                                 const float* __restrict__ P, const int* __restrict__ T, uint8_t* __restrict__ S,
+                                 unsigned long long* __restrict__ stats) {
   long long n = (long long)*npairs;
   if (n > pair_cap) n = pair_cap;
-  if (stats && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[2], (unsigned long long)n);
+  if (stats && bid == 0 && threadIdx.x == 0) atomicAdd(&stats[2], (unsigned long long)n);
   unsigned hits = 0;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+  for (long long i = bid * blockDim.x + threadIdx.x; i < n; i += nb * blockDim.x) {
     int2 pr = pairs[i];
     I3 tf = ldt(T, pr.x), tg = ldt(T, pr.y);
     float pf[3][3], pg[3][3];
@@ -1205,6 +1327,12 @@ __global__ void __launch_bounds__(BLOCK) k_pair_test(const int2* __restrict__ pa
     if ((threadIdx.x & 63) == 0 && h64) atomicAdd(&stats[1], h64);
   }
 }
+__global__ void __launch_bounds__(BLOCK) k_pair_test(const int2* __restrict__ pairs,
+                                                     const unsigned long long* __restrict__ npairs, long long pair_cap,
+                                                     const float* __restrict__ P, const int* __restrict__ T,
+                                                     uint8_t* __restrict__ S, unsigned long long* __restrict__ stats) {
+  pair_test_blocks(blockIdx.x, gridDim.x, pairs, npairs, pair_cap, P, T, S, stats);
+}
 
 // --------------------------------------------- exhaustive candidate search
 // For small meshes the tree costs more than it saves: a check then tests
@@ -1218,12 +1346,15 @@ __global__ void __launch_bounds__(BLOCK) k_pair_test(const int2* __restrict__ pa
 // same time.  Candidates go through the same LDS lists, flushes and pair
 // buffer as the walk.
 constexpr int BF_TILE = 256;
-__global__ void k_bf_blocks(int M, const long long* __restrict__ foff, const int* __restrict__ qpos,
-                            int* __restrict__ nb) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= M) return;
-  int nq = qpos[foff[k + 1]] - qpos[foff[k]];
-  nb[k] = (nq + BLOCK - 1) / BLOCK;
+__global__ void __launch_bounds__(BLOCK) k_bf_blocks(int M, const long long* __restrict__ foff,
+                                                     const int* __restrict__ qpos, int* __restrict__ nb,
+                                                     long long* __restrict__ boff) {
+  for (int k = threadIdx.x; k < M; k += BLOCK) {
+    int nq = qpos[foff[k + 1]] - qpos[foff[k]];
+    nb[k] = (nq + BLOCK - 1) / BLOCK;
+  }
+  __syncthreads();
+  block_small_scan(nb, M, boff, 1, nullptr, 0);   // query blocks -> block offsets per mesh
 }
 __global__ void __launch_bounds__(BLOCK) k_bf_emit(
     const int* __restrict__ qlist, const int* __restrict__ qpos, const int* __restrict__ apos,
@@ -1269,9 +1400,19 @@ __global__ void __launch_bounds__(BLOCK) k_bf_emit(
     const float qlo2 = fminf(a.z, fminf(b.z, c.z)), qhi2 = fmaxf(a.z, fmaxf(b.z, c.z));
     // the six comparisons of the overlap test, evaluated without short
     // circuits (they have no side effects) so the box loads are not
-    // serialised behind branches; four boxes per step
+    // serialised behind branches; four boxes per step.  The append is
+    // predicated: every box writes the lane's next list slot and only a hit
+    // advances the count, so the step has no branch; the list is moved to
+    // the pair buffer before a step that could overrun it.
     const int tn4 = (tn + 3) & ~3;
-    for (int j0 = 0; j0 < tn4 && !full; j0 += 4) {
+    for (int j0 = 0; j0 < tn4; j0 += 4) {
+      if (cnt > CAND_CAP - 4) {
+        long long b0 = (long long)atomicAdd(npairs, (unsigned long long)cnt);
+        for (int q = 0; q < cnt; q++)
+          if (b0 + q < pair_cap) pairs[b0 + q] = make_int2(f, cl[q * BLOCK + t]);
+        if (b0 + cnt > pair_cap) { full = true; break; }
+        cnt = 0;
+      }
       unsigned sep[4];
       int gg[4];
 #pragma unroll
@@ -1283,18 +1424,11 @@ __global__ void __launch_bounds__(BLOCK) k_bf_emit(
       }
 #pragma unroll
       for (int u = 0; u < 4; u++) {
-        int g = gg[u];
-        if (sep[u] || g < 0 || g == f || full) continue;
-        cand++;
-        if (cnt == CAND_CAP) {   // LDS list full: move it to the pair buffer
-          long long b0 = (long long)atomicAdd(npairs, (unsigned long long)CAND_CAP);
-          for (int q = 0; q < CAND_CAP; q++)
-            if (b0 + q < pair_cap) pairs[b0 + q] = make_int2(f, cl[q * BLOCK + t]);
-          cnt = 0;
-          if (b0 + CAND_CAP > pair_cap) { full = true; continue; }
-        }
+        const int g = gg[u];
+        const int hit = (int)(sep[u] == 0u) & (int)(g >= 0) & (int)(g != f);
         cl[cnt * BLOCK + t] = g;
-        cnt++;
+        cnt += hit;
+        cand += (unsigned)hit;
       }
     }
   }
@@ -1340,15 +1474,26 @@ __global__ void __launch_bounds__(BLOCK) k_bf_emit(
 }
 // overflow queries of the exhaustive search: every alive face of the mesh,
 // with the narrow phase inline, S[f] = 1 on the first hit
-__global__ void __launch_bounds__(BLOCK) k_bf_full(const int* __restrict__ ovf, const int* __restrict__ novf,
-                                                   const int* __restrict__ fmesh, const int* __restrict__ apos,
-                                                   const long long* __restrict__ foff,
-                                                   const int* __restrict__ leaf_face,
-                                                   const float4* __restrict__ lbox_lo,
-                                                   const float4* __restrict__ lbox_hi, const float* __restrict__ P,
-                                                   const int* __restrict__ T, uint8_t* __restrict__ S) {
-  int qi = blockIdx.x * blockDim.x + threadIdx.x;
-  if (qi >= *novf) return;
+struct BfFull {
+  const int* ovf;
+  const int* novf;
+  const int* fmesh;
+  const int* apos;
+  const long long* foff;
+  const int* leaf_face;
+  const float4* lbox_lo;
+  const float4* lbox_hi;
+};
+__device__ void bf_full_query(long long qi, const BfFull& a, const float* __restrict__ P, const int* __restrict__ T,
+                              uint8_t* __restrict__ S) {
+  const int* __restrict__ ovf = a.ovf;
+  const int* __restrict__ fmesh = a.fmesh;
+  const int* __restrict__ apos = a.apos;
+  const long long* __restrict__ foff = a.foff;
+  const int* __restrict__ leaf_face = a.leaf_face;
+  const float4* __restrict__ lbox_lo = a.lbox_lo;
+  const float4* __restrict__ lbox_hi = a.lbox_hi;
+  if (qi >= *a.novf) return;
   int f = ovf[qi];
   int k = fmesh[f];
   long long a0 = apos[foff[k]], a1 = apos[foff[k + 1]];
@@ -1382,19 +1527,24 @@ __global__ void __launch_bounds__(BLOCK) k_bf_full(const int* __restrict__ ovf, 
   }
   if (hit) S[f] = 1;
 }
-
-__global__ void k_pairs_reset(int* novf, unsigned long long* npairs) {
-  *novf = 0;
-  *npairs = 0;
+// the flat pair test (blocks below nbp) and the overflow queries (the rest)
+// of an exhaustive check, in one launch: both only set flags
+__global__ void __launch_bounds__(BLOCK) k_pairs_bf(int nbp, const int2* __restrict__ pairs,
+                                                    const unsigned long long* __restrict__ npairs, long long pair_cap,
+                                                    BfFull a, const float* __restrict__ P, const int* __restrict__ T,
+                                                    uint8_t* __restrict__ S, unsigned long long* __restrict__ stats) {
+  if ((int)blockIdx.x < nbp) pair_test_blocks(blockIdx.x, nbp, pairs, npairs, pair_cap, P, T, S, stats);
+  else bf_full_query((long long)(blockIdx.x - nbp) * blockDim.x + threadIdx.x, a, P, T, S);
 }
 
 // --------------------------------------------------------------- marks
 __global__ void k_mark_b(const long long* __restrict__ cutoff, int M, const long long* __restrict__ eoffm,
                          const int* __restrict__ coll, const int* __restrict__ edges,
                          const int* __restrict__ fan_off, const int* __restrict__ fan, const int* __restrict__ part,
-                         const uint8_t* __restrict__ S, int* __restrict__ mflag) {
+                         const uint8_t* __restrict__ S, int* __restrict__ mflag, int* __restrict__ total) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long tot = cutoff[M];
+  if (i == 0) *total = (int)tot;   // live length of the scan of the marks
   if (i > tot) return;
   if (i == tot) { mflag[tot] = 0; return; }
   int k = find_seg(cutoff, M, i);
@@ -1409,10 +1559,23 @@ __global__ void k_mark_b(const long long* __restrict__ cutoff, int M, const long
   mflag[i] = hit;
 }
 // marked entries -> per-mesh marked list (flat, base cutoff[k]) and endpoint
-// pairs (per mesh, base 2 cutoff[k]); hm[k] = marks of mesh k
+// pairs (per mesh, base 2 cutoff[k]); blocks from nb0 on count marks per
+// mesh: hm[k] = marks of mesh k
 __global__ void k_marked_b(const long long* __restrict__ cutoff, int M, const long long* __restrict__ eoffm,
                            const int* __restrict__ coll, const int* __restrict__ edges, const int* __restrict__ mflag,
-                           const int* __restrict__ mpos, int* __restrict__ marked, int* __restrict__ tmp) {
+                           const int* __restrict__ mpos, int* __restrict__ marked, int* __restrict__ tmp, int nb0,
+                           int* __restrict__ hm, int* __restrict__ part, int* __restrict__ nvu,
+                           int* __restrict__ n_active) {
+  if ((int)blockIdx.x >= nb0) {
+    int k = (blockIdx.x - nb0) * blockDim.x + threadIdx.x;
+    if (k >= M) return;
+    int c = part[k] ? (mpos[cutoff[k + 1]] - mpos[cutoff[k]]) : 0;
+    hm[k] = c;
+    nvu[k] += 2 * c;
+    part[k] = c > 0 ? 1 : 0;
+    if (c > 0) atomicAdd(n_active, 1);
+    return;
+  }
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= cutoff[M] || !mflag[i]) return;
   int k = find_seg(cutoff, M, i);
@@ -1422,37 +1585,28 @@ __global__ void k_marked_b(const long long* __restrict__ cutoff, int M, const lo
   tmp[2 * cutoff[k] + 2 * j] = edges[2 * e];
   tmp[2 * cutoff[k] + 2 * j + 1] = edges[2 * e + 1];
 }
-__global__ void k_mark_counts(int M, const long long* __restrict__ cutoff, const int* __restrict__ mpos,
-                              int* __restrict__ hm, int* __restrict__ part, int* __restrict__ nvu,
-                              int* __restrict__ n_active) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= M) return;
-  int c = part[k] ? (mpos[cutoff[k + 1]] - mpos[cutoff[k]]) : 0;
-  hm[k] = c;
-  nvu[k] += 2 * c;
-  part[k] = c > 0 ? 1 : 0;
-  if (c > 0) atomicAdd(n_active, 1);
-}
-// undo of the marked edges of participating meshes (cusimp_free.cu:829-867)
-__global__ void k_undo_b(const long long* __restrict__ cutoff, int M, const int* __restrict__ hm,
+// One undo pass in one launch: blocks below nbu restore the marked edges of
+// participating meshes (cusimp_free.cu:829-867); the rest write this pass's
+// undo-list entries through the stale map (cusimp_free.cu:869-878), which
+// reads only the endpoint pairs and the previous round's map.
+__global__ void k_undo_b(int nbu, const long long* __restrict__ cutoff, int M, const int* __restrict__ hm,
                          const int* __restrict__ marked, const int* __restrict__ edges, float* P,
                          const float* __restrict__ P0, int* T, const int* __restrict__ T0, int* occ,
-                         const int* __restrict__ fan_off, const int* __restrict__ fan) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= cutoff[M]) return;
-  int k = find_seg(cutoff, M, i);
-  long long j = i - cutoff[k];
-  if (j >= hm[k]) return;
-  int e = marked[cutoff[k] + j];
-  restore_edge(edges[2 * e], edges[2 * e + 1], P, P0, T, T0, occ, fan_off, fan);
-}
-// undo-list entries of this pass through the stale map (cusimp_free.cu:869-878)
-__global__ void k_undo_remap_b(const long long* __restrict__ cutoff, int M, const int* __restrict__ hm,
-                               const int* __restrict__ tmp, const long long* __restrict__ voff,
-                               const MeshParam* __restrict__ mp, const int* __restrict__ pm,
-                               const long long* __restrict__ uobase, int* __restrict__ first,
-                               int* __restrict__ undo_out, int* err) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+                         const int* __restrict__ fan_off, const int* __restrict__ fan, const int* __restrict__ tmp,
+                         const long long* __restrict__ voff, const MeshParam* __restrict__ mp,
+                         const int* __restrict__ pm, const long long* __restrict__ uobase,
+                         const int* __restrict__ first, int* __restrict__ undo_out, int* err) {
+  if ((int)blockIdx.x < nbu) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cutoff[M]) return;
+    int k = find_seg(cutoff, M, i);
+    long long j = i - cutoff[k];
+    if (j >= hm[k]) return;
+    int e = marked[cutoff[k] + j];
+    restore_edge(edges[2 * e], edges[2 * e + 1], P, P0, T, T0, occ, fan_off, fan);
+    return;
+  }
+  long long i = (long long)(blockIdx.x - nbu) * blockDim.x + threadIdx.x;
   if (i >= 2 * cutoff[M]) return;
   int k = find_seg(cutoff, M, i / 2);
   long long j = i - 2 * cutoff[k];
@@ -1488,38 +1642,18 @@ __global__ void k_local_map(const int* __restrict__ g, const long long* __restri
   pm[mp[k].pmo + (v - voff[k])] = g[v] - g[voff[k]];
   if (v == voff[k + 1] - 1) pm[mp[k].pmo + (v - voff[k]) + 1] = g[v + 1] - g[voff[k]];
 }
-__global__ void k_compact_verts_b(const float* __restrict__ P, const int* __restrict__ occ, const int* __restrict__ g,
-                                  const int* __restrict__ vmesh, const int* __restrict__ V_ptr, float* __restrict__ outP,
-                                  int* __restrict__ outm) {
-  int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= *V_ptr || !occ[v]) return;
-  int o = g[v];
-  outP[3 * o] = P[3 * v]; outP[3 * o + 1] = P[3 * v + 1]; outP[3 * o + 2] = P[3 * v + 2];
-  outm[o] = vmesh[v];
-}
-__global__ void k_face_keep_b(const int* __restrict__ T, const int* __restrict__ F_ptr, int* __restrict__ keep) {
-  int f = blockIdx.x * blockDim.x + threadIdx.x;
-  int F = *F_ptr;
-  if (f > F) return;
-  keep[f] = f < F && T[3 * f] >= 0 ? 1 : 0;
-}
-__global__ void k_compact_faces_b(const int* __restrict__ T, const int* __restrict__ keep, const int* __restrict__ pos,
-                                  const int* __restrict__ g, const int* __restrict__ fmesh, const int* __restrict__ F_ptr,
-                                  int* __restrict__ outT, int* __restrict__ outm) {
-  int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= *F_ptr || !keep[f]) return;
-  int o = pos[f];
-  outT[3 * o] = g[T[3 * f]]; outT[3 * o + 1] = g[T[3 * f + 1]]; outT[3 * o + 2] = g[T[3 * f + 2]];
-  outm[o] = fmesh[f];
-}
-// new per-mesh offsets and sizes
-__global__ void k_new_offsets(int M, const long long* __restrict__ voff, const long long* __restrict__ foff,
-                              const int* __restrict__ g, const int* __restrict__ fpos, long long* __restrict__ nvoff,
-                              long long* __restrict__ nfoff) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k > M) return;
-  nvoff[k] = g[voff[k]];
-  nfoff[k] = fpos[foff[k]];
+// new per-mesh offsets and sizes, and the offsets of next round's packed
+// undo input (one block)
+__global__ void __launch_bounds__(BLOCK) k_new_offsets(int M, const long long* __restrict__ voff,
+                                                       const long long* __restrict__ foff, const int* __restrict__ g,
+                                                       const int* __restrict__ fpos, long long* __restrict__ nvoff,
+                                                       long long* __restrict__ nfoff, const int* __restrict__ nvu,
+                                                       long long* __restrict__ uioff_new) {
+  for (int k = threadIdx.x; k <= M; k += BLOCK) {
+    nvoff[k] = g[voff[k]];
+    nfoff[k] = fpos[foff[k]];
+  }
+  block_small_scan(nvu, M, uioff_new, 1, nullptr, 0);
 }
 // next round's packed undo input from the per-mesh undo outputs
 __global__ void k_pack_undo(int M, const long long* __restrict__ uioff_new, const long long* __restrict__ uobase,

@@ -195,13 +195,28 @@ struct pamo_ctx {
     }
     return scan_launch(in, out, n, nullptr, 0, total_slot >= 0 ? cnt_dev(total_slot) : nullptr);
   }
+  // scan of 0/1 flags that also writes the index of every set flag at its
+  // position in list (stream compaction in the same pass)
+  bool scan_compact(const int* in, int* out, long long n, int total_slot, int* list) {
+    if (n <= 0) {
+      if (total_slot >= 0) return set_counter(total_slot, 0);
+      return true;
+    }
+    return scan_launch(in, out, n, nullptr, 0, total_slot >= 0 ? cnt_dev(total_slot) : nullptr, list);
+  }
   // the same scan over a capacity n whose live length is on the device
   // (*d_n + add); entries past the live length are neither read nor written
-  bool scan_dev(const int* in, int* out, long long n, const int* d_n, int add) {
+  bool scan_dev(const int* in, int* out, long long n, const int* d_n, int add, int* total = nullptr,
+                int* list = nullptr) {
     if (n <= 0) return true;
-    return scan_launch(in, out, n, d_n, add, nullptr);
+    return scan_launch(in, out, n, d_n, add, total, list);
   }
-  bool scan_launch(const int* in, int* out, long long n, const int* d_n, int add, int* total) {
+  bool scan_launch(const int* in, int* out, long long n, const int* d_n, int add, int* total, int* list = nullptr) {
+    return scan_op(ScanArray{in, out, list}, n, d_n, add, total);
+  }
+  // one-pass scan whose input and output are the op's count and emit
+  template <class Op> bool scan_op(const Op& op, long long n, const int* d_n, int add, int* total) {
+    if (n <= 0) return true;
     long long nb = (n + SCAN1_TILE - 1) / SCAN1_TILE;
     if (scan_desc.bytes < (size_t)nb * 8) {
       if (!ensure(scan_desc, (size_t)nb * 8)) return false;
@@ -211,8 +226,8 @@ struct pamo_ctx {
       scan_epoch = 1;
       if (!hip(hipMemsetAsync(scan_desc.p, 0, scan_desc.bytes, st), "scan desc")) return false;
     }
-    k_scan_onepass<<<(unsigned)nb, BLOCK1K, 0, st>>>(in, out, n, d_n, add, ptr<unsigned long long>(scan_desc),
-                                                   scan_epoch, total, cnt_dev(C_ERR));
+    k_scan_op<Op><<<(unsigned)nb, BLOCK1K, 0, st>>>(op, n, d_n, add, ptr<unsigned long long>(scan_desc), scan_epoch,
+                                                    total, cnt_dev(C_ERR));
     return check_launch("scan");
   }
   bool read_counters() {

@@ -129,10 +129,13 @@ constexpr int SCAN1_ITEMS = 8, SCAN1_TILE = BLOCK1K * SCAN1_ITEMS;   // 8192 int
 __device__ __forceinline__ unsigned long long scan_word(unsigned epoch, unsigned long long st, long long v) {
   return ((unsigned long long)epoch << 42) | (st << 40) | ((unsigned long long)v & ((1ull << 40) - 1));
 }
-__global__ void __launch_bounds__(BLOCK1K) k_scan_onepass(const int* __restrict__ in, int* __restrict__ out,
-                                                          long long n, const int* __restrict__ d_n, int add,
-                                                          unsigned long long* desc, unsigned epoch,
-                                                          int* __restrict__ total, int* err) {
+// The scan takes its input from op.count(i) and hands each exclusive prefix
+// to op.emit(i, pos), so a producer (counts from faces) and a consumer (a
+// scatter to the prefix position) can share the scan's launch.
+template <class Op>
+__global__ void __launch_bounds__(BLOCK1K) k_scan_op(Op op, long long n, const int* __restrict__ d_n, int add,
+                                                     unsigned long long* desc, unsigned epoch,
+                                                     int* __restrict__ total, int* err) {
   __shared__ unsigned lds[BLOCK1K / 64];
   __shared__ int tile[SCAN1_TILE];
   __shared__ long long excl_s;
@@ -144,9 +147,11 @@ __global__ void __launch_bounds__(BLOCK1K) k_scan_onepass(const int* __restrict_
     return;
   }
   if (base >= live) return;   // tiles past the live length contribute nothing
+  // unrolled so the items' loads are in flight together
+#pragma unroll
   for (int i = 0; i < SCAN1_ITEMS; i++) {
     long long idx = base + (long long)i * BLOCK1K + threadIdx.x;
-    tile[i * BLOCK1K + threadIdx.x] = idx < live ? in[idx] : 0;
+    tile[i * BLOCK1K + threadIdx.x] = idx < live ? op.count(idx) : 0;
   }
   __syncthreads();
   int loc[SCAN1_ITEMS];
@@ -188,11 +193,24 @@ __global__ void __launch_bounds__(BLOCK1K) k_scan_onepass(const int* __restrict_
 #pragma unroll
   for (int i = 0; i < SCAN1_ITEMS; i++) tile[threadIdx.x * SCAN1_ITEMS + i] = off + loc[i];
   __syncthreads();
+#pragma unroll
   for (int i = 0; i < SCAN1_ITEMS; i++) {
     long long idx = base + (long long)i * BLOCK1K + threadIdx.x;
-    if (idx < live) out[idx] = tile[i * BLOCK1K + threadIdx.x];
+    if (idx < live) op.emit(idx, tile[i * BLOCK1K + threadIdx.x]);
   }
 }
+// the plain scan of an int array, and optionally the compaction of 0/1
+// flags: the index of every set entry at its position in list
+struct ScanArray {
+  const int* in;
+  int* out;
+  int* list;
+  __device__ int count(long long i) const { return in[i]; }
+  __device__ void emit(long long i, int pos) const {
+    out[i] = pos;
+    if (list && in[i]) list[pos] = (int)i;
+  }
+};
 
 // ------------------------------------------------------------ step 1
 // cusimp_free.cu:869-878 (stale pts_map remap) and :880-889 (table)
