@@ -292,6 +292,7 @@ struct EdgeScanOp {
   int* edges;
   int* ucnt;
   int* erank;
+  int* win;   // winner flags of the edges (and the entry past the last), cleared
   __device__ int count(long long f) const {
     if (f >= F) return 0;
     I3 t = ldt(T, (int)f);
@@ -299,12 +300,12 @@ struct EdgeScanOp {
   }
   __device__ void emit(long long f, int o) const {
     eoff[f] = o;
-    if (f >= F) return;
+    if (f >= F) { win[o] = 0; return; }
     I3 t = ldt(T, (int)f);
     if (dead3(t)) return;
-    if (t.i > t.j) { edges[2 * o] = t.j; edges[2 * o + 1] = t.i; erank[o] = atomicAdd(&ucnt[t.j], 1); o++; }
-    if (t.j > t.k) { edges[2 * o] = t.k; edges[2 * o + 1] = t.j; erank[o] = atomicAdd(&ucnt[t.k], 1); o++; }
-    if (t.k > t.i) { edges[2 * o] = t.i; edges[2 * o + 1] = t.k; erank[o] = atomicAdd(&ucnt[t.i], 1); o++; }
+    if (t.i > t.j) { edges[2 * o] = t.j; edges[2 * o + 1] = t.i; erank[o] = atomicAdd(&ucnt[t.j], 1); win[o++] = 0; }
+    if (t.j > t.k) { edges[2 * o] = t.k; edges[2 * o + 1] = t.j; erank[o] = atomicAdd(&ucnt[t.k], 1); win[o++] = 0; }
+    if (t.k > t.i) { edges[2 * o] = t.i; edges[2 * o + 1] = t.k; erank[o] = atomicAdd(&ucnt[t.i], 1); win[o++] = 0; }
   }
 };
 // End-of-round vertex compaction: the scan of occupancy is the new index g,
@@ -420,6 +421,18 @@ __device__ __forceinline__ void ec_fan(const float* __restrict__ P, const int4* 
   }
 }
 
+// the cost's destinations: the mesh's persistent region (when the
+// reference writes it), the flat array the claim reads by global edge index,
+// and the claim's per-vertex minima of (cost << 32 | edge)
+__device__ __forceinline__ void ec_finish(uint32_t* c, uint32_t* __restrict__ flat, u64* vmin, int e, int u, int v,
+                                          uint32_t val, bool write_region) {
+  if (write_region) *c = val;
+  flat[e] = val;
+  const u64 key = ((u64)val << 32) | (u64)(uint32_t)e;
+  atomicMin(&vmin[u], key);
+  atomicMin(&vmin[v], key);
+}
+
 __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__ P, const int4* __restrict__ vedge,
                                                        const int* __restrict__ E_ptr, const int* __restrict__ fan_off,
                                                        const int* __restrict__ fsucc, const int4* __restrict__ sface,
@@ -430,7 +443,7 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__
                                                        const long long* __restrict__ voff,
                                                        const long long* __restrict__ eoffm,
                                                        const MeshParam* __restrict__ mp,
-                                                       uint32_t* __restrict__ flat) {
+                                                       uint32_t* __restrict__ flat, u64* vmin) {
   // every lane of a group has the same i, so each return below is taken by
   // the whole group and the width-EC_G shuffles always see all its lanes
   const int i = (blockIdx.x * blockDim.x + threadIdx.x) / EC_G;
@@ -442,15 +455,13 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__
   const long long pmo = mp[k].pmo;
   long long lu = u - voff[k], lv = v - voff[k];
   uint32_t* c = cost + mp[k].eco + (e - eoffm[k]);
-  // the cost lands in the mesh's persistent region and, by global edge
-  // index, in the flat array the claim and collapse read; where the
-  // reference leaves the cost unchanged the flat entry takes the kept value
+  // where the reference leaves the cost unchanged, the kept value is used
   if (table[pmo + lu] && table[pmo + lv]) {
-    if (g == 0) *c = flat[e] = 0xFFFFFFFFu;
+    if (g == 0) ec_finish(c, flat, vmin, e, u, v, 0xFFFFFFFFu, true);
     return;
   }
   if (mp[k].stuck) {
-    if (g == 0) flat[e] = *c;
+    if (g == 0) ec_finish(c, flat, vmin, e, u, v, *c, false);
     return;
   }
   int ua = fan_off[u], ub = fan_off[u + 1], va = fan_off[v], vb = fan_off[v + 1];
@@ -462,11 +473,11 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__
   }
   dup = ec_group_sum(dup);
   if (dup > 2) {
-    if (g == 0) *c = flat[e] = 0xFFFFFFFFu;
+    if (g == 0) ec_finish(c, flat, vmin, e, u, v, 0xFFFFFFFFu, true);
     return;
   }
   if (dup != 2) {
-    if (g == 0) flat[e] = *c;
+    if (g == 0) ec_finish(c, flat, vmin, e, u, v, *c, false);
     return;
   }
   const float scale = mp[k].scale, tau = mp[k].tau;
@@ -480,7 +491,7 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__
   nt = ec_group_sum(nt);
   flip = ec_group_sum(flip);
   if (g != 0) return;
-  if (flip) { *c = flat[e] = 0xFFFFFFFFu; return; }
+  if (flip) { ec_finish(c, flat, vmin, e, u, v, 0xFFFFFFFFu, true); return; }
   el = el / (float)ne / scale * tau;
   el = el + vnorm(vsub(v0, v1)) / scale * tau;
   Qa = 5.0f * Qa;
@@ -495,7 +506,8 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__
   float t3 = q[3] * m.x + q[6] * m.y + q[8] * m.z + q[9] * 1.0f;
   float qe = t0 * m.x + t1 * m.y + t2 * m.z + t3 * 1.0f;
   float cc = qe / (scale * scale);
-  *c = flat[e] = u32sat(clamp_ref(cc + el + Qa / (float)nt * tau, 0.0f, COST_RANGE) / COST_RANGE * U32F);
+  ec_finish(c, flat, vmin, e, u, v,
+            u32sat(clamp_ref(cc + el + Qa / (float)nt * tau, 0.0f, COST_RANGE) / COST_RANGE * U32F), true);
 }
 
 // ---------------------------------------------------------------- claim
@@ -503,45 +515,32 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__
 // over every edge whose endpoint's fan holds the face (cusimp_free.cu:
 // 587-610).  A face is in the fan of x exactly when x is one of its
 // vertices, so that minimum is the minimum over the face's three vertices
-// of the per-vertex minimum over incident edges: two atomics per edge
-// instead of one per fan face, and min is exact in any order.  The collapse
-// reads the face minimum from vmin and the round-start faces directly.
-__global__ void __launch_bounds__(BLOCK) k_vertex_min(const int4* __restrict__ vedge, const int* __restrict__ E_ptr,
-                                                      const uint32_t* __restrict__ cost, u64* vmin) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= *E_ptr) return;
-  int4 ev = vedge[i];
-  u64 key = ((u64)cost[ev.x] << 32) | (u64)(uint32_t)ev.x;
-  atomicMin(&vmin[ev.y], key);
-  atomicMin(&vmin[ev.z], key);
-}
-
-// Winner decision, one lane per edge: an edge wins when it passes the
+// of the per-vertex minimum over incident edges (vmin, two atomics per edge
+// in the cost kernel; min is exact in any order).
+//
+// Winner decision, one lane per vertex: an edge wins when it passes the
 // threshold (cusimp_free.cu:650) and holds the minimum key of every face in
-// both fans (:658-681).  A face's minimum is the least vertex minimum over
-// its round-start corners (the fan slots' sface, written before any
-// collapse of this round).  u and v have non-empty fans (the edge's own
-// face), so vmin[u] == vmin[v] == key is necessary; it rejects almost every
-// edge with two loads issued with the first ones.  Given it, a face holds
-// key as its minimum exactly when no corner has a smaller vertex minimum.
-// Winners' fans are disjoint, so applying them later, in any order, gives
-// the reference's result.
-__global__ void __launch_bounds__(BLOCK) k_winners_b(const int* __restrict__ edges, const int* __restrict__ E_ptr,
-                                                     const uint32_t* __restrict__ cost, const int* __restrict__ vmesh,
+// both fans (:658-681).  u and v have non-empty fans (the edge's own face),
+// so vmin[u] == vmin[v] == key is necessary: the only candidate of a vertex
+// is the edge its minimum names, taken at its first endpoint.  Given that, a
+// face holds key as its minimum exactly when no corner of it (the fan slots'
+// round-start sface) has a smaller vertex minimum.  win[] was cleared by the
+// edge emit.  Winners' fans are disjoint, so applying them later, in any
+// order, gives the reference's result.
+__global__ void __launch_bounds__(BLOCK) k_winners_v(int V, const u64* __restrict__ vmin,
+                                                     const int* __restrict__ edges, const int* __restrict__ vmesh,
                                                      const MeshParam* __restrict__ mp,
-                                                     const int* __restrict__ fan_off, const u64* __restrict__ vmin,
+                                                     const int* __restrict__ fan_off,
                                                      const int4* __restrict__ sface, int* __restrict__ win) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  int E = *E_ptr;
-  if (e > E) return;
-  if (e == E) { win[E] = 0; return; }
-  uint32_t c = cost[e];
-  int u = edges[2 * e], v = edges[2 * e + 1];
-  const u64 key = ((u64)c << 32) | (u64)(uint32_t)e;
-  const u64 mu = vmin[u], mv = vmin[v];
-  const int k = vmesh[u];
-  bool ok = mu == key && mv == key;
-  ok = ok && c <= mp[k].ct;
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= V) return;
+  const u64 key = vmin[u];
+  if (key == ~0ull) return;   // no incident edge
+  const int e = (int)(uint32_t)key;
+  if (edges[2 * e] != u) return;
+  const int v = edges[2 * e + 1];
+  if (vmin[v] != key || (uint32_t)(key >> 32) > mp[vmesh[u]].ct) return;
+  bool ok = true;
   for (int w = 0; w < 2 && ok; w++) {
     int x = w == 0 ? u : v;
     for (int s = fan_off[x]; s < fan_off[x + 1] && ok; s++) {
@@ -549,7 +548,7 @@ __global__ void __launch_bounds__(BLOCK) k_winners_b(const int* __restrict__ edg
       ok = vmin[t.x] >= key && vmin[t.y] >= key && vmin[t.z] >= key;
     }
   }
-  win[e] = ok ? 1 : 0;
+  if (ok) win[e] = 1;
 }
 // The winners from the dense list (ascending edge order), one group of
 // CL_G lanes each: the entry in the mesh's collapsed region, then the
