@@ -6,16 +6,25 @@ ratio 0.001, tolerance 4, threshold 1e-3, min_verts 0) of a seeded 2M-face
 SDF sphere-union mesh whose vertices and faces are already resident in HBM.
 value = input faces / seconds per step / 1e6.
 
-Also reported on the same line:
-  roofline      the edge-cost kernel's algorithmic bytes per launch over its
-                average launch time (HIP events around every launch in the
-                timed steps), against the 8 TB/s HBM peak;
-  cpu_baseline  the CPU oracle (the reference's stage-2 logic restated in
-                C++/OpenMP) on the host cores, timed on the first rounds of
-                the same workload and scaled by the work left (see sample);
-  verify        the exact FP64 check of the output (manifold, watertight,
-                self-intersections) and the per-round face counts' agreement
-                with the oracle on the rounds it ran.
+Workloads (--workload): sdf2m (default, 2.0M faces), sdf7m (7.05M faces
+with 64 thin slab pairs: the guard-heavy case), ico20k (20,480-face
+displaced icosphere, ratio 0.01), batch512 (512 meshes in one packed
+engine) and serve (Poisson arrivals into the continuous-batching server).
+
+Also reported on the same line (plain workloads):
+  roofline        the edge-cost kernel's algorithmic bytes per launch over
+                  its average launch time (HIP events around every launch in
+                  the timed steps), against the 8 TB/s HBM peak;
+  kernel_classes  the same for the quadric (fan slots), edge cost, claim and
+                  LBVH build classes, and the traversal's time, from one
+                  extra untimed run with every class timed;
+  cpu_baseline    the CPU oracle (the reference's stage-2 logic restated in
+                  C++/OpenMP) on the host cores, timed on the first rounds of
+                  the same workload and scaled by the work left (see sample),
+                  and the same on one thread (one_thread);
+  verify          the exact FP64 check of the output (manifold, watertight,
+                  self-intersections); rounds_match_gpu is the per-round face
+                  counts' agreement with the oracle on the rounds it ran.
 """
 import argparse
 import json
@@ -38,12 +47,15 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", default="sdf2m", choices=["sdf2m", "ico20k", "batch512", "serve"])
+    p.add_argument("--workload", default="sdf2m", choices=["sdf2m", "sdf7m", "ico20k", "batch512", "serve"])
     p.add_argument("--meshes", type=int, default=512, help="batch512: number of meshes")
     p.add_argument("--jobs", type=int, default=1000, help="serve: number of jobs")
-    p.add_argument("--rate", type=float, default=200.0, help="serve: Poisson arrival rate (jobs/s)")
+    p.add_argument("--rate", default="200", help="serve: Poisson arrival rate(s) in jobs/s, comma-separated")
     p.add_argument("--seq-sample", type=int, default=8, help="batch/serve: plain-path timing on every n-th mesh")
-    p.add_argument("--grid", type=int, default=300, help="lattice for sdf2m (300 gives ~2.0M faces)")
+    p.add_argument("--grid", type=int, default=0,
+                   help="SDF lattice (default: 300 for sdf2m, ~2.0M faces; 584 for sdf7m, ~7.05M faces)")
+    p.add_argument("--cpu1-seconds", type=float, default=10.0,
+                   help="budget of the single-thread CPU sample (0: skip)")
     p.add_argument("--cpu-seconds", type=float, default=20.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-verify", action="store_true")
@@ -58,12 +70,21 @@ def make_mesh(args):
     from pamo_amd import synth
     t = time.time()
     if args.workload == "sdf2m":
-        v, f = synth.sdf_union(200, args.grid, seed=2, k=0.02)
+        grid = args.grid or 300
+        v, f = synth.sdf_union(200, grid, seed=2, k=0.02)
         cfg = dict(workload="sdf_union_2M", generator="smooth-min union of 200 spheres (k=0.02), seed 2",
-                   grid=args.grid)
+                   grid=grid, ratio=0.001)
+    elif args.workload == "sdf7m":
+        grid = args.grid or 584
+        v, f = synth.sdf_union(600, grid, seed=3, k=0.02, n_slab_pairs=64)
+        cfg = dict(workload="sdf_union_slabs_7M",
+                   generator="smooth-min union of 600 spheres (k=0.02) and 64 thin slab pairs "
+                             "(thickness U(1e-3,5e-3), gap about one lattice step), seed 3",
+                   grid=grid, ratio=0.001)
     else:
         v, f = synth.displaced_icosphere(5, 1)
-        cfg = dict(workload="displaced_icosphere_20k", generator="icosphere L5 + 16 Gaussian bumps, seed 1")
+        cfg = dict(workload="displaced_icosphere_20k", generator="icosphere L5 + 16 Gaussian bumps, seed 1",
+                   ratio=0.01)
     log("mesh %s: %d verts %d faces (%.1fs)" % (cfg["workload"], v.shape[0], f.shape[0], time.time() - t))
     return v, f, cfg
 
@@ -90,7 +111,8 @@ def run_plain(args):
     v, f, cfg = make_mesh(args)
     F0 = int(f.shape[0])
     verts, scale, tris_mean = preprocess_numpy(v, f)
-    target = int(0.001 * F0)
+    ratio = cfg.pop("ratio")
+    target = int(ratio * F0)
     dev = torch.device("cuda:0")
     dv = torch.from_numpy(verts).to(dev)
     df = torch.from_numpy(f).to(dev)
@@ -121,6 +143,7 @@ def run_plain(args):
         log("step %d: %.1f ms, rounds %d, final faces %d" % (s, 1e3 * (t1 - t0), st["rounds"], of.shape[0]))
     prof = ctx.profile_read(dom)
     ctx.profile([])
+    classes = _class_profile(ctx, lambda: (simplify_stage2(dv, df, scale, target, ctx=ctx), torch.cuda.synchronize()))
     sec = float(np.mean(times))
     value = F0 / sec / 1e6
     avg_ms = prof["ms"] / max(prof["launches"], 1)
@@ -139,42 +162,75 @@ def run_plain(args):
     out_v = ov.cpu().numpy()
     out_f = of.cpu().numpy()
     round_faces = [int(x) for x in st["round_faces"]]
-    result = dict(metric="input Mfaces/s to 0.1% (plain, one mesh)", value=round(value, 3), unit="Mfaces/s",
+    result = dict(metric="input Mfaces/s to %g%% (plain, one mesh)" % (100 * ratio), value=round(value, 3),
+                  unit="Mfaces/s",
                   n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=round(1e3 * sec, 3),
                   higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32",
                   data="synthetic (seeded SDF mesh generated in-process)",
-                  config=dict(cfg, faces=F0, vertices=int(v.shape[0]), ratio=0.001, tolerance=4, threshold=1e-3,
+                  config=dict(cfg, faces=F0, vertices=int(v.shape[0]), ratio=ratio, tolerance=4, threshold=1e-3,
                               min_verts=0, target_faces=target, rounds=st["rounds"], final_faces=int(out_f.shape[0]),
                               exit_reason=st["exit_reason"], parallelism="single GPU",
                               guard_checks=st["checks"], query_faces=st["query_faces"],
                               candidate_pairs=st["candidate_pairs"]),
                   latency_ms=dict(p50=round(1e3 * float(np.percentile(times, 50)), 3),
                                   p99=round(1e3 * float(np.percentile(times, 99)), 3)),
-                  roofline=roofline)
+                  roofline=roofline, kernel_classes=classes)
     if not args.no_cpu:
-        import oracle
         ncores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
-        ncores = oracle.set_threads(ncores)
-        log("cpu oracle: %d threads, budget %.0fs" % (ncores, args.cpu_seconds))
-        cst, crf, crs = oracle.run_timed(verts, f, scale, target, args.cpu_seconds)
-        n = cst["rounds"]
-        match = [int(x) for x in crf] == round_faces[:n]
-        # work of a round is taken proportional to the faces entering it
-        entering = [F0] + round_faces[:-1]
-        frac = sum(entering[:n]) / float(sum(entering))
-        est_total = crs[n - 1] / frac if n else float("nan")
-        result["cpu_baseline"] = dict(value=round(F0 / est_total / 1e6, 4), unit="Mfaces/s", cores=ncores,
-                                      kind="port",
-                                      sample="oracle ran the first %d of %d rounds (%.1f%% of the face-rounds) in "
-                                             "%.1fs; total time scaled by face-rounds" %
-                                             (n, len(round_faces), 100 * frac, crs[n - 1] if n else 0.0),
-                                      rounds_match_gpu=bool(match))
+        result["cpu_baseline"] = _cpu_plain(verts, f, scale, target, round_faces, ncores, args.cpu_seconds)
+        if args.cpu1_seconds > 0:
+            one = _cpu_plain(verts, f, scale, target, round_faces, 1, args.cpu1_seconds)
+            result["cpu_baseline"]["one_thread"] = {k: one[k] for k in ("value", "unit", "cores", "sample",
+                                                                        "rounds_match_gpu")}
     if not args.no_verify:
         from pamo_amd.verify import verify
         r = verify(out_v, out_f)
         result["verify"] = {k: r[k] for k in ("manifold", "watertight", "degenerate_faces", "intersecting_pairs",
                                                "ok")}
     print(json.dumps(result), flush=True)
+
+
+def _cpu_plain(verts, f, scale, target, round_faces, threads, seconds):
+    """The oracle's loop on the host with the given threads, stopped after
+    the first round that ends past the budget; the full-run time is the
+    sample's time scaled by the fraction of face-rounds it covered."""
+    import oracle
+    threads = oracle.set_threads(threads)
+    log("cpu oracle: %d threads, budget %.0fs" % (threads, seconds))
+    F0 = int(f.shape[0])
+    cst, crf, crs = oracle.run_timed(verts, f, scale, target, seconds)
+    n = cst["rounds"]
+    match = [int(x) for x in crf] == round_faces[:n]
+    # work of a round is taken proportional to the faces entering it
+    entering = [F0] + round_faces[:-1]
+    frac = sum(entering[:n]) / float(sum(entering))
+    est_total = crs[n - 1] / frac if n else float("nan")
+    return dict(value=round(F0 / est_total / 1e6, 4), unit="Mfaces/s", cores=threads, kind="port",
+                sample="oracle ran the first %d of %d rounds (%.1f%% of the face-rounds) in %.1fs; total time "
+                       "scaled by face-rounds" % (n, len(round_faces), 100 * frac, crs[n - 1] if n else 0.0),
+                rounds_match_gpu=bool(match))
+
+
+def _class_profile(ctx, run):
+    """One extra, untimed run with every kernel class timed by HIP events on
+    the launch stream: achieved algorithmic GB/s per class (the traversal's
+    bytes are not defined; it reports time only)."""
+    from pamo_amd import _lib
+    ks = [_lib.K_QUADRIC, _lib.K_EDGE_COST, _lib.K_CLAIM, _lib.K_LBVH, _lib.K_TRAVERSE]
+    ctx.profile(ks)
+    run()
+    out = {}
+    for k in ks:
+        p = ctx.profile_read(k)
+        n = max(p["launches"], 1)
+        avg_s = p["ms"] / n / 1e3
+        gbs = (p["bytes"] / n) / avg_s / 1e9 if avg_s > 0 and p["bytes"] > 0 else None
+        out[_lib.KERNEL_NAMES[k]] = dict(launches=p["launches"], total_ms=round(p["ms"], 3),
+                                         avg_us=round(1e6 * avg_s, 3),
+                                         achieved_gbs=round(gbs, 2) if gbs else None,
+                                         frac=round(gbs / HBM_PEAK_GBS, 4) if gbs else None)
+    ctx.profile([])
+    return out
 
 
 # ------------------------------------------------------------ many meshes
@@ -314,24 +370,17 @@ def run_batch(args):
 
 
 def run_serve(args):
+    """Poisson arrivals of the many-mesh mix into the continuous-batching
+    server, one JSON line per arrival rate (--rate takes a comma list); the
+    meshes and the plain-path service-time sample are shared by the rates."""
     import numpy as np
-    import torch
     from pamo_amd import _lib, batch, synth
+    rates = [float(x) for x in str(args.rate).split(",") if x]
     specs = synth.batch_specs(args.jobs, seed_base=5_000_000)
     meshes = _gen(specs)
-    rng = np.random.default_rng(5)
-    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, size=len(meshes)))
-    jobs = [dict(points=v, faces=f, ratio=r, arrival=float(a)) for (v, f, r), a in zip(meshes, arrivals)]
     ctx = _lib.Context(0)
     sample = list(range(0, len(meshes), max(1, args.seq_sample)))
     _plain_times(meshes, sample[:2], ctx)   # warm-up
-    eng = batch.BatchEngine(ctx=ctx)
-    t0 = time.perf_counter()
-    recs = batch.Server(engine=eng).run(jobs)
-    wall = time.perf_counter() - t0
-    lat = np.array([r["latency"] for r in recs])
-    F_total = sum(len(m[1]) for m in meshes)
-    span = max(r["finished"] for r in recs) - float(arrivals[0])
     # plain path: measured service time of every seq_sample-th job, a linear
     # fit in input faces for the rest, and a FIFO single-server queue over the
     # same arrivals
@@ -342,29 +391,52 @@ def run_serve(args):
     svc = np.array([max(a + b * len(m[1]), ys.min()) for m in meshes])
     for i in sample:
         svc[i] = seq[i][0]
-    done = 0.0
-    seq_lat = []
-    for t_arr, s_i in zip(arrivals, svc):
-        start = max(done, t_arr)
-        done = start + s_i
-        seq_lat.append(done - t_arr)
-    seq_lat = np.array(seq_lat)
-    result = dict(metric="served input Mfaces/s at %g jobs/s (Poisson)" % args.rate,
-                  value=round(F_total / span / 1e6, 3), unit="Mfaces/s", n_gpus=1, steps=1, warmup=1,
-                  ms_per_step=round(1e3 * wall, 3), higher_is_better=True, scaling="weak", vs_baseline=None,
-                  dtype="f32", data="synthetic (seeded generators, in-process)",
-                  config=dict(workload="serve", jobs=len(jobs), faces=F_total, rate_jobs_s=args.rate,
-                              arrival_seed=5, seeds="5000000+i", hbm_budget_gb=200,
-                              parallelism="single GPU, continuous batching"),
-                  latency_ms=dict(p50=round(1e3 * float(np.percentile(lat, 50)), 2),
-                                  p99=round(1e3 * float(np.percentile(lat, 99)), 2)),
-                  sequential_plain=dict(latency_ms=dict(p50=round(1e3 * float(np.percentile(seq_lat, 50)), 2),
-                                                        p99=round(1e3 * float(np.percentile(seq_lat, 99)), 2)),
-                                        measured_jobs=len(sample),
-                                        model="measured service times on the sample, linear fit in faces "
-                                              "for the rest, FIFO queue over the same arrivals"),
-                  jobs_per_s=round(len(jobs) / span, 2))
-    print(json.dumps(result), flush=True)
+    F_total = sum(len(m[1]) for m in meshes)
+    # the plain path's capacity: the whole stream served one mesh at a time
+    plain_mfs = F_total / float(svc.sum()) / 1e6
+    plain_jps = len(meshes) / float(svc.sum())
+    for rate in rates:
+        rng = np.random.default_rng(5)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(meshes)))
+        jobs = [dict(points=v, faces=f, ratio=r, arrival=float(t)) for (v, f, r), t in zip(meshes, arrivals)]
+        eng = batch.BatchEngine(ctx=ctx)
+        srv = batch.Server(engine=eng)
+        t0 = time.perf_counter()
+        recs = srv.run(jobs)
+        wall = time.perf_counter() - t0
+        del eng
+        lat = np.array([r["latency"] for r in recs])
+        span = max(r["finished"] for r in recs) - float(arrivals[0])
+        done = 0.0
+        seq_lat = []
+        for t_arr, s_i in zip(arrivals, svc):
+            start = max(done, t_arr)
+            done = start + s_i
+            seq_lat.append(done - t_arr)
+        seq_lat = np.array(seq_lat)
+        log("serve %g jobs/s: p50 %.0f ms, p99 %.0f ms, %.1f jobs/s" %
+            (rate, 1e3 * np.percentile(lat, 50), 1e3 * np.percentile(lat, 99), len(jobs) / span))
+        result = dict(metric="served input Mfaces/s at %g jobs/s (Poisson)" % rate,
+                      value=round(F_total / span / 1e6, 3), unit="Mfaces/s", n_gpus=1, steps=1, warmup=1,
+                      ms_per_step=round(1e3 * wall, 3), higher_is_better=True, scaling="weak", vs_baseline=None,
+                      dtype="f32", data="synthetic (seeded generators, in-process)",
+                      config=dict(workload="serve", jobs=len(jobs), faces=F_total, rate_jobs_s=rate,
+                                  arrival_seed=5, seeds="5000000+i", hbm_budget_gb=200,
+                                  parallelism="single GPU, continuous batching"),
+                      latency_ms=dict(p50=round(1e3 * float(np.percentile(lat, 50)), 2),
+                                      p99=round(1e3 * float(np.percentile(lat, 99)), 2),
+                                      max=round(1e3 * float(lat.max()), 2)),
+                      sequential_plain=dict(latency_ms=dict(p50=round(1e3 * float(np.percentile(seq_lat, 50)), 2),
+                                                            p99=round(1e3 * float(np.percentile(seq_lat, 99)), 2)),
+                                            capacity_mfaces_s=round(plain_mfs, 3),
+                                            capacity_jobs_s=round(plain_jps, 2),
+                                            measured_jobs=len(sample),
+                                            model="measured service times on the sample, linear fit in faces "
+                                                  "for the rest, FIFO queue over the same arrivals"),
+                      jobs_per_s=round(len(jobs) / span, 2),
+                      served_over_plain=round((F_total / span / 1e6) / plain_mfs, 2),
+                      server_host_s={k: (round(v, 3) if isinstance(v, float) else v) for k, v in srv.stats.items()})
+        print(json.dumps(result), flush=True)
 
 
 if __name__ == "__main__":

@@ -109,6 +109,17 @@ int pamo_run_fetch(pamo_ctx* ctx, void* stream, float* d_verts_out, int* d_faces
  * reference, errors are printed to stderr; the sizes are then set to -1. */
 void pamo_simplify(float* in_verts, int* in_tris, float* out_verts, int* out_tris, int* in_out_nv_ni, float threshold);
 
+/* The centring and scale of PaMO.run (simp_cuda/pamo/__init__.py:60-64,117,
+ * 126) on host arrays, with numpy's float32 arithmetic: tris_mean = the
+ * sequential float32 sum from +0 over faces of (((0 + a) + b) + c) / 3,
+ * divided by F (the order numpy's mean(axis=1).mean(axis=0) takes),
+ * verts_out = points -
+ * tris_mean, *scale = the largest extent of verts_out.  Host only (no GPU
+ * call); face indices must lie in [0, V).  Returns 0, or -1 for F <= 0 or
+ * V <= 0. */
+int pamo_prepare_host(const float* points, const int* faces, int V, int F, float* verts_out, float* tris_mean,
+                      float* scale);
+
 /* Batched stage 2 (no counterpart in the reference, which simplifies one mesh
  * per PaMO.run call, simp_cuda/pamo/__init__.py:107-165).  Many meshes share
  * one packed device state; every round of pamo_batch_step runs the loop body

@@ -15,7 +15,7 @@ import torch
 from . import _C, _lib
 from ._lib import PamoError
 
-__all__ = ["PaMO", "PaSP", "PamoError", "simplify_stage2", "preprocess_numpy"]
+__all__ = ["PaMO", "PaSP", "PamoError", "simplify_stage2", "preprocess_numpy", "prepare_host"]
 
 
 def _as_vf(input_mesh):
@@ -39,6 +39,27 @@ def preprocess_numpy(points, faces):
     verts = (points - tris_mean).astype(np.float32)
     ext = verts.max(axis=0) - verts.min(axis=0)
     return verts, float(max(max(ext[0], ext[1]), ext[2])), tris_mean
+
+
+def prepare_host(points, faces):
+    """PaMO.run's centring and scale (simp_cuda/pamo/__init__.py:60-64,117,
+    126) on the host arrays, in the library (pamo_prepare_host: numpy's
+    float32 summation order, one pass, no temporaries; the call releases the
+    GIL, so worker threads run it in parallel).  Returns (centred float32
+    vertices, int32 faces, scale, tris_mean)."""
+    pts = np.ascontiguousarray(points, np.float32)
+    tri = np.ascontiguousarray(faces, np.int32)
+    if len(tri) == 0 or len(pts) == 0 or tri.min() < 0 or tri.max() >= len(pts):
+        verts, scale, tris_mean = preprocess_numpy(pts, tri)   # numpy raises on bad indices
+        return verts, tri, scale, tris_mean
+    verts = np.empty_like(pts)
+    mean = np.empty(3, np.float32)
+    scale = np.empty(1, np.float32)
+    rc = _lib.lib().pamo_prepare_host(pts.ctypes.data, tri.ctypes.data, len(pts), len(tri), verts.ctypes.data,
+                                       mean.ctypes.data, scale.ctypes.data)
+    if rc != 0:
+        raise RuntimeError("pamo_prepare_host failed")
+    return verts, tri, float(scale[0]), mean
 
 
 def _scale(verts):
@@ -112,11 +133,12 @@ class PaMO(torch.nn.Module):
         self.target_faces = max(int(ratio * len(triangles)), min_verts)
         if self.verbose:
             print("Target faces : {}".format(self.target_faces))
-        tris_mean = _preprocess(points, triangles)
-        verts = points - torch.from_numpy(tris_mean).to(points.device)
+        # the reference's centring and scale (its torch/numpy float32
+        # expressions, evaluated in one host pass with the same results)
+        hv, _, scale, tris_mean = prepare_host(points.cpu().numpy(), triangles.cpu().numpy())
+        verts = torch.from_numpy(hv).to(points.device)
         faces = triangles
         start = time.time()
-        scale = _scale(verts)
         if self.engine == "device":
             verts, faces, stats = simplify_stage2(verts, faces, scale, self.target_faces, threshold, tolerance,
                                                   iter, ctx=self.pamo._ctx)

@@ -72,6 +72,8 @@ def lib():
     L.pamo_debug_guard.restype = I
     L.pamo_simplify.argtypes = [P, P, P, P, P, F]
     L.pamo_simplify.restype = None
+    L.pamo_prepare_host.argtypes = [P, P, I, I, P, P, P]
+    L.pamo_prepare_host.restype = I
     _lib = L
     return L
 

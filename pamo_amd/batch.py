@@ -18,7 +18,9 @@ import time
 import numpy as np
 import torch
 
-from . import _lib, _preprocess, _scale
+from concurrent.futures import ThreadPoolExecutor
+
+from . import _lib, prepare_host
 
 # device bytes per face of the packed state (positions, faces and their
 # round-start copies, fans, edges, costs, claims, LBVH nodes, scan scratch)
@@ -104,12 +106,9 @@ class BatchEngine:
 
 
 def prepare(points, faces, device):
-    """PaMO.run's centring and scale (simp_cuda/pamo/__init__.py:60-64,117,126)."""
-    pts = torch.as_tensor(np.asarray(points, np.float32)).to(device)
-    tri = torch.as_tensor(np.asarray(faces, np.int32)).to(device)
-    tris_mean = _preprocess(pts, tri)
-    verts = pts - torch.from_numpy(tris_mean).to(device)
-    return verts, tri, _scale(verts), tris_mean
+    """prepare_host, with the vertices and faces moved to the device."""
+    verts, tri, scale, tris_mean = prepare_host(points, faces)
+    return torch.from_numpy(verts).to(device), torch.from_numpy(tri).to(device), scale, tris_mean
 
 
 def simplify_batch(meshes, ratios, threshold=1e-3, tolerance=4, min_verts=0, engine=None, rounds_per_step=1):
@@ -146,12 +145,15 @@ class Server:
     HBM budget; every loop iteration runs one round for all admitted jobs.
     Latency = completion time - arrival time (both on the host clock)."""
 
-    def __init__(self, budget_bytes=200 << 30, threshold=1e-3, tolerance=4, min_verts=0, engine=None):
+    def __init__(self, budget_bytes=200 << 30, threshold=1e-3, tolerance=4, min_verts=0, engine=None,
+                 prep_threads=4):
         self.eng = engine or BatchEngine()
         self.budget = budget_bytes
         self.threshold = threshold
         self.tolerance = tolerance
         self.min_verts = min_verts
+        self.prep_threads = prep_threads
+        self.stats = {}
 
     def run(self, jobs, realtime=True, on_result=None):
         """jobs: list of dicts {points, faces, ratio, arrival} (arrival in
@@ -163,39 +165,69 @@ class Server:
         live = {}
         used = 0
         recs = [None] * len(jobs)
+        # host preparation (centring and scale) runs on worker threads from
+        # the job's arrival on, overlapping the rounds already on the device
+        pool = ThreadPoolExecutor(max_workers=max(1, self.prep_threads))
+        prepared = {}
+        # host seconds per part of the loop, rounds run, peak admitted jobs
+        st = dict(step_s=0.0, submit_s=0.0, prep_wait_s=0.0, take_s=0.0, idle_s=0.0, rounds=0, max_live=0)
+        self.stats = st
         t0 = time.perf_counter()
         done = 0
-        while done < len(jobs):
-            now = time.perf_counter() - t0
-            while nxt < len(order) and (jobs[order[nxt]]["arrival"] <= now or not realtime):
-                waiting.append(order[nxt])
-                nxt += 1
-            while waiting:
-                i = waiting[0]
-                need = BYTES_PER_FACE * len(jobs[i]["faces"])
-                if live and used + need > self.budget:
-                    break
-                waiting.pop(0)
-                j = jobs[i]
-                v, t, s, m = prepare(j["points"], j["faces"], dev)
-                target = max(int(j["ratio"] * len(j["faces"])), self.min_verts)
-                h = self.eng.submit(v, t, s, target, self.threshold, self.tolerance, job=i)
-                live[h] = (i, m, need, time.perf_counter() - t0)
-                used += need
-            if not live:
-                if nxt < len(order):
-                    time.sleep(max(0.0, min(0.001, jobs[order[nxt]]["arrival"] - now)))
-                continue
-            self.eng.step(1)
-            for h, _ in self.eng.poll():
-                v, f, info = self.eng.take(h)
-                i, m, need, t_adm = live.pop(h)
-                used -= need
-                t_end = time.perf_counter() - t0
-                recs[i] = dict(job=i, arrival=jobs[i]["arrival"], admitted=t_adm, finished=t_end,
-                               latency=t_end - jobs[i]["arrival"], faces_in=len(jobs[i]["faces"]), faces_out=info["F"],
-                               rounds=info["rounds"])
-                if on_result is not None:
-                    on_result(i, v.cpu().numpy() + m, f.cpu().numpy(), info)
-                done += 1
+        try:
+            while done < len(jobs):
+                now = time.perf_counter() - t0
+                while nxt < len(order) and (jobs[order[nxt]]["arrival"] <= now or not realtime):
+                    i = order[nxt]
+                    waiting.append(i)
+                    prepared[i] = pool.submit(prepare_host, jobs[i]["points"], jobs[i]["faces"])
+                    nxt += 1
+                while waiting:
+                    i = waiting[0]
+                    need = BYTES_PER_FACE * len(jobs[i]["faces"])
+                    if live and used + need > self.budget:
+                        break
+                    # admit in arrival order once prepared; while the device
+                    # has work, rounds go on instead of waiting for the host
+                    if live and not prepared[i].done():
+                        break
+                    waiting.pop(0)
+                    ta = time.perf_counter()
+                    hv, ht, s, m = prepared.pop(i).result()
+                    tb = time.perf_counter()
+                    target = max(int(jobs[i]["ratio"] * len(jobs[i]["faces"])), self.min_verts)
+                    h = self.eng.submit(torch.from_numpy(hv).to(dev), torch.from_numpy(ht).to(dev), s, target,
+                                        self.threshold, self.tolerance, job=i)
+                    tc = time.perf_counter()
+                    st["prep_wait_s"] += tb - ta
+                    st["submit_s"] += tc - tb
+                    live[h] = (i, m, need, tc - t0)
+                    used += need
+                if not live:
+                    if nxt < len(order):
+                        ta = time.perf_counter()
+                        time.sleep(max(0.0, min(0.001, jobs[order[nxt]]["arrival"] - now)))
+                        st["idle_s"] += time.perf_counter() - ta
+                    continue
+                st["max_live"] = max(st["max_live"], len(live))
+                ta = time.perf_counter()
+                self.eng.step(1)
+                tb = time.perf_counter()
+                st["step_s"] += tb - ta
+                st["rounds"] += 1
+                for h, _ in self.eng.poll():
+                    ta = time.perf_counter()
+                    v, f, info = self.eng.take(h)
+                    st["take_s"] += time.perf_counter() - ta
+                    i, m, need, t_adm = live.pop(h)
+                    used -= need
+                    t_end = time.perf_counter() - t0
+                    recs[i] = dict(job=i, arrival=jobs[i]["arrival"], admitted=t_adm, finished=t_end,
+                                   latency=t_end - jobs[i]["arrival"], faces_in=len(jobs[i]["faces"]),
+                                   faces_out=info["F"], rounds=info["rounds"])
+                    if on_result is not None:
+                        on_result(i, v.cpu().numpy() + m, f.cpu().numpy(), info)
+                    done += 1
+        finally:
+            pool.shutdown(wait=False, cancel_futures=True)
         return recs

@@ -1703,5 +1703,28 @@ __global__ void k_retire_undo2(const int* __restrict__ undo_in, const long long*
   out[newuo[k] + (i - uioff[k])] = undo_in[i];
 }
 
+// Persistent regions of the kept meshes moved down to their new offsets
+// (kept mesh j: source offset src[j], destination dst[j], dst[nk] = total),
+// so the regions take space only for active meshes.  grid.y: 0 = the vertex
+// maps and invalid-vertex tables (same offsets), 1 = the edge costs.
+__global__ void k_move_regions(int nk, const long long* __restrict__ pm_src, const long long* __restrict__ pm_dst,
+                               const long long* __restrict__ ec_src, const long long* __restrict__ ec_dst,
+                               const int* __restrict__ pm, const uint8_t* __restrict__ table,
+                               const uint32_t* __restrict__ ecost, int* __restrict__ pm2, uint8_t* __restrict__ table2,
+                               uint32_t* __restrict__ ecost2) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.y == 0) {
+    if (i >= pm_dst[nk]) return;
+    int j = find_seg(pm_dst, nk, i);
+    long long from = pm_src[j] + (i - pm_dst[j]);
+    pm2[i] = pm[from];
+    table2[i] = table[from];
+  } else {
+    if (i >= ec_dst[nk]) return;
+    int j = find_seg(ec_dst, nk, i);
+    ecost2[i] = ecost[ec_src[j] + (i - ec_dst[j])];
+  }
+}
+
 }  // namespace eng
 }  // namespace pamo

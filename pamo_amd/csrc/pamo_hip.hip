@@ -740,6 +740,39 @@ static int pamo_simplify_run(pamo_ctx* ctx, const float* in_verts, const int* in
   return 0;
 }
 
+int pamo_prepare_host(const float* points, const int* faces, int V, int F, float* verts_out, float* tris_mean,
+                      float* scale) {
+  if (F <= 0 || V <= 0) return -1;
+  // numpy: tris.mean(axis=1) is (((0 + a) + b) + c) / 3 per face and
+  // coordinate; .mean(axis=0) adds the rows in order to 0, then divides by F
+  // (add.reduce starts from its identity +0, which only matters for the sign
+  // of a zero; numpy divides in double and rounds, for F < 2^24 the same as
+  // a float division)
+  float acc[3] = {0.0f, 0.0f, 0.0f};
+  for (int f = 0; f < F; f++) {
+    const float* a = points + 3LL * faces[3LL * f];
+    const float* b = points + 3LL * faces[3LL * f + 1];
+    const float* d = points + 3LL * faces[3LL * f + 2];
+    for (int c = 0; c < 3; c++) acc[c] += (float)((double)(((0.0f + a[c]) + b[c]) + d[c]) / 3.0);
+  }
+  for (int c = 0; c < 3; c++) tris_mean[c] = (float)((double)acc[c] / (double)F);
+  float lo[3], hi[3];
+  for (long long i = 0; i < 3LL * V; i++) {
+    const int c = (int)(i % 3);
+    const float x = points[i] - tris_mean[c];
+    verts_out[i] = x;
+    if (i < 3) { lo[c] = hi[c] = x; continue; }
+    // numpy's max and min propagate NaN
+    if (x != x || x < lo[c]) lo[c] = (lo[c] != lo[c]) ? lo[c] : x;
+    if (x != x || x > hi[c]) hi[c] = (hi[c] != hi[c]) ? hi[c] : x;
+  }
+  const float e0 = hi[0] - lo[0], e1 = hi[1] - lo[1], e2 = hi[2] - lo[2];
+  // Python's max(max(e0, e1), e2)
+  float m01 = e1 > e0 ? e1 : e0;
+  *scale = e2 > m01 ? e2 : m01;
+  return 0;
+}
+
 void pamo_simplify(float* in_verts, int* in_tris, float* out_verts, int* out_tris, int* in_out_nv_ni, float threshold) {
   static std::mutex mu;
   static pamo_ctx* ctx = nullptr;

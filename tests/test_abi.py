@@ -78,3 +78,38 @@ def test_preprocess_matches_reference_expression():
     vt = torch.from_numpy(v) - torch.from_numpy(tm2)
     assert vt.numpy().tobytes() == verts.tobytes()
     assert _scale(vt) == scale
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_prepare_host_matches_numpy(seed):
+    """pamo_prepare_host (one pass in the library) equals the numpy
+    expressions of the centring and scale bit for bit: random meshes of
+    every size class with offsets and scales far from 1, and ragged index
+    use (unreferenced vertices, repeated corners)."""
+    import numpy as np
+    from pamo_amd import batch, preprocess_numpy
+    rng = np.random.default_rng(seed)
+    for _ in range(6):
+        V = int(rng.integers(1, 30000))
+        F = int(rng.integers(1, 60000))
+        p = (rng.standard_normal((V, 3)) * rng.uniform(1e-4, 1e4) + rng.uniform(-1e3, 1e3, 3)).astype(np.float32)
+        f = rng.integers(0, V, (F, 3)).astype(np.int32)
+        v1, t1, s1, m1 = batch.prepare_host(p, f)
+        v2, s2, m2 = preprocess_numpy(p, f)
+        assert v1.tobytes() == v2.tobytes()
+        assert m1.tobytes() == m2.astype(np.float32).tobytes()
+        assert s1 == s2
+        assert t1.dtype == np.int32 and np.array_equal(t1, f)
+
+
+def test_prepare_host_signed_zero_and_single_face():
+    """Both sums start from +0 as numpy's add.reduce does (-0.0 terms add
+    up to +0.0)."""
+    import numpy as np
+    from pamo_amd import batch, preprocess_numpy
+    p = np.array([[-0.0, 1.0, 2.0], [-0.0, 3.0, 4.0], [-0.0, 5.0, 7.0]], np.float32)
+    f = np.array([[0, 1, 2]], np.int32)
+    v1, _, s1, m1 = batch.prepare_host(p, f)
+    v2, s2, m2 = preprocess_numpy(p, f)
+    assert m1.tobytes() == m2.astype(np.float32).tobytes()
+    assert v1.tobytes() == v2.tobytes() and s1 == s2
