@@ -22,6 +22,9 @@ Also reported on the same line (plain workloads):
                   C++/OpenMP) on the host cores, timed on the first rounds of
                   the same workload and scaled by the work left (see sample),
                   and the same on one thread (one_thread);
+  fp64_error      the first round's edge costs and collapse positions in FP32
+                  (the reference's arithmetic) against the same expressions
+                  in FP64: relative cost error, changed decisions (winners);
   verify          the exact FP64 check of the output (manifold, watertight,
                   self-intersections); rounds_match_gpu is the per-round face
                   counts' agreement with the oracle on the rounds it ran.
@@ -59,6 +62,11 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=20.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-verify", action="store_true")
+    p.add_argument("--save-mesh", default="",
+                   help="plain workloads: write the output mesh (.obj or .ply) here, in the centred frame the "
+                        "guard works in")
+    p.add_argument("--no-classes", action="store_true",
+                   help="skip the extra untimed run that times every kernel class (for kernel traces of one run)")
     p.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic.json"))
     p.add_argument("--copy-probe", type=int, default=0,
                    help="MiB copied device-to-device once before timing: a copy kernel of known size that "
@@ -114,9 +122,13 @@ def run_plain(args):
     ratio = cfg.pop("ratio")
     target = int(ratio * F0)
     dev = torch.device("cuda:0")
+    torch.zeros(1, device=dev)
+    torch.cuda.synchronize()
+    t_up = time.perf_counter()
     dv = torch.from_numpy(verts).to(dev)
     df = torch.from_numpy(f).to(dev)
     torch.cuda.synchronize()
+    h2d_ms = 1e3 * (time.perf_counter() - t_up)   # host buffers -> HBM (pageable), outside the timed region
     ctx = _lib.Context(0)
     if args.copy_probe:
         src = torch.ones(args.copy_probe << 20, dtype=torch.uint8, device=dev)
@@ -143,7 +155,10 @@ def run_plain(args):
         log("step %d: %.1f ms, rounds %d, final faces %d" % (s, 1e3 * (t1 - t0), st["rounds"], of.shape[0]))
     prof = ctx.profile_read(dom)
     ctx.profile([])
-    classes = _class_profile(ctx, lambda: (simplify_stage2(dv, df, scale, target, ctx=ctx), torch.cuda.synchronize()))
+    classes = None
+    if not args.no_classes:
+        classes = _class_profile(ctx, lambda: (simplify_stage2(dv, df, scale, target, ctx=ctx),
+                                               torch.cuda.synchronize()))
     sec = float(np.mean(times))
     value = F0 / sec / 1e6
     avg_ms = prof["ms"] / max(prof["launches"], 1)
@@ -161,6 +176,9 @@ def run_plain(args):
                     avg_algorithmic_bytes=round(avg_bytes))
     out_v = ov.cpu().numpy()
     out_f = of.cpu().numpy()
+    if args.save_mesh:
+        from pamo_amd import meshio
+        meshio.save(args.save_mesh, out_v, out_f)
     round_faces = [int(x) for x in st["round_faces"]]
     result = dict(metric="input Mfaces/s to %g%% (plain, one mesh)" % (100 * ratio), value=round(value, 3),
                   unit="Mfaces/s",
@@ -174,7 +192,8 @@ def run_plain(args):
                               candidate_pairs=st["candidate_pairs"]),
                   latency_ms=dict(p50=round(1e3 * float(np.percentile(times, 50)), 3),
                                   p99=round(1e3 * float(np.percentile(times, 99)), 3)),
-                  roofline=roofline, kernel_classes=classes)
+                  roofline=roofline, kernel_classes=classes,
+                  h2d_ms=round(h2d_ms, 3))
     if not args.no_cpu:
         ncores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
         result["cpu_baseline"] = _cpu_plain(verts, f, scale, target, round_faces, ncores, args.cpu_seconds)
@@ -182,11 +201,19 @@ def run_plain(args):
             one = _cpu_plain(verts, f, scale, target, round_faces, 1, args.cpu1_seconds)
             result["cpu_baseline"]["one_thread"] = {k: one[k] for k in ("value", "unit", "cores", "sample",
                                                                         "rounds_match_gpu")}
+        # FP32 (the reference's arithmetic, which the GPU matches bit for bit)
+        # against the same expressions in FP64, on the first round
+        import oracle
+        oracle.set_threads(ncores)
+        t0 = time.perf_counter()
+        result["fp64_error"] = dict(oracle.precision_report(verts, f, scale), scope="first round, unshuffled faces",
+                                    seconds=round(time.perf_counter() - t0, 2))
     if not args.no_verify:
         from pamo_amd.verify import verify
         r = verify(out_v, out_f)
         result["verify"] = {k: r[k] for k in ("manifold", "watertight", "degenerate_faces", "intersecting_pairs",
                                                "ok")}
+        result["verify"]["max_undo_passes"] = st["max_undo_passes"]
     print(json.dumps(result), flush=True)
 
 
@@ -262,7 +289,8 @@ def _plain_times(meshes, idx, ctx):
     from pamo_amd import batch, simplify_stage2
     dev = torch.device("cuda:0")
     out = {}
-    for i in idx:
+    t_log = time.perf_counter()
+    for n, i in enumerate(idx):
         v, f, r = meshes[i]
         dv, dt, sc, m = batch.prepare(v, f, dev)
         torch.cuda.synchronize()
@@ -270,6 +298,9 @@ def _plain_times(meshes, idx, ctx):
         ov, of, st = simplify_stage2(dv, dt, sc, int(r * len(f)), ctx=ctx)
         torch.cuda.synchronize()
         out[i] = (time.perf_counter() - t0, ov, of)
+        if time.perf_counter() - t_log > 10:
+            t_log = time.perf_counter()
+            log("plain path: %d of %d meshes" % (n + 1, len(idx)))
     return out
 
 
@@ -402,7 +433,13 @@ def run_serve(args):
         eng = batch.BatchEngine(ctx=ctx)
         srv = batch.Server(engine=eng)
         t0 = time.perf_counter()
-        recs = srv.run(jobs)
+        last = [t0]
+
+        def progress(done, total):
+            if time.perf_counter() - last[0] > 10:
+                last[0] = time.perf_counter()
+                log("serve %g jobs/s: %d of %d jobs done" % (rate, done, total))
+        recs = srv.run(jobs, progress=progress)
         wall = time.perf_counter() - t0
         del eng
         lat = np.array([r["latency"] for r in recs])

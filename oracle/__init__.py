@@ -61,6 +61,8 @@ def _lib():
         _orc.orc_run_timed.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_longlong,
                                        ctypes.c_float, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, ctypes.c_double,
                                        P, P, P, ctypes.c_int]
+        _orc.orc_cost_twin.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, P, P, P, P,
+                                       P, P, ctypes.c_int]
     return _orc
 
 
@@ -216,6 +218,75 @@ def is_coplanar(a, b):
     a = np.ascontiguousarray(a, np.float32).reshape(9)
     b = np.ascontiguousarray(b, np.float32).reshape(9)
     return bool(_lib().orc_is_coplanar(_ptr(a), _ptr(b)))
+
+
+def cost_twin(points, faces, scale, threshold=1e-3):
+    """First-round edge costs of a fresh mesh (no shuffle) evaluated in
+    float and in double from the same float positions: dict of c32, c64
+    (unquantized values), st32, st64 (0 value, 1 maximum, 2 unchanged), q32
+    (the float value quantized) and mid (relative midpoint difference)."""
+    lib = _lib()
+    pts = np.ascontiguousarray(points, np.float32)
+    tri = np.ascontiguousarray(faces, np.int32)
+    cap = 2 * len(tri) + 1
+    out = dict(c32=np.zeros(cap, np.float32), c64=np.zeros(cap, np.float64), st32=np.zeros(cap, np.int32),
+               st64=np.zeros(cap, np.int32), q32=np.zeros(cap, np.uint32), mid=np.zeros(cap, np.float64))
+    E = lib.orc_cost_twin(_ptr(pts), _ptr(tri), len(pts), len(tri), float(scale), float(threshold),
+                          _ptr(out["c32"]), _ptr(out["c64"]), _ptr(out["st32"]), _ptr(out["st64"]), _ptr(out["q32"]),
+                          _ptr(out["mid"]), cap)
+    if E < 0:
+        raise RuntimeError("orc_cost_twin failed")
+    return {k: v[:E] for k, v in out.items()}
+
+
+def precision_report(points, faces, scale, threshold=1e-3):
+    """FP32 against FP64 on the first round: the relative error of the
+    unquantized cost (edges valued in both), the share of edges whose
+    quantized cost or status differs, and the largest relative midpoint
+    (collapse position) difference."""
+    t = cost_twin(points, faces, scale, threshold)
+    both = (t["st32"] == 0) & (t["st64"] == 0)
+    c64 = t["c64"][both]
+    rel = np.abs(t["c32"][both].astype(np.float64) - c64) / np.maximum(np.abs(c64), 1e-300)
+    q64 = np.where(t["st64"] == 1, np.float64(4294967295.0),
+                   np.floor(np.clip(t["c64"], 0.0, 10.0) / 10.0 * 4294967296.0))
+    q64 = np.minimum(q64, 4294967295.0).astype(np.uint64)
+    qdiff = (t["q32"].astype(np.uint64) != q64) & (t["st32"] != 2)
+    # first-round winners under each cost (unchanged costs are 0 in a fresh
+    # context): the claim of cusimp_free.cu:587-681 as vertex minima
+    tri = np.ascontiguousarray(faces, np.int64)
+    alive = ~(tri == -1).all(axis=1)
+    a = tri[alive]
+    parts = []
+    for (x, y) in ((0, 1), (1, 2), (2, 0)):
+        sel = a[:, x] > a[:, y]
+        parts.append((np.where(sel)[0] * 3 + x, a[sel][:, [y, x]]))
+    order = np.argsort(np.concatenate([p[0] for p in parts]), kind="stable")
+    edges = np.concatenate([p[1] for p in parts])[order]
+    ct = np.uint64(int(np.float32(min(max(threshold, 0.0), 10.0)) / np.float32(10.0) * np.float32(4294967296.0)))
+
+    def winners(q):
+        q = np.where(t["st32"] == 2, 0, q).astype(np.uint64)
+        key = (q << np.uint64(32)) | np.arange(len(q), dtype=np.uint64)
+        V = int(a.max()) + 1 if a.size else 0
+        vmin = np.full(V, np.iinfo(np.uint64).max, np.uint64)
+        np.minimum.at(vmin, edges[:, 0], key)
+        np.minimum.at(vmin, edges[:, 1], key)
+        fmin = np.minimum(np.minimum(vmin[a[:, 0]], vmin[a[:, 1]]), vmin[a[:, 2]])
+        vf = np.full(V, np.iinfo(np.uint64).max, np.uint64)
+        for k in range(3):
+            np.minimum.at(vf, a[:, k], fmin)
+        u, v = edges[:, 0], edges[:, 1]
+        return (q <= ct) & (vmin[u] == key) & (vmin[v] == key) & (vf[u] == key) & (vf[v] == key)
+
+    w32, w64 = winners(t["q32"]), winners(q64)
+    return dict(edges=int(len(t["c32"])), valued=int(both.sum()), status_differs=int((t["st32"] != t["st64"]).sum()),
+                cost_rel_err_median=float(np.median(rel)) if rel.size else 0.0,
+                cost_rel_err_p99=float(np.percentile(rel, 99)) if rel.size else 0.0,
+                cost_rel_err_max=float(rel.max()) if rel.size else 0.0,
+                quantized_differs=int(qdiff.sum()),
+                winners_fp32=int(w32.sum()), winners_fp64=int(w64.sum()), winners_differ=int((w32 != w64).sum()),
+                position_rel_err_max=float(t["mid"].max()) if len(t["mid"]) else 0.0)
 
 
 def full_check(points, tris):

@@ -925,6 +925,115 @@ static int run_loop(Ctx& c, const float* verts, const int* faces, int V, int F, 
   return 0;
 }
 
+// ------------------------------------------------------------ FP64 twin
+// Precision measurement, not part of the restatement: the unquantized cost
+// value (cost + el + Qa / nt * tau, before clamp and quantization) of every
+// edge of a fresh mesh's first round (no table, not stuck), evaluated with
+// the expressions of edge_cost_one and build_quadrics in T = float (whose
+// quantization equals edge_cost_one's, checked in tests) and in T = double
+// from the same float positions.
+template <class T> struct W3 { T x, y, z; };
+template <class T> static inline W3<T> wsub(W3<T> a, W3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T> static inline W3<T> wadd(W3<T> a, W3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T> static inline T wdot(W3<T> a, W3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T> static inline W3<T> wcross(W3<T> a, W3<T> b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class T> static inline T wnorm(W3<T> a) { return std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+template <class T> static inline W3<T> wdivs(W3<T> a, T s) { return {a.x / s, a.y / s, a.z / s}; }
+template <class T> static inline T tclamp(T x, T a, T b) {
+  T y = a > x ? a : x;   // NaN -> upper bound, as clamp_ref
+  return y < b ? y : b;
+}
+
+template <class T> struct Twin {
+  const Ctx& c;
+  std::vector<W3<T>> P;
+  std::vector<T> Q;
+  explicit Twin(const Ctx& ctx) : c(ctx) {
+    P.resize(c.V);
+    for (int v = 0; v < c.V; v++) P[v] = {(T)c.P[v].x, (T)c.P[v].y, (T)c.P[v].z};
+    Q.assign((size_t)16 * c.V, (T)0);
+    for (int v = 0; v < c.V; v++) {
+      T K[16] = {0};
+      for (int s = c.fan_off[v]; s < c.fan_off[v + 1]; s++) {
+        int f = c.fan[s];
+        W3<T> v0 = P[c.T[3 * f]], v1 = P[c.T[3 * f + 1]], v2 = P[c.T[3 * f + 2]];
+        W3<T> n = wcross(wsub(v1, v0), wsub(v2, v0));
+        n = wdivs(n, wnorm(n));
+        T off = -wdot(n, v0);
+        T p[4] = {n.x, n.y, n.z, off};
+        for (int r = 0; r < 4; r++) for (int q = 0; q < 4; q++) K[4 * r + q] += p[r] * p[q];
+      }
+      for (int r = 0; r < 16; r++) Q[(size_t)16 * v + r] = K[r];
+    }
+  }
+  static T area(W3<T> p0, W3<T> p1, W3<T> p2) {
+    T a = wnorm(wsub(p0, p1)), b = wnorm(wsub(p1, p2)), d = wnorm(wsub(p2, p0));
+    T s = (a + b + d) / 2;
+    return std::sqrt(s * (s - a) * (s - b) * (s - d));
+  }
+  bool skinny(int moved, int other, W3<T> m, T& Qa, int& nt) const {
+    for (int s = c.fan_off[moved]; s < c.fan_off[moved + 1]; s++) {
+      int f = c.fan[s];
+      int i = c.T[3 * f], j = c.T[3 * f + 1], k = c.T[3 * f + 2];
+      if (i == other || j == other || k == other) continue;
+      W3<T> o0 = P[i], o1 = P[j], o2 = P[k];
+      W3<T> n0 = i == moved ? m : o0, n1 = j == moved ? m : o1, n2 = k == moved ? m : o2;
+      W3<T> on = wcross(wsub(o1, o0), wsub(o2, o0));
+      on = wdivs(on, wnorm(on));
+      W3<T> nn = wcross(wsub(n1, n0), wsub(n2, n0));
+      nn = wdivs(nn, wnorm(nn));
+      if (wdot(on, nn) < 0) return false;
+      T e01 = wnorm(wsub(n0, n1)), e12 = wnorm(wsub(n1, n2)), e20 = wnorm(wsub(n2, n0));
+      T term = (T)4 * std::sqrt((T)3) * area(n0, n1, n2) / (e01 * e01) + e12 * e12 + e20 * e20 + (T)0.0000001f;
+      Qa += (T)1 - tclamp(term, (T)0, (T)1);
+      nt++;
+    }
+    return true;
+  }
+  // 0: value in out and the midpoint in m; 1: maximum (link > 2 or a flip);
+  // 2: unchanged (link != 2)
+  int cost(int e, T& out, W3<T>& m) const {
+    int u = c.edges[2 * e], v = c.edges[2 * e + 1];
+    int dup = 0;
+    for (int a = c.fan_off[u]; a < c.fan_off[u + 1]; a++) {
+      int iu = Ctx::succ(&c.T[3 * c.fan[a]], u);
+      for (int b = c.fan_off[v]; b < c.fan_off[v + 1]; b++) {
+        if (iu == Ctx::succ(&c.T[3 * c.fan[b]], v)) dup++;
+        if (dup > 2) return 1;
+      }
+    }
+    if (dup != 2) return 2;
+    T el = 0;
+    int ne = 0;
+    for (int w : {u, v}) {
+      for (int s = c.fan_off[w]; s < c.fan_off[w + 1]; s++) {
+        const int* t = &c.T[3 * c.fan[s]];
+        if (t[0] > t[1]) { el += wnorm(wsub(P[t[0]], P[t[1]])); ne++; }
+        if (t[1] > t[2]) { el += wnorm(wsub(P[t[1]], P[t[2]])); ne++; }
+        if (t[2] > t[0]) { el += wnorm(wsub(P[t[2]], P[t[0]])); ne++; }
+      }
+    }
+    const T scale = (T)c.scale, tau = (T)c.tau;
+    el = el / (T)ne / scale * tau;
+    el = el + wnorm(wsub(P[u], P[v])) / scale * tau;
+    m = wdivs(wadd(P[u], P[v]), (T)2);
+    T Qa = 0;
+    int nt = 0;
+    if (!skinny(u, v, m, Qa, nt) || !skinny(v, u, m, Qa, nt)) return 1;
+    Qa = (T)5 * Qa;
+    T M[16];
+    for (int r = 0; r < 16; r++) M[r] = Q[(size_t)16 * u + r] + Q[(size_t)16 * v + r];
+    T w[4] = {m.x, m.y, m.z, (T)1};
+    T t4[4];
+    for (int q = 0; q < 4; q++) t4[q] = M[q] * w[0] + M[4 + q] * w[1] + M[8 + q] * w[2] + M[12 + q] * w[3];
+    T qe = t4[0] * w[0] + t4[1] * w[1] + t4[2] * w[2] + t4[3] * w[3];
+    out = qe / (scale * scale) + el + Qa / (T)nt * tau;
+    return 0;
+  }
+};
+
 }  // namespace orc
 
 // ================================================================ C API
@@ -1083,6 +1192,49 @@ int orc_full_check(const float* pts, const int* tris, int V, int F, int* cand, l
   for (long long i = 0; i < (long long)cp.size() && 2 * i + 1 < cand_cap; i++) { cand[2 * i] = cp[i].first; cand[2 * i + 1] = cp[i].second; }
   for (long long i = 0; i < (long long)hp.size() && 2 * i + 1 < hit_cap; i++) { hits[2 * i] = hp[i].first; hits[2 * i + 1] = hp[i].second; }
   return 0;
+}
+
+// First-round edge costs of a fresh mesh in float and in double (see the
+// FP64 twin above): c32/c64 the unquantized values, st32/st64 the status
+// (0 value, 1 maximum, 2 unchanged), q32 the float value quantized as the
+// cost kernel does, and mid[e] the largest relative difference of the float
+// and double midpoints.  Returns E, or -1 when E exceeds cap.
+int orc_cost_twin(const float* pts, const int* tris, int nV, int nF, float scale, float threshold, float* c32,
+                  double* c64, int* st32, int* st64, uint32_t* q32, double* mid, int cap) {
+  orc::Ctx c;
+  c.V = nV;
+  c.F = nF;
+  c.P.resize(nV);
+  for (int v = 0; v < nV; v++) c.P[v] = {pts[3 * v], pts[3 * v + 1], pts[3 * v + 2]};
+  c.T.assign(tris, tris + 3LL * nF);
+  c.scale = scale;
+  c.tau = threshold;
+  c.build_fans();
+  c.build_edges();
+  if (c.E > cap) return -1;
+  orc::Twin<float> tf(c);
+  orc::Twin<double> td(c);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int e = 0; e < c.E; e++) {
+    float vf = 0;
+    double vd = 0;
+    orc::W3<float> mf{0, 0, 0};
+    orc::W3<double> md{0, 0, 0};
+    st32[e] = tf.cost(e, vf, mf);
+    st64[e] = td.cost(e, vd, md);
+    c32[e] = vf;
+    c64[e] = vd;
+    q32[e] = st32[e] == 1 ? 0xFFFFFFFFu
+                          : orc::u32sat(orc::clamp_ref(vf, 0.0f, orc::COST_RANGE) / orc::COST_RANGE * orc::U32F);
+    double r = 0;
+    const double a[3] = {mf.x, mf.y, mf.z}, b[3] = {md.x, md.y, md.z};
+    for (int k = 0; k < 3; k++) {
+      double d = std::fabs(a[k] - b[k]) / std::max(std::fabs(b[k]), 1e-30);
+      r = std::max(r, d);
+    }
+    mid[e] = st32[e] == 0 ? r : 0.0;
+  }
+  return c.E;
 }
 
 }  // extern "C"

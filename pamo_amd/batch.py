@@ -155,9 +155,11 @@ class Server:
         self.prep_threads = prep_threads
         self.stats = {}
 
-    def run(self, jobs, realtime=True, on_result=None):
+    def run(self, jobs, realtime=True, on_result=None, progress=None):
         """jobs: list of dicts {points, faces, ratio, arrival} (arrival in
-        seconds from the start).  Returns per-job records."""
+        seconds from the start).  on_result(i, verts, faces, info) receives
+        each result as host arrays; progress(done, total) is called after
+        each completion.  Returns per-job records."""
         dev = torch.device("cuda", torch.cuda.current_device())
         order = sorted(range(len(jobs)), key=lambda i: jobs[i]["arrival"])
         waiting = []
@@ -228,6 +230,8 @@ class Server:
                     if on_result is not None:
                         on_result(i, v.cpu().numpy() + m, f.cpu().numpy(), info)
                     done += 1
+                    if progress is not None:
+                        progress(done, len(jobs))
         finally:
             pool.shutdown(wait=False, cancel_futures=True)
         return recs

@@ -485,3 +485,26 @@ def test_sdf_generator_is_closed_manifold():
     v, f = synth.sdf_union(30, 100, seed=5, n_slab_pairs=4)
     r = pv.verify(v, f)
     assert r["manifold"] and r["watertight"] and r["degenerate_faces"] == 0
+
+
+# ------------------------------------------------------- FP64 precision
+def test_cost_twin_float_equals_oracle_and_fp64_bounds():
+    """The float instance of the cost twin quantizes to the oracle's own
+    first-round costs, and its winners (the claim rule in numpy) are the
+    oracle's; the double instance gives the FP64 reference values.  The
+    collapse positions (midpoints) agree to one float rounding."""
+    v, f = synth.displaced_icosphere(4, 7)
+    verts, scale, _ = oracle.preprocess(v, f)
+    tw = oracle.cost_twin(verts, f, scale)
+    o = oracle.Oracle(oracle.SHUFFLE_IDENTITY)
+    o.forward(verts, f.astype(np.int32), np.zeros(0, np.int32), float(scale), 1e-3, False, True)
+    ec = o.get("edge_cost")[:len(tw["q32"])]
+    sel = tw["st32"] != 2
+    assert sel.any() and np.array_equal(tw["q32"][sel], ec[sel])
+    r = oracle.precision_report(verts, f, scale)
+    assert r["winners_fp32"] == o.sizes()["n_winners"]
+    assert r["status_differs"] == 0
+    assert r["position_rel_err_max"] <= 2.0 ** -23
+    # FP32 evaluation of the reference's expressions stays near FP64 for
+    # most edges (the QEM term cancels near the surface, hence the tail)
+    assert r["cost_rel_err_median"] < 0.05
