@@ -62,6 +62,8 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=20.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-verify", action="store_true")
+    p.add_argument("--strict", action="store_true",
+                   help="plain workloads: the strict guard (exact pair test, undo until clean; not the reference's)")
     p.add_argument("--save-mesh", default="",
                    help="plain workloads: write the output mesh (.obj or .ply) here, in the centred frame the "
                         "guard works in")
@@ -130,6 +132,7 @@ def run_plain(args):
     torch.cuda.synchronize()
     h2d_ms = 1e3 * (time.perf_counter() - t_up)   # host buffers -> HBM (pageable), outside the timed region
     ctx = _lib.Context(0)
+    ctx.set_strict(args.strict)
     if args.copy_probe:
         src = torch.ones(args.copy_probe << 20, dtype=torch.uint8, device=dev)
         dst = torch.empty_like(src)
@@ -187,7 +190,7 @@ def run_plain(args):
                   data="synthetic (seeded SDF mesh generated in-process)",
                   config=dict(cfg, faces=F0, vertices=int(v.shape[0]), ratio=ratio, tolerance=4, threshold=1e-3,
                               min_verts=0, target_faces=target, rounds=st["rounds"], final_faces=int(out_f.shape[0]),
-                              exit_reason=st["exit_reason"], parallelism="single GPU",
+                              exit_reason=st["exit_reason"], parallelism="single GPU", strict_guard=args.strict,
                               guard_checks=st["checks"], query_faces=st["query_faces"],
                               candidate_pairs=st["candidate_pairs"]),
                   latency_ms=dict(p50=round(1e3 * float(np.percentile(times, 50)), 3),

@@ -190,6 +190,18 @@ int pamo_debug_pair_cap(pamo_ctx* ctx, long long cap);
  * exhaustive search.  Both give the same candidate set. */
 int pamo_debug_guard(pamo_ctx* ctx, int mode);
 
+/* Strict guard for the engine (pamo_run_stage2, pamo_batch_*); no
+ * counterpart in the reference, off by default.  For callers that need every
+ * output to pass an exact self-intersection check: a candidate pair also
+ * counts as intersecting when the exact closed triangle test (filtered FP64
+ * orientation predicates with an exact fallback) says so, which catches the
+ * pairs the reference's narrow phase skips (coplanar, shared-edge
+ * coordinates, short shared-vertex contacts); the undo scan covers every
+ * collapsed edge, and undo passes repeat until no collapse is marked (at
+ * most 64).  Results then differ from the reference's wherever it keeps an
+ * intersection.  The round kernels (pamo_free_forward) are unaffected. */
+int pamo_set_strict(pamo_ctx* ctx, int strict);
+
 /* Parity hook: every ordered candidate pair (query, current) of one
  * self-intersection check over all alive faces of a mesh, with the
  * reference narrow-phase verdict (simp_cuda/src/bvh/self_intersect.cuh:151-392).

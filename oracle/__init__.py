@@ -51,7 +51,7 @@ def _lib():
         _orc.orc_copy.argtypes = [P, ctypes.c_int, P, ctypes.c_longlong]
         _orc.orc_copy.restype = ctypes.c_longlong
         _orc.orc_run.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_longlong, ctypes.c_float,
-                                 ctypes.c_int, ctypes.c_longlong, ctypes.c_int, P, P, P, P, ctypes.c_int]
+                                 ctypes.c_int, ctypes.c_longlong, ctypes.c_int, P, P, P, P, ctypes.c_int, ctypes.c_int]
         _orc.orc_shuffle_perm.argtypes = [ctypes.c_longlong, P]
         _orc.orc_narrow.argtypes = [P, P, P, P]
         _orc.orc_isect_raw.argtypes = [P, P, P, P]
@@ -147,8 +147,10 @@ class Oracle:
 
 
 def run(verts, faces, scale, target, threshold=1e-3, tolerance=4, iters=1000000, shuffle=SHUFFLE_FEISTEL,
-        return_rounds=False):
-    """The stage-2 round loop of ``PaMO.run`` on centred vertices (C++)."""
+        return_rounds=False, strict=False):
+    """The stage-2 round loop of ``PaMO.run`` on centred vertices (C++).
+    strict: the strict guard (not in the reference): exact pair test added
+    to the reference's, whole collapsed list scanned, undo until clean."""
     lib = _lib()
     verts = np.ascontiguousarray(verts, np.float32)
     faces = np.ascontiguousarray(faces, np.int32)
@@ -159,7 +161,7 @@ def run(verts, faces, scale, target, threshold=1e-3, tolerance=4, iters=1000000,
     cap = 100000
     rf = np.zeros(cap, np.int32)
     rc = lib.orc_run(_ptr(verts), _ptr(faces), V, F, float(scale), int(target), float(threshold), int(tolerance),
-                     int(iters), int(shuffle), _ptr(ov), _ptr(of), _ptr(st), _ptr(rf), cap)
+                     int(iters), int(shuffle), _ptr(ov), _ptr(of), _ptr(st), _ptr(rf), cap, int(bool(strict)))
     if rc != 0:
         raise RuntimeError("oracle run failed with code %d" % rc)
     stats = dict(rounds=int(st[0]), V=int(st[1]), F=int(st[2]), exit_reason=int(st[3]),

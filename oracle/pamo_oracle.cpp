@@ -61,6 +61,13 @@
 //   run_loop              termination by target and by tolerance; with no
 //                         collapse possible it stops after exactly `tolerance`
 //                         rounds and returns the mesh unchanged
+//
+// Not part of the restatement (measurement and an opt-in extension):
+//   Twin (FP64 twin)      its float instance quantizes to edge_cost_one's
+//                         costs and its winners are claim's (tests)
+//   strict guard          exact pair test from ../pamo_amd/csrc/exact_pred.h
+//                         (pinned through the verifier's tests); outputs pass
+//                         the exact verifier where the reference's do not
 
 #include <algorithm>
 #include <atomic>
@@ -71,6 +78,8 @@
 #include <cstring>
 #include <numeric>
 #include <vector>
+
+#include "../pamo_amd/csrc/exact_pred.h"
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -381,6 +390,12 @@ struct Ctx {
   long long n_queries = 0, n_candidates = 0, n_hits = 0;  // last call, summed over checks
   int n_checks = 0;
   int err = 0;
+  // Strict guard (not in the reference): a candidate pair also counts when
+  // the exact closed triangle test says the faces intersect, the undo scan
+  // covers the whole collapsed list and the undo loop runs until nothing is
+  // marked (at most STRICT_PASSES passes).
+  bool strict = false;
+  static const int STRICT_PASSES = 64;
 
   void ensure_pts(size_t n) {
     if (n > alloc_pts) {
@@ -700,8 +715,18 @@ struct Ctx {
         ncand++;
         float pg[9]; face_pts(g, pg);
         const int* tg = &T[3 * g];
-        if (narrow_pair(pg, pf, tg, tf) == NP_HIT) { hit = true; nhit++; }
-        if (narrow_pair(pf, pg, tf, tg) == NP_HIT) { hit = true; nhit++; }
+        bool ph = false;
+        if (narrow_pair(pg, pf, tg, tf) == NP_HIT) { ph = true; nhit++; }
+        if (narrow_pair(pf, pg, tf, tg) == NP_HIT) { ph = true; nhit++; }
+        if (!ph && strict) {
+          xp::P3 A[3], B[3];
+          for (int k = 0; k < 3; k++) {
+            A[k] = {(double)pf[3 * k], (double)pf[3 * k + 1], (double)pf[3 * k + 2]};
+            B[k] = {(double)pg[3 * k], (double)pg[3 * k + 1], (double)pg[3 * k + 2]};
+          }
+          if (xp::tri_pair(A, tf, B, tg)) { ph = true; nhit++; }
+        }
+        if (ph) hit = true;
       });
       if (hit) S[f] = 1;
     }
@@ -833,7 +858,7 @@ struct Ctx {
     remove_line_edges();
     {
       int h = n_winners, tot = (int)collapsed.size();
-      undo_cut = h == 0 ? 0 : std::min(tot, ((h + 511) / 512) * 512);
+      undo_cut = strict ? tot : (h == 0 ? 0 : std::min(tot, ((h + 511) / 512) * 512));
     }
     check();
     S_first = S;
@@ -853,7 +878,7 @@ struct Ctx {
       check();
       hm = mark(marked);
       n_vertices_undo += 2 * hm;
-      if (it == 5) break;
+      if (it == (strict ? STRICT_PASSES : 5)) break;
     }
     undo_passes = it;
     ensure_undo((size_t)n_vertices_undo + 1);
@@ -1105,8 +1130,9 @@ long long orc_copy(void* ctx, int which, void* dst, long long cap_bytes) {
 
 int orc_run(const float* verts, const int* faces, int V, int F, float scale, long long target, float threshold,
             int tolerance, long long iters, int shuffle_mode, float* outV, int* outF, long long stats[8],
-            int* round_faces, int round_cap) {
+            int* round_faces, int round_cap, int strict) {
   orc::Ctx c;
+  c.strict = strict != 0;
   std::vector<float> ov; std::vector<int> of; orc::RunStats st;
   std::vector<int> rf;
   int rc = orc::run_loop(c, verts, faces, V, F, scale, target, threshold, tolerance, iters, shuffle_mode, ov, of, st, &rf);

@@ -69,8 +69,9 @@ def _scale(verts):
 
 
 def simplify_stage2(verts, faces, scale, target, threshold=1e-3, tolerance=4, iters=1000000, ctx=None,
-                    shuffle=_lib.SHUFFLE_FEISTEL):
+                    shuffle=_lib.SHUFFLE_FEISTEL, strict=None):
     """Device-resident stage-2 loop on centred vertices (torch CUDA tensors).
+    strict: True/False sets the context's strict guard (None keeps it).
 
     Returns (verts, faces, stats) as torch CUDA tensors and a dict."""
     import ctypes
@@ -81,6 +82,8 @@ def simplify_stage2(verts, faces, scale, target, threshold=1e-3, tolerance=4, it
     faces = faces.contiguous()
     if verts.dtype != torch.float32 or faces.dtype != torch.int32:
         raise RuntimeError("verts must be float32 and faces int32")
+    if strict is not None:
+        ctx.set_strict(strict)
     L = ctx._L
     stream = torch.cuda.current_stream().cuda_stream
     oV, oF = ctypes.c_int(), ctypes.c_int()
@@ -109,7 +112,7 @@ class PaMO(torch.nn.Module):
     results."""
 
     def __init__(self, input_mesh, use_stage1=True, use_stage3=True, verbose=True, engine="device",
-                 shuffle=_lib.SHUFFLE_FEISTEL):
+                 shuffle=_lib.SHUFFLE_FEISTEL, strict=False):
         super().__init__()
         if use_stage1:
             raise NotImplementedError("stage 1 (SDF remeshing) is not part of pamo_amd; pass use_stage1=False")
@@ -125,6 +128,9 @@ class PaMO(torch.nn.Module):
         v, _ = _as_vf(input_mesh)
         self.bbox = np.stack([v.min(0), v.max(0)])
         self.pamo = _C.CUDSP_Free(shuffle)
+        if strict and engine != "device":
+            raise ValueError("strict=True needs engine='device' (the round kernels follow the reference)")
+        self.pamo._ctx.set_strict(strict)
         self.target_faces = None
         self.last_stats = None
         self.last_stage2_seconds = None

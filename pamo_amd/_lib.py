@@ -69,6 +69,7 @@ def lib():
     L.pamo_debug_pair_cap.argtypes = [P, LL]
     L.pamo_debug_pair_cap.restype = I
     L.pamo_debug_guard.argtypes = [P, I]
+    L.pamo_set_strict.argtypes = [P, I]
     L.pamo_debug_guard.restype = I
     L.pamo_simplify.argtypes = [P, P, P, P, P, F]
     L.pamo_simplify.restype = None
@@ -124,6 +125,11 @@ class Context:
     def debug_guard(self, mode):
         """Candidate search of the engine's check: 0 by size, 1 LBVH, 2 exhaustive; test hook."""
         self.check(self._L.pamo_debug_guard(self.handle, int(mode)), "pamo_debug_guard")
+
+    def set_strict(self, strict):
+        """Strict guard for the engine (off by default; not in the reference):
+        exact pair test added, every collapse scanned, undo until clean."""
+        self.check(self._L.pamo_set_strict(self.handle, 1 if strict else 0), "pamo_set_strict")
 
     def check(self, rc, what):
         if rc < 0:

@@ -14,6 +14,7 @@
 // box, touching included) does not depend on the tree shape.
 #pragma once
 #include "pamo_kernels.h"
+#include "exact_pred.h"
 
 namespace pamo {
 
@@ -198,6 +199,29 @@ __device__ __forceinline__ bool pair_hits(const float pf[3][3], const float pg[3
   int nc = shared_indices(tg, tf);
   if (nc == 2) return false;
   return narrow_hit(pg, pf, nc) || narrow_hit(pf, pg, nc);
+}
+// The strict guard (not in the reference; off by default): a pair also
+// counts when the exact closed triangle test of exact_pred.h says the faces
+// intersect, which decides the pairs the reference's narrow phase skips
+// (coplanar, shared-edge coordinates, short shared-vertex contacts) and
+// those its float interval test misses.
+__device__ __attribute__((noinline)) bool exact_pair(const float pf[3][3], const float pg[3][3], I3 tf, I3 tg) {
+  xp::P3 A[3], B[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    A[k] = {(double)pf[k][0], (double)pf[k][1], (double)pf[k][2]};
+    B[k] = {(double)pg[k][0], (double)pg[k][1], (double)pg[k][2]};
+  }
+  const int ia[3] = {tf.i, tf.j, tf.k}, ib[3] = {tg.i, tg.j, tg.k};
+  return xp::tri_pair(A, ia, B, ib);
+}
+// STRICT is a template parameter: the default kernels never reference the
+// exact test, which would otherwise set their register budget to its own
+template <int STRICT>
+__device__ __forceinline__ bool pair_hits_g(const float pf[3][3], const float pg[3][3], I3 tf, I3 tg) {
+  if (pair_hits(pf, pg, tf, tg)) return true;
+  if constexpr (STRICT != 0) return exact_pair(pf, pg, tf, tg);
+  return false;
 }
 
 // ------------------------------------------------------------- LBVH

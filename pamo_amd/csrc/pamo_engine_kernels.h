@@ -707,7 +707,8 @@ __global__ void __launch_bounds__(BLOCK) k_line_cut_tail(int M, const int* __res
                                                          int* __restrict__ h, int* __restrict__ ncoll,
                                                          int* __restrict__ cut, long long* __restrict__ cutoff,
                                                          long long* __restrict__ uobase,
-                                                         const int* __restrict__ nustk, int* __restrict__ part) {
+                                                         const int* __restrict__ nustk, int* __restrict__ part,
+                                                         int strict) {
   for (int k = threadIdx.x; k < M; k += BLOCK) {
     const int hk = winpos[eoffm[k + 1]] - winpos[eoffm[k]];   // the mesh's winners
     h[k] = hk;
@@ -725,7 +726,8 @@ __global__ void __launch_bounds__(BLOCK) k_line_cut_tail(int M, const int* __res
       }
     }
     ncoll[k] = n;
-    int ck = hk == 0 ? 0 : min(n, ((hk + 511) / 512) * 512);
+    // the strict guard scans every collapsed entry
+    int ck = strict ? n : (hk == 0 ? 0 : min(n, ((hk + 511) / 512) * 512));
     cut[k] = ck;
     part[k] = ck > 0 ? 1 : 0;
   }
@@ -1077,6 +1079,7 @@ __global__ void k_mesh_roots(int M, const int* __restrict__ n_ptr, const int* __
 }
 
 // traversal from the query's mesh root to the end of that subtree
+template <int STRICT>
 __global__ void __launch_bounds__(BLOCK) k_traverse_b(const int* __restrict__ qlist, const int* __restrict__ nq_ptr,
                                                       const int* __restrict__ n_ptr, const int* __restrict__ fmesh,
                                                       const int* __restrict__ root, const int* __restrict__ root_end,
@@ -1125,7 +1128,7 @@ __global__ void __launch_bounds__(BLOCK) k_traverse_b(const int* __restrict__ ql
           pg[0][0] = a.x; pg[0][1] = a.y; pg[0][2] = a.z;
           pg[1][0] = b.x; pg[1][1] = b.y; pg[1][2] = b.z;
           pg[2][0] = cc.x; pg[2][1] = cc.y; pg[2][2] = cc.z;
-          if (pair_hits(pf, pg, tf, tg)) { hit = true; break; }
+          if (pair_hits_g<STRICT>(pf, pg, tf, tg)) { hit = true; break; }
         }
       }
       node = esc;
@@ -1290,6 +1293,7 @@ __global__ void __launch_bounds__(BLOCK) k_traverse_emit(
 // One candidate pair per lane (grid-stride over the pairs the walk wrote):
 // both orientations of the reference narrow phase; a hit flags the query.
 // pairs handled by blocks bid of nb (grid-stride)
+template <int STRICT>
 __device__ void pair_test_blocks(long long bid, long long nb, const int2* __restrict__ pairs,
                                  const unsigned long long* __restrict__ npairs, long long pair_cap,
                                  const float* __restrict__ P, const int* __restrict__ T, uint8_t* __restrict__ S,
@@ -1314,7 +1318,7 @@ __device__ void pair_test_blocks(long long bid, long long nb, const int2* __rest
       pg[1][0] = b.x; pg[1][1] = b.y; pg[1][2] = b.z;
       pg[2][0] = c.x; pg[2][1] = c.y; pg[2][2] = c.z;
     }
-    if (pair_hits(pf, pg, tf, tg)) {
+    if (pair_hits_g<STRICT>(pf, pg, tf, tg)) {
       S[pr.x] = 1;
       hits++;
     }
@@ -1326,11 +1330,12 @@ __device__ void pair_test_blocks(long long bid, long long nb, const int2* __rest
     if ((threadIdx.x & 63) == 0 && h64) atomicAdd(&stats[1], h64);
   }
 }
+template <int STRICT>
 __global__ void __launch_bounds__(BLOCK) k_pair_test(const int2* __restrict__ pairs,
                                                      const unsigned long long* __restrict__ npairs, long long pair_cap,
                                                      const float* __restrict__ P, const int* __restrict__ T,
                                                      uint8_t* __restrict__ S, unsigned long long* __restrict__ stats) {
-  pair_test_blocks(blockIdx.x, gridDim.x, pairs, npairs, pair_cap, P, T, S, stats);
+  pair_test_blocks<STRICT>(blockIdx.x, gridDim.x, pairs, npairs, pair_cap, P, T, S, stats);
 }
 
 // --------------------------------------------- exhaustive candidate search
@@ -1483,6 +1488,7 @@ struct BfFull {
   const float4* lbox_lo;
   const float4* lbox_hi;
 };
+template <int STRICT>
 __device__ void bf_full_query(long long qi, const BfFull& a, const float* __restrict__ P, const int* __restrict__ T,
                               uint8_t* __restrict__ S) {
   const int* __restrict__ ovf = a.ovf;
@@ -1522,18 +1528,19 @@ __device__ void bf_full_query(long long qi, const BfFull& a, const float* __rest
     pg[0][0] = a.x; pg[0][1] = a.y; pg[0][2] = a.z;
     pg[1][0] = b.x; pg[1][1] = b.y; pg[1][2] = b.z;
     pg[2][0] = c.x; pg[2][1] = c.y; pg[2][2] = c.z;
-    hit = pair_hits(pf, pg, tf, tg);
+    hit = pair_hits_g<STRICT>(pf, pg, tf, tg);
   }
   if (hit) S[f] = 1;
 }
 // the flat pair test (blocks below nbp) and the overflow queries (the rest)
 // of an exhaustive check, in one launch: both only set flags
+template <int STRICT>
 __global__ void __launch_bounds__(BLOCK) k_pairs_bf(int nbp, const int2* __restrict__ pairs,
                                                     const unsigned long long* __restrict__ npairs, long long pair_cap,
                                                     BfFull a, const float* __restrict__ P, const int* __restrict__ T,
                                                     uint8_t* __restrict__ S, unsigned long long* __restrict__ stats) {
-  if ((int)blockIdx.x < nbp) pair_test_blocks(blockIdx.x, nbp, pairs, npairs, pair_cap, P, T, S, stats);
-  else bf_full_query((long long)(blockIdx.x - nbp) * blockDim.x + threadIdx.x, a, P, T, S);
+  if ((int)blockIdx.x < nbp) pair_test_blocks<STRICT>(blockIdx.x, nbp, pairs, npairs, pair_cap, P, T, S, stats);
+  else bf_full_query<STRICT>((long long)(blockIdx.x - nbp) * blockDim.x + threadIdx.x, a, P, T, S);
 }
 
 // --------------------------------------------------------------- marks

@@ -80,6 +80,7 @@ struct pamo_ctx {
   int run_handle = -1;          // engine slot of the last pamo_run_stage2
   long long pair_cap_limit = -1;  // test hook: caps the traversal's pair buffer (-1: none)
   int guard_mode = 0;             // test hook: 0 by mesh size, 1 tree only, 2 exhaustive only
+  int strict = 0;                 // strict guard (engine; not in the reference)
 
   int* hc = nullptr;   // pinned host mirror of the counters
 
@@ -788,6 +789,12 @@ void pamo_simplify(float* in_verts, int* in_tris, float* out_verts, int* out_tri
     fprintf(stderr, "pamo_simplify: %s\n", pamo_last_error(ctx));
     in_out_nv_ni[0] = in_out_nv_ni[1] = -1;
   }
+}
+
+int pamo_set_strict(pamo_ctx* ctx, int strict) {
+  if (!ctx) return -1;
+  ctx->strict = strict ? 1 : 0;
+  return 0;
 }
 
 int pamo_debug_guard(pamo_ctx* ctx, int mode) {

@@ -164,3 +164,25 @@ def test_pamo_simplify_host_abi(cuda):
         print("pamo_simplify", f.shape, st, flush=True)
         assert st["exit_reason"] in (3, 4)
         assert _same(hv, ov) and _same(hf, of)
+
+
+@pytest.mark.parametrize("guard", [0, 1, 2])
+def test_strict_guard_matches_oracle(cuda, guard):
+    """The strict guard on the device equals the oracle's, bit for bit, with
+    either candidate search; its output passes the exact verifier where the
+    reference's guard keeps intersections."""
+    from pamo_amd import _lib
+    from pamo_amd.verify import verify
+    cases = [synth.sdf_union(20, 80, 3, n_slab_pairs=6), synth.displaced_icosphere(4, 5)]
+    for v, f in cases:
+        verts, scale, _ = oracle.preprocess(v, f)
+        target = int(0.001 * f.shape[0])
+        ov, of, st = oracle.run(verts, f, scale, target, strict=True)
+        ctx = _lib.Context(0)
+        ctx.debug_guard(guard)
+        gv, gf, gst = simplify_stage2(torch.from_numpy(verts).to(cuda), torch.from_numpy(f).to(cuda), scale, target,
+                                      ctx=ctx, strict=True)
+        print("strict", f.shape, guard, st, gst["rounds"], gst["max_undo_passes"], flush=True)
+        assert gst["rounds"] == st["rounds"]
+        assert _same(gv.cpu().numpy(), ov) and _same(gf.cpu().numpy(), of)
+        assert verify(ov, of)["ok"]

@@ -508,3 +508,29 @@ def test_cost_twin_float_equals_oracle_and_fp64_bounds():
     # FP32 evaluation of the reference's expressions stays near FP64 for
     # most edges (the QEM term cancels near the surface, hence the tail)
     assert r["cost_rel_err_median"] < 0.05
+
+
+# ------------------------------------------------------- strict guard
+def test_strict_guard_removes_what_the_reference_keeps():
+    """A thin-slab union (32,436 faces): the reference's guard skips
+    coplanar pairs, shared-edge coordinates and short shared-vertex
+    contacts, and its output keeps exact intersections.  The strict guard
+    (exact pair test, every collapse scanned, undo until clean) leaves none,
+    and on a mesh where nothing is skipped it changes nothing."""
+    v, f = synth.sdf_union(20, 80, 3, n_slab_pairs=6)
+    assert pv.verify(v, f)["ok"]
+    verts, scale, _ = oracle.preprocess(v, f)
+    target = int(0.001 * len(f))
+    ov, of, st = oracle.run(verts, f, scale, target)
+    r = pv.verify(ov, of)
+    assert r["manifold"] and r["watertight"] and r["intersecting_pairs"] > 0
+    sv, sf, sst = oracle.run(verts, f, scale, target, strict=True)
+    rs = pv.verify(sv, sf)
+    assert rs["ok"], rs
+    assert sst["max_undo_passes"] < 64
+    # a star-shaped surface: the strict and the reference guard agree
+    v, f = synth.displaced_icosphere(4, 5)
+    verts, scale, _ = oracle.preprocess(v, f)
+    a = oracle.run(verts, f, scale, int(0.01 * len(f)))
+    b = oracle.run(verts, f, scale, int(0.01 * len(f)), strict=True)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
