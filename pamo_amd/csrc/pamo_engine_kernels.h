@@ -45,13 +45,17 @@ __device__ __forceinline__ int find_seg(const long long* __restrict__ off, int M
 // ------------------------------------------------------- step 1 (undo in)
 // clear the table of meshes that bring an undo list and are not stuck
 // (cusimp_free.cu:905-908); one block per mesh
-__global__ void k_clear_tables(const MeshParam* __restrict__ mp, const long long* __restrict__ uioff, int M,
-                               uint8_t* __restrict__ table) {
+// Only the entries of the mesh's current vertices are cleared: the table is
+// read at local vertex indices, which stay below the current vertex count
+// for the rest of the run (it never grows), so entries past it are never
+// read again.
+__global__ void k_clear_tables(const MeshParam* __restrict__ mp, const long long* __restrict__ uioff,
+                               const long long* __restrict__ voff, int M, uint8_t* __restrict__ table) {
   int k = blockIdx.y;
   if (k >= M) return;
   if (mp[k].stuck || uioff[k + 1] == uioff[k]) return;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < mp[k].pmcap;
-       i += (long long)gridDim.x * blockDim.x)
+  const long long n = min(voff[k + 1] - voff[k], mp[k].pmcap);
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     table[mp[k].pmo + i] = 0;
 }
 // every incoming undo entry marks its mesh's table (cusimp_free.cu:910-916);
@@ -222,7 +226,8 @@ __device__ __forceinline__ void quadric_add(float* q, float4 p) {
 // (face ids in a fan are distinct, so the rank of x is the count of smaller
 // entries), the lanes fill the slots in parallel, and one lane adds the
 // planes into the quadric serially in fan order.  Fans longer than FS_MAX
-// take the serial path on one lane.
+// rank from global memory instead: each lane fills the slot at its entry's
+// rank, and the sorted fan is read back from the slots' face indices.
 constexpr int FS_G = 8;
 constexpr int FS_MAX = 32;
 constexpr int FS_GROUPS = BLOCK / FS_G;
@@ -247,19 +252,37 @@ __global__ void __launch_bounds__(BLOCK) k_fan_slots(const float* __restrict__ P
   if (valid && small)
     for (int s = g; s < deg; s += FS_G) s_raw[grp][s] = fan[a + s];
   __syncthreads();
-  if (valid && small)
+  if (valid && small) {
     for (int s = g; s < deg; s += FS_G) {
       int x = s_raw[grp][s], r = 0;
       for (int j = 0; j < deg; j++) r += s_raw[grp][j] < x;
       s_sorted[grp][r] = x;
     }
-  __syncthreads();
-  if (valid && small)
+  } else if (valid) {
+    for (int s = g; s < deg; s += FS_G) {
+      int x = fan[a + s], r = 0, j = 0;
+      // eight loads in flight per step
+      for (; j + 8 <= deg; j += 8) {
+        int y[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) y[q] = fan[a + j + q];
+#pragma unroll
+        for (int q = 0; q < 8; q++) r += y[q] < x;
+      }
+      for (; j < deg; j++) r += fan[a + j] < x;
+      fan_slot_fill(P, T, v, x, a + r, fsucc, sface, splane, slen);
+    }
+  }
+  __syncthreads();   // also orders the long fans' slot writes before their reads below
+  if (valid && small) {
     for (int s = g; s < deg; s += FS_G) {
       int f = s_sorted[grp][s];
       fan[a + s] = f;
       s_plane[grp][s] = fan_slot_fill(P, T, v, f, a + s, fsucc, sface, splane, slen);
     }
+  } else if (valid) {
+    for (int s = g; s < deg; s += FS_G) fan[a + s] = sface[a + s].w;
+  }
   __syncthreads();
   if (!valid || g != 0) return;
   stp(P0, v, ldp(P, v));   // round-start position snapshot (the undo restore source)
@@ -267,14 +290,15 @@ __global__ void __launch_bounds__(BLOCK) k_fan_slots(const float* __restrict__ P
   if (small) {
     for (int s = 0; s < deg; s++) quadric_add(q, s_plane[grp][s]);
   } else {
-    const int b = a + deg;
-    for (int i = a + 1; i < b; i++) {
-      int x = fan[i];
-      int j = i - 1;
-      while (j >= a && fan[j] > x) { fan[j + 1] = fan[j]; j--; }
-      fan[j + 1] = x;
+    int s = a;
+    for (; s + 8 <= a + deg; s += 8) {
+      float4 pl[8];
+#pragma unroll
+      for (int j = 0; j < 8; j++) pl[j] = splane[s + j];
+#pragma unroll
+      for (int j = 0; j < 8; j++) quadric_add(q, pl[j]);
     }
-    for (int s = a; s < b; s++) quadric_add(q, fan_slot_fill(P, T, v, fan[s], s, fsucc, sface, splane, slen));
+    for (; s < a + deg; s++) quadric_add(q, splane[s]);
   }
 #pragma unroll
   for (int i = 0; i < 10; i++) Q[(long long)10 * v + i] = q[i];
@@ -293,19 +317,33 @@ struct EdgeScanOp {
   int* ucnt;
   int* erank;
   int* win;   // winner flags of the edges (and the entry past the last), cleared
-  __device__ int count(long long f) const {
-    if (f >= F) return 0;
-    I3 t = ldt(T, (int)f);
+  struct Rec { I3 t; int r0, r1, r2; };
+  // f = F (the entry past the last face) reads face max(F - 1, 0), which
+  // the buffer always holds, and takes it as dead
+  __device__ Rec load(long long f) const {
+    const I3 t = ldt(T, (int)max(min(f, (long long)F - 1), 0LL));
+    Rec r;
+    r.t = f < F ? t : I3{-1, -1, -1};
+    return r;
+  }
+  __device__ int count(const Rec& r) const {
+    const I3 t = r.t;
     return dead3(t) ? 0 : (int)(t.i > t.j) + (int)(t.j > t.k) + (int)(t.k > t.i);
   }
-  __device__ void emit(long long f, int o) const {
+  // ranks within the first endpoints (dead faces have no descending edge)
+  __device__ void gather(long long, Rec& r) const {
+    const I3 t = r.t;
+    if (t.i > t.j) r.r0 = atomicAdd(&ucnt[t.j], 1);
+    if (t.j > t.k) r.r1 = atomicAdd(&ucnt[t.k], 1);
+    if (t.k > t.i) r.r2 = atomicAdd(&ucnt[t.i], 1);
+  }
+  __device__ void emit(long long f, const Rec& r, int o) const {
     eoff[f] = o;
     if (f >= F) { win[o] = 0; return; }
-    I3 t = ldt(T, (int)f);
-    if (dead3(t)) return;
-    if (t.i > t.j) { edges[2 * o] = t.j; edges[2 * o + 1] = t.i; erank[o] = atomicAdd(&ucnt[t.j], 1); win[o++] = 0; }
-    if (t.j > t.k) { edges[2 * o] = t.k; edges[2 * o + 1] = t.j; erank[o] = atomicAdd(&ucnt[t.k], 1); win[o++] = 0; }
-    if (t.k > t.i) { edges[2 * o] = t.i; edges[2 * o + 1] = t.k; erank[o] = atomicAdd(&ucnt[t.i], 1); win[o++] = 0; }
+    const I3 t = r.t;
+    if (t.i > t.j) { edges[2 * o] = t.j; edges[2 * o + 1] = t.i; erank[o] = r.r0; win[o++] = 0; }
+    if (t.j > t.k) { edges[2 * o] = t.k; edges[2 * o + 1] = t.j; erank[o] = r.r1; win[o++] = 0; }
+    if (t.k > t.i) { edges[2 * o] = t.i; edges[2 * o + 1] = t.k; erank[o] = r.r2; win[o++] = 0; }
   }
 };
 // End-of-round vertex compaction: the scan of occupancy is the new index g,
@@ -318,12 +356,26 @@ struct VertCompactOp {
   const int* vmesh;
   float* outP;
   int* outm;
-  __device__ int count(long long v) const { return occ[v]; }
-  __device__ void emit(long long v, int o) const {
+  struct Rec { int o; bool keep; V3 p; int m; };
+  __device__ Rec load(long long v) const {
+    Rec r;
+    r.o = occ[v];
+    r.keep = false;
+    return r;
+  }
+  __device__ int count(const Rec& r) const { return r.o; }
+  __device__ void gather(long long v, Rec& r) const {
+    r.keep = v < *V_ptr && r.o;
+    if (r.keep) {
+      r.p = ldp(P, (int)v);
+      r.m = vmesh[v];
+    }
+  }
+  __device__ void emit(long long v, const Rec& r, int o) const {
     g[v] = o;
-    if (v >= *V_ptr || !occ[v]) return;
-    outP[3 * o] = P[3 * v]; outP[3 * o + 1] = P[3 * v + 1]; outP[3 * o + 2] = P[3 * v + 2];
-    outm[o] = vmesh[v];
+    if (!r.keep) return;
+    stp(outP, o, r.p);
+    outm[o] = r.m;
   }
 };
 // End-of-round face compaction: kept faces (not removed) move to the scan
@@ -336,12 +388,27 @@ struct FaceCompactOp {
   const int* fmesh;
   int* outT;
   int* outm;
-  __device__ int count(long long f) const { return f < *F_ptr && T[3 * f] >= 0 ? 1 : 0; }
-  __device__ void emit(long long f, int o) const {
+  struct Rec { I3 t; int m; };
+  // the same clamp as the edge list's
+  __device__ Rec load(long long f) const {
+    const long long Fd = *F_ptr;
+    const I3 t = ldt(T, (int)max(min(f, Fd - 1), 0LL));
+    Rec r;
+    r.t = f < Fd ? t : I3{-1, -1, -1};
+    return r;
+  }
+  __device__ int count(const Rec& r) const { return r.t.i >= 0 ? 1 : 0; }
+  // the corners renamed through g
+  __device__ void gather(long long f, Rec& r) const {
+    if (r.t.i < 0) return;
+    r.t = I3{g[r.t.i], g[r.t.j], g[r.t.k]};
+    r.m = fmesh[f];
+  }
+  __device__ void emit(long long f, const Rec& r, int o) const {
     kpos[f] = o;
-    if (f >= *F_ptr || T[3 * f] < 0) return;
-    outT[3 * o] = g[T[3 * f]]; outT[3 * o + 1] = g[T[3 * f + 1]]; outT[3 * o + 2] = g[T[3 * f + 2]];
-    outm[o] = fmesh[f];
+    if (r.t.i < 0) return;
+    outT[3 * o] = r.t.i; outT[3 * o + 1] = r.t.j; outT[3 * o + 2] = r.t.k;
+    outm[o] = r.m;
   }
 };
 
@@ -510,6 +577,18 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__
             u32sat(clamp_ref(cc + el + Qa / (float)nt * tau, 0.0f, COST_RANGE) / COST_RANGE * U32F), true);
 }
 
+// ------------------------------------------------ grouped fan walks
+// Kernels that walk the fans of one vertex or one collapse give each item a
+// group of FAN_G lanes: the group splits the fan slots and combines its
+// lanes' answers with shuffles, so the chain of dependent loads is
+// deg / FAN_G long instead of deg.
+constexpr int FAN_G = 8;
+__device__ __forceinline__ int fan_group_or(int x) {
+#pragma unroll
+  for (int o = 1; o < FAN_G; o <<= 1) x |= __shfl_xor(x, o, FAN_G);
+  return x;
+}
+
 // ---------------------------------------------------------------- claim
 // The reference takes, per alive face, the minimum key (cost << 32 | edge)
 // over every edge whose endpoint's fan holds the face (cusimp_free.cu:
@@ -532,7 +611,10 @@ __global__ void __launch_bounds__(BLOCK) k_winners_v(int V, const u64* __restric
                                                      const MeshParam* __restrict__ mp,
                                                      const int* __restrict__ fan_off,
                                                      const int4* __restrict__ sface, int* __restrict__ win) {
-  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  // a group of FAN_G lanes per vertex; every early return depends on u
+  // alone, so it is taken by the whole group
+  const int u = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) / FAN_G);
+  const int g = threadIdx.x & (FAN_G - 1);
   if (u >= V) return;
   const u64 key = vmin[u];
   if (key == ~0ull) return;   // no incident edge
@@ -540,15 +622,14 @@ __global__ void __launch_bounds__(BLOCK) k_winners_v(int V, const u64* __restric
   if (edges[2 * e] != u) return;
   const int v = edges[2 * e + 1];
   if (vmin[v] != key || (uint32_t)(key >> 32) > mp[vmesh[u]].ct) return;
-  bool ok = true;
-  for (int w = 0; w < 2 && ok; w++) {
-    int x = w == 0 ? u : v;
-    for (int s = fan_off[x]; s < fan_off[x + 1] && ok; s++) {
-      int4 t = sface[s];
-      ok = vmin[t.x] >= key && vmin[t.y] >= key && vmin[t.z] >= key;
-    }
+  // every corner of every face of both fans has its minimum at key or above
+  const int a0 = fan_off[u], n0 = fan_off[u + 1] - a0, a1 = fan_off[v], n = n0 + fan_off[v + 1] - a1;
+  int bad = 0;
+  for (int j = g; j < n; j += FAN_G) {
+    const int4 t = sface[j < n0 ? a0 + j : a1 + (j - n0)];
+    bad |= (int)(vmin[t.x] < key) | (int)(vmin[t.y] < key) | (int)(vmin[t.z] < key);
   }
-  if (ok) win[e] = 1;
+  if (!fan_group_or(bad) && g == 0) win[e] = 1;
 }
 // The winners from the dense list (ascending edge order), one group of
 // CL_G lanes each: the entry in the mesh's collapsed region, then the
@@ -741,16 +822,17 @@ __global__ void __launch_bounds__(BLOCK) k_line_cut_tail(int M, const int* __res
 __global__ void k_qflag_b(const long long* __restrict__ cutoff, int M, const long long* __restrict__ eoffm,
                           const int* __restrict__ coll, const int* __restrict__ edges,
                           const int* __restrict__ fan_off, const int* __restrict__ fan,
-                          const int* __restrict__ part, uint8_t* qflag) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+                          const int* __restrict__ part, uint8_t* __restrict__ qflag) {
+  // a group of FAN_G lanes per collapse takes the two fans' slots in turn
+  const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / FAN_G;
+  const int g = threadIdx.x & (FAN_G - 1);
   if (i >= cutoff[M]) return;
   int k = find_seg(cutoff, M, i);
   if (!part[k]) return;
   int e = coll[2 * eoffm[k] + (i - cutoff[k])];
-  for (int w = 0; w < 2; w++) {
-    int x = edges[2 * e + w];
-    for (int s = fan_off[x]; s < fan_off[x + 1]; s++) qflag[fan[s]] = 1;
-  }
+  const int u = edges[2 * e], v = edges[2 * e + 1];
+  const int a0 = fan_off[u], n0 = fan_off[u + 1] - a0, a1 = fan_off[v], n = n0 + fan_off[v + 1] - a1;
+  for (int j = g; j < n; j += FAN_G) qflag[fan[j < n0 ? a0 + j : a1 + (j - n0)]] = 1;
 }
 __global__ void k_query_flags_b(const uint8_t* __restrict__ qflag, const int* __restrict__ T,
                                 const int* __restrict__ F_ptr, const int* __restrict__ fmesh,
@@ -812,15 +894,9 @@ __global__ void k_scene_init_b(int* scene, int M) {
 }
 // 64-bit keys: mesh index above the 30-bit Morton code of the box centre in
 // the mesh's admission box
-__global__ void k_morton_b(const float4* __restrict__ lbox_lo, const float4* __restrict__ lbox_hi,
-                           const int* __restrict__ n_ptr, const int* __restrict__ leaf_face,
-                           const int* __restrict__ fmesh, const MeshParam* __restrict__ mp, u64* __restrict__ keys,
-                           int* __restrict__ vals) {
-  int l = blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= *n_ptr) return;
-  int k = fmesh[leaf_face[l]];
+// mesh id above the 30-bit Morton code of the box centre in the mesh's box
+__device__ __forceinline__ u64 mesh_morton_key(int k, const MeshParam* __restrict__ mp, float4 lo, float4 hi) {
   const float* box = mp[k].box;
-  float4 lo = lbox_lo[l], hi = lbox_hi[l];
   float c[3] = {0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
   uint32_t q[3];
 #pragma unroll
@@ -830,9 +906,102 @@ __global__ void k_morton_b(const float4* __restrict__ lbox_lo, const float4* __r
     t = fminf(fmaxf(t * 1024.0f, 0.0f), 1023.0f);
     q[a] = (uint32_t)t;
   }
-  keys[l] = ((u64)(uint32_t)k << 30) | (u64)(expand10(q[0]) * 4 + expand10(q[1]) * 2 + expand10(q[2]));
+  return ((u64)(uint32_t)k << 30) | (u64)(expand10(q[0]) * 4 + expand10(q[1]) * 2 + expand10(q[2]));
+}
+__global__ void k_morton_b(const float4* __restrict__ lbox_lo, const float4* __restrict__ lbox_hi,
+                           const int* __restrict__ n_ptr, const int* __restrict__ leaf_face,
+                           const int* __restrict__ fmesh, const MeshParam* __restrict__ mp, u64* __restrict__ keys,
+                           int* __restrict__ vals) {
+  int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= *n_ptr) return;
+  keys[l] = mesh_morton_key(fmesh[leaf_face[l]], mp, lbox_lo[l], lbox_hi[l]);
   vals[l] = l;
 }
+// the face's box (the expressions of k_leaf_boxes_b)
+__device__ __forceinline__ void face_box(const float* __restrict__ P, I3 t, float4& lo, float4& hi) {
+  V3 a = ldp(P, t.i), b = ldp(P, t.j), c = ldp(P, t.k);
+  lo = make_float4(fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)), fminf(a.z, fminf(b.z, c.z)), 0.0f);
+  hi = make_float4(fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)), fmaxf(a.z, fmaxf(b.z, c.z)), 0.0f);
+}
+
+// ------------------------------------------------- spatial face order
+// The exhaustive search walks each mesh's faces in a spatial (Morton) order,
+// so the faces of a tile group, and the queries of a wave, are neighbours
+// and most group tests miss.  The order (sorder: every face slot once,
+// mesh-major) is sorted from the round-start snapshot the first time a
+// check needs it, and then carried through each round's face compaction,
+// so it stays spatial without further sorts.  Any order gives the same
+// candidate set; the order only decides how much the culling saves.
+__global__ void k_sorder_keys(const int* __restrict__ T0, const float* __restrict__ P0, int F,
+                              const int* __restrict__ fmesh, const MeshParam* __restrict__ mp, u64* __restrict__ keys,
+                              int* __restrict__ vals) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  const I3 t = ldt(T0, f);   // every face of the snapshot is alive
+  float4 lo, hi;
+  face_box(P0, t, lo, hi);
+  keys[f] = mesh_morton_key(fmesh[f], mp, lo, hi);
+  vals[f] = f;
+}
+// per position of the order: the face's box and index if it takes part in
+// the check (alive, in a participating mesh), else an empty box and -1
+__global__ void __launch_bounds__(BLOCK) k_sorder_boxes(const int* __restrict__ sorder, int F,
+                                                        const int* __restrict__ lf, const float* __restrict__ P,
+                                                        const int* __restrict__ T, int* __restrict__ sid,
+                                                        float4* __restrict__ slo, float4* __restrict__ shi) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= F) return;
+  const int f = sorder[p];
+  float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.0f), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+  const bool in = lf[f] != 0;
+  if (in) face_box(P, ldt(T, f), lo, hi);
+  sid[p] = in ? f : -1;
+  slo[p] = lo;
+  shi[p] = hi;
+}
+// queries (flagged faces) listed in the order; qpos per position
+struct SorderQueryOp {
+  const int* sorder;
+  int F;   // >= 1
+  const int* qf;
+  int* qpos;
+  int* qlist;
+  struct Rec { int f, q; };
+  __device__ Rec load(long long p) const {
+    const int f = sorder[min(p, (long long)F - 1)];
+    Rec r;
+    r.f = f;
+    r.q = p < F ? qf[f] : 0;
+    return r;
+  }
+  __device__ int count(const Rec& r) const { return r.q; }
+  __device__ void gather(long long, Rec&) const {}
+  __device__ void emit(long long p, const Rec& r, int pos) const {
+    qpos[p] = pos;
+    if (r.q) qlist[pos] = r.f;
+  }
+};
+// the order through the round's face compaction: kept faces, renamed
+struct SorderCompactOp {
+  const int* sorder;
+  int F;   // >= 1
+  const int* kpos;
+  int* out;
+  struct Rec { int keep, to; };
+  __device__ Rec load(long long p) const {
+    const int f = sorder[min(p, (long long)F - 1)];
+    const int a = kpos[f], b = kpos[f + 1];
+    Rec r;
+    r.keep = p < F ? b - a : 0;
+    r.to = a;
+    return r;
+  }
+  __device__ int count(const Rec& r) const { return r.keep; }
+  __device__ void gather(long long, Rec&) const {}
+  __device__ void emit(long long, const Rec& r, int pos) const {
+    if (r.keep) out[pos] = r.to;
+  }
+};
 
 // queries in Morton order: flag per sorted leaf position, then compaction
 __global__ void k_query_sorted(const int* __restrict__ sorted_leaf, const int* __restrict__ leaf_face,
@@ -1343,13 +1512,41 @@ __global__ void __launch_bounds__(BLOCK) k_pair_test(const int2* __restrict__ pa
 // every query against every alive face of its mesh.  The candidate set is
 // the same (alive faces g != f whose float AABB overlaps the query's,
 // touching included, with the boxes of k_leaf_boxes_b), so the flags are
-// the same.  Queries are listed in face order, which groups them by mesh;
+// the same.  Faces and queries are taken in the spatial order (positions
+// foff[k] .. foff[k+1] hold mesh k; dead faces have empty boxes and id -1);
 // each mesh gets ceil(queries/BLOCK) query blocks (bx), and grid row by
-// covers alive faces [by*BF_TILE, (by+1)*BF_TILE) of the block's mesh.  The
+// covers positions [by*BF_TILE, (by+1)*BF_TILE) of the block's mesh.  The
 // tile's boxes sit in LDS and every lane of a wave reads the same box at the
 // same time.  Candidates go through the same LDS lists, flushes and pair
 // buffer as the walk.
+//
+// Culling: the tile is cut into groups of BF_GRP consecutive boxes, each
+// with the union of its boxes, and a wave skips a group when the union of
+// its lanes' query boxes misses the group's union.  The unions contain
+// every member box, so no overlapping pair is skipped; a box or query with
+// a NaN coordinate (which the overlap test never separates) makes its union
+// unbounded.
 constexpr int BF_TILE = 256;
+constexpr int BF_GRP = 16;
+static_assert(BF_TILE == BLOCK, "one tile box per thread at load");
+__device__ __forceinline__ bool box_has_nan(float4 lo, float4 hi) {
+  return (lo.x != lo.x) | (lo.y != lo.y) | (lo.z != lo.z) | (hi.x != hi.x) | (hi.y != hi.y) | (hi.z != hi.z);
+}
+// union of the box over `width` consecutive lanes (width a power of two <= 64)
+__device__ __forceinline__ void lane_union(float& l0, float& l1, float& l2, float& h0, float& h1, float& h2,
+                                           int width) {
+  for (int o = 1; o < width; o <<= 1) {
+    l0 = fminf(l0, __shfl_xor(l0, o, 64));
+    l1 = fminf(l1, __shfl_xor(l1, o, 64));
+    l2 = fminf(l2, __shfl_xor(l2, o, 64));
+    h0 = fmaxf(h0, __shfl_xor(h0, o, 64));
+    h1 = fmaxf(h1, __shfl_xor(h1, o, 64));
+    h2 = fmaxf(h2, __shfl_xor(h2, o, 64));
+  }
+}
+__device__ __forceinline__ float wave_uniform(float x) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
 __global__ void __launch_bounds__(BLOCK) k_bf_blocks(int M, const long long* __restrict__ foff,
                                                      const int* __restrict__ qpos, int* __restrict__ nb,
                                                      long long* __restrict__ boff) {
@@ -1361,31 +1558,41 @@ __global__ void __launch_bounds__(BLOCK) k_bf_blocks(int M, const long long* __r
   block_small_scan(nb, M, boff, 1, nullptr, 0);   // query blocks -> block offsets per mesh
 }
 __global__ void __launch_bounds__(BLOCK) k_bf_emit(
-    const int* __restrict__ qlist, const int* __restrict__ qpos, const int* __restrict__ apos,
-    const long long* __restrict__ foff, const long long* __restrict__ boff, int M,
-    const int* __restrict__ leaf_face, const float4* __restrict__ lbox_lo, const float4* __restrict__ lbox_hi,
-    const float* __restrict__ P, const int* __restrict__ T, int2* __restrict__ pairs, long long pair_cap,
-    unsigned long long* __restrict__ npairs, int* __restrict__ ovf, int* __restrict__ novf, int* __restrict__ oflag,
-    unsigned long long* __restrict__ stats) {
+    const int* __restrict__ qlist, const int* __restrict__ qpos, const long long* __restrict__ foff,
+    const long long* __restrict__ boff, int M, const int* __restrict__ sid, const float4* __restrict__ slo,
+    const float4* __restrict__ shi, const float* __restrict__ P, const int* __restrict__ T, int2* __restrict__ pairs,
+    long long pair_cap, unsigned long long* __restrict__ npairs, int* __restrict__ ovf, int* __restrict__ novf,
+    int* __restrict__ oflag, unsigned long long* __restrict__ stats) {
   __shared__ int cl[CAND_CAP * BLOCK];
   __shared__ float4 blo[BF_TILE], bhi[BF_TILE];
   __shared__ int bid[BF_TILE];
+  __shared__ float4 glo[BF_TILE / BF_GRP], ghi[BF_TILE / BF_GRP];
   const int t = threadIdx.x;
   const int lane = t & 63;
   const long long bx = blockIdx.x;
   if (bx >= boff[M]) return;                     // block-uniform exits
   const int k = find_seg(boff, M, bx);
-  const long long a0 = apos[foff[k]], a1 = apos[foff[k + 1]];
+  const long long a0 = foff[k], a1 = foff[k + 1];
   const long long tile0 = a0 + (long long)blockIdx.y * BF_TILE;
   if (tile0 >= a1) return;
   const int tn = (int)min((long long)BF_TILE, a1 - tile0);
-  for (int j = t; j < BF_TILE; j += BLOCK) {
-    if (j < tn) {
-      blo[j] = lbox_lo[tile0 + j];
-      bhi[j] = lbox_hi[tile0 + j];
-      bid[j] = leaf_face[tile0 + j];
-    } else {
-      bid[j] = -1;   // padding: never a candidate
+  {
+    float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.f), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+    int id = -1;   // padding: never a candidate, empty in the union
+    if (t < tn) {
+      lo = slo[tile0 + t];
+      hi = shi[tile0 + t];
+      id = sid[tile0 + t];
+    }
+    blo[t] = lo;
+    bhi[t] = hi;
+    bid[t] = id;
+    float l0 = lo.x, l1 = lo.y, l2 = lo.z, h0 = hi.x, h1 = hi.y, h2 = hi.z;
+    if (box_has_nan(lo, hi)) l0 = l1 = l2 = -INFINITY, h0 = h1 = h2 = INFINITY;
+    lane_union(l0, l1, l2, h0, h1, h2, BF_GRP);
+    if ((t & (BF_GRP - 1)) == 0) {
+      glo[t / BF_GRP] = make_float4(l0, l1, l2, 0.f);
+      ghi[t / BF_GRP] = make_float4(h0, h1, h2, 0.f);
     }
   }
   __syncthreads();
@@ -1395,44 +1602,58 @@ __global__ void __launch_bounds__(BLOCK) k_bf_emit(
   int cnt = 0;
   int f = -1;
   bool full = false;
+  float qlo0 = INFINITY, qlo1 = INFINITY, qlo2 = INFINITY, qhi0 = -INFINITY, qhi1 = -INFINITY, qhi2 = -INFINITY;
   if (qi < q1) {
     f = qlist[qi];
     I3 tf = ldt(T, f);
     V3 a = ldp(P, tf.i), b = ldp(P, tf.j), c = ldp(P, tf.k);
-    const float qlo0 = fminf(a.x, fminf(b.x, c.x)), qhi0 = fmaxf(a.x, fmaxf(b.x, c.x));
-    const float qlo1 = fminf(a.y, fminf(b.y, c.y)), qhi1 = fmaxf(a.y, fmaxf(b.y, c.y));
-    const float qlo2 = fminf(a.z, fminf(b.z, c.z)), qhi2 = fmaxf(a.z, fmaxf(b.z, c.z));
+    qlo0 = fminf(a.x, fminf(b.x, c.x)), qhi0 = fmaxf(a.x, fmaxf(b.x, c.x));
+    qlo1 = fminf(a.y, fminf(b.y, c.y)), qhi1 = fmaxf(a.y, fmaxf(b.y, c.y));
+    qlo2 = fminf(a.z, fminf(b.z, c.z)), qhi2 = fmaxf(a.z, fmaxf(b.z, c.z));
+  }
+  // the wave's query union, in scalar registers
+  float w0 = qlo0, w1 = qlo1, w2 = qlo2, x0 = qhi0, x1 = qhi1, x2 = qhi2;
+  if ((w0 != w0) | (w1 != w1) | (w2 != w2) | (x0 != x0) | (x1 != x1) | (x2 != x2))
+    w0 = w1 = w2 = -INFINITY, x0 = x1 = x2 = INFINITY;
+  lane_union(w0, w1, w2, x0, x1, x2, 64);
+  w0 = wave_uniform(w0), w1 = wave_uniform(w1), w2 = wave_uniform(w2);
+  x0 = wave_uniform(x0), x1 = wave_uniform(x1), x2 = wave_uniform(x2);
+  if (qi < q1) {
     // the six comparisons of the overlap test, evaluated without short
     // circuits (they have no side effects) so the box loads are not
     // serialised behind branches; four boxes per step.  The append is
     // predicated: every box writes the lane's next list slot and only a hit
     // advances the count, so the step has no branch; the list is moved to
     // the pair buffer before a step that could overrun it.
-    const int tn4 = (tn + 3) & ~3;
-    for (int j0 = 0; j0 < tn4; j0 += 4) {
-      if (cnt > CAND_CAP - 4) {
-        long long b0 = (long long)atomicAdd(npairs, (unsigned long long)cnt);
-        for (int q = 0; q < cnt; q++)
-          if (b0 + q < pair_cap) pairs[b0 + q] = make_int2(f, cl[q * BLOCK + t]);
-        if (b0 + cnt > pair_cap) { full = true; break; }
-        cnt = 0;
-      }
-      unsigned sep[4];
-      int gg[4];
+    const int ng = (tn + BF_GRP - 1) / BF_GRP;
+    for (int gi = 0; gi < ng && !full; gi++) {
+      const float4 gl = glo[gi], gh = ghi[gi];
+      if ((x0 < gl.x) | (gh.x < w0) | (x1 < gl.y) | (gh.y < w1) | (x2 < gl.z) | (gh.z < w2)) continue;
+      for (int j0 = gi * BF_GRP; j0 < gi * BF_GRP + BF_GRP; j0 += 4) {
+        if (cnt > CAND_CAP - 4) {
+          long long b0 = (long long)atomicAdd(npairs, (unsigned long long)cnt);
+          for (int q = 0; q < cnt; q++)
+            if (b0 + q < pair_cap) pairs[b0 + q] = make_int2(f, cl[q * BLOCK + t]);
+          if (b0 + cnt > pair_cap) { full = true; break; }
+          cnt = 0;
+        }
+        unsigned sep[4];
+        int gg[4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        float4 lo = blo[j0 + u], hi = bhi[j0 + u];
-        gg[u] = bid[j0 + u];
-        sep[u] = (unsigned)(qhi0 < lo.x) | (unsigned)(hi.x < qlo0) | (unsigned)(qhi1 < lo.y) |
-                 (unsigned)(hi.y < qlo1) | (unsigned)(qhi2 < lo.z) | (unsigned)(hi.z < qlo2);
-      }
+        for (int u = 0; u < 4; u++) {
+          float4 lo = blo[j0 + u], hi = bhi[j0 + u];
+          gg[u] = bid[j0 + u];
+          sep[u] = (unsigned)(qhi0 < lo.x) | (unsigned)(hi.x < qlo0) | (unsigned)(qhi1 < lo.y) |
+                   (unsigned)(hi.y < qlo1) | (unsigned)(qhi2 < lo.z) | (unsigned)(hi.z < qlo2);
+        }
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int g = gg[u];
-        const int hit = (int)(sep[u] == 0u) & (int)(g >= 0) & (int)(g != f);
-        cl[cnt * BLOCK + t] = g;
-        cnt += hit;
-        cand += (unsigned)hit;
+        for (int u = 0; u < 4; u++) {
+          const int g = gg[u];
+          const int hit = (int)(sep[u] == 0u) & (int)(g >= 0) & (int)(g != f);
+          cl[cnt * BLOCK + t] = g;
+          cnt += hit;
+          cand += (unsigned)hit;
+        }
       }
     }
   }
@@ -1482,26 +1703,24 @@ struct BfFull {
   const int* ovf;
   const int* novf;
   const int* fmesh;
-  const int* apos;
   const long long* foff;
-  const int* leaf_face;
-  const float4* lbox_lo;
-  const float4* lbox_hi;
+  const int* sid;        // by position of the spatial order (-1: not a candidate)
+  const float4* slo;
+  const float4* shi;
 };
 template <int STRICT>
 __device__ void bf_full_query(long long qi, const BfFull& a, const float* __restrict__ P, const int* __restrict__ T,
                               uint8_t* __restrict__ S) {
   const int* __restrict__ ovf = a.ovf;
   const int* __restrict__ fmesh = a.fmesh;
-  const int* __restrict__ apos = a.apos;
   const long long* __restrict__ foff = a.foff;
-  const int* __restrict__ leaf_face = a.leaf_face;
-  const float4* __restrict__ lbox_lo = a.lbox_lo;
-  const float4* __restrict__ lbox_hi = a.lbox_hi;
+  const int* __restrict__ sid = a.sid;
+  const float4* __restrict__ slo = a.slo;
+  const float4* __restrict__ shi = a.shi;
   if (qi >= *a.novf) return;
   int f = ovf[qi];
   int k = fmesh[f];
-  long long a0 = apos[foff[k]], a1 = apos[foff[k + 1]];
+  long long a0 = foff[k], a1 = foff[k + 1];
   I3 tf = ldt(T, f);
   float pf[3][3];
   {
@@ -1518,10 +1737,10 @@ __device__ void bf_full_query(long long qi, const BfFull& a, const float* __rest
   }
   bool hit = false;
   for (long long l = a0; l < a1 && !hit; l++) {
-    float4 lo = lbox_lo[l], hi = lbox_hi[l];
+    float4 lo = slo[l], hi = shi[l];
     if (qhi[0] < lo.x || hi.x < qlo[0] || qhi[1] < lo.y || hi.y < qlo[1] || qhi[2] < lo.z || hi.z < qlo[2]) continue;
-    int g = leaf_face[l];
-    if (g == f) continue;
+    int g = sid[l];
+    if (g < 0 || g == f) continue;
     I3 tg = ldt(T, g);
     float pg[3][3];
     V3 a = ldp(P, tg.i), b = ldp(P, tg.j), c = ldp(P, tg.k);
@@ -1548,21 +1767,27 @@ __global__ void k_mark_b(const long long* __restrict__ cutoff, int M, const long
                          const int* __restrict__ coll, const int* __restrict__ edges,
                          const int* __restrict__ fan_off, const int* __restrict__ fan, const int* __restrict__ part,
                          const uint8_t* __restrict__ S, int* __restrict__ mflag, int* __restrict__ total) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  // a group of FAN_G lanes per collapse: hit if a face of either fan is flagged
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long i = gid / FAN_G;
+  const int g = threadIdx.x & (FAN_G - 1);
   long long tot = cutoff[M];
-  if (i == 0) *total = (int)tot;   // live length of the scan of the marks
+  if (gid == 0) *total = (int)tot;   // live length of the scan of the marks
   if (i > tot) return;
-  if (i == tot) { mflag[tot] = 0; return; }
+  if (i == tot) {
+    if (g == 0) mflag[tot] = 0;
+    return;
+  }
   int k = find_seg(cutoff, M, i);
   int hit = 0;
   if (part[k]) {
     int e = coll[2 * eoffm[k] + (i - cutoff[k])];
-    for (int w = 0; w < 2 && !hit; w++) {
-      int x = edges[2 * e + w];
-      for (int s = fan_off[x]; s < fan_off[x + 1]; s++) if (S[fan[s]]) { hit = 1; break; }
-    }
+    const int u = edges[2 * e], v = edges[2 * e + 1];
+    const int a0 = fan_off[u], n0 = fan_off[u + 1] - a0, a1 = fan_off[v], n = n0 + fan_off[v + 1] - a1;
+    for (int j = g; j < n; j += FAN_G) hit |= (int)(S[fan[j < n0 ? a0 + j : a1 + (j - n0)]] != 0);
   }
-  mflag[i] = hit;
+  hit = fan_group_or(hit);
+  if (g == 0) mflag[i] = hit;
 }
 // marked entries -> per-mesh marked list (flat, base cutoff[k]) and endpoint
 // pairs (per mesh, base 2 cutoff[k]); blocks from nb0 on count marks per
@@ -1603,13 +1828,28 @@ __global__ void k_undo_b(int nbu, const long long* __restrict__ cutoff, int M, c
                          const int* __restrict__ pm, const long long* __restrict__ uobase,
                          const int* __restrict__ first, int* __restrict__ undo_out, int* err) {
   if ((int)blockIdx.x < nbu) {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    // a group of FAN_G lanes per marked collapse (restore_edge with the
+    // fan slots split over the group; every restorer of a face writes the
+    // same snapshot)
+    long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / FAN_G;
+    const int g = threadIdx.x & (FAN_G - 1);
     if (i >= cutoff[M]) return;
     int k = find_seg(cutoff, M, i);
     long long j = i - cutoff[k];
     if (j >= hm[k]) return;
     int e = marked[cutoff[k] + j];
-    restore_edge(edges[2 * e], edges[2 * e + 1], P, P0, T, T0, occ, fan_off, fan);
+    const int u = edges[2 * e], v = edges[2 * e + 1];
+    if (g == 0) {
+      occ[v] = 1;
+      stp(P, u, ldp(P0, u));
+      stp(P, v, ldp(P0, v));
+    }
+    const int a0 = fan_off[u], n0 = fan_off[u + 1] - a0, a1 = fan_off[v], n = n0 + fan_off[v + 1] - a1;
+    for (int q = g; q < n; q += FAN_G) {
+      const int f = fan[q < n0 ? a0 + q : a1 + (q - n0)];
+      const I3 t0 = ldt(T0, f);
+      T[3 * f] = t0.i; T[3 * f + 1] = t0.j; T[3 * f + 2] = t0.k;
+    }
     return;
   }
   long long i = (long long)(blockIdx.x - nbu) * blockDim.x + threadIdx.x;

@@ -216,9 +216,13 @@ struct pamo_ctx {
     return scan_op(ScanArray{in, out, list}, n, d_n, add, total);
   }
   // one-pass scan whose input and output are the op's count and emit
-  template <class Op> bool scan_op(const Op& op, long long n, const int* d_n, int add, int* total) {
+  // (inputs up to small_max take the small tile shape)
+  template <class Op>
+  bool scan_op(const Op& op, long long n, const int* d_n, int add, int* total, long long small_max = SCAN_SMALL_MAX) {
     if (n <= 0) return true;
-    long long nb = (n + SCAN1_TILE - 1) / SCAN1_TILE;
+    const bool small = n <= small_max;
+    const long long tile = small ? SCANS_TILE : SCAN1_TILE;
+    long long nb = (n + tile - 1) / tile;
     if (scan_desc.bytes < (size_t)nb * 8) {
       if (!ensure(scan_desc, (size_t)nb * 8)) return false;
       if (!hip(hipMemsetAsync(scan_desc.p, 0, scan_desc.bytes, st), "scan desc")) return false;
@@ -227,8 +231,12 @@ struct pamo_ctx {
       scan_epoch = 1;
       if (!hip(hipMemsetAsync(scan_desc.p, 0, scan_desc.bytes, st), "scan desc")) return false;
     }
-    k_scan_op<Op><<<(unsigned)nb, BLOCK1K, 0, st>>>(op, n, d_n, add, ptr<unsigned long long>(scan_desc), scan_epoch,
-                                                    total, cnt_dev(C_ERR));
+    if (small)
+      k_scan_op<Op, SCANS_NT, SCANS_ITEMS><<<(unsigned)nb, SCANS_NT, 0, st>>>(
+          op, n, d_n, add, ptr<unsigned long long>(scan_desc), scan_epoch, total, cnt_dev(C_ERR));
+    else
+      k_scan_op<Op><<<(unsigned)nb, BLOCK1K, 0, st>>>(op, n, d_n, add, ptr<unsigned long long>(scan_desc), scan_epoch,
+                                                      total, cnt_dev(C_ERR));
     return check_launch("scan");
   }
   bool read_counters() {

@@ -87,9 +87,10 @@ __device__ __forceinline__ int block_exclusive_scan(int x, int* lds, int& total)
   return off + v - x;
 }
 
-// the same for a block of BLOCK1K threads (unsigned values)
+// the same for a block of NT threads (unsigned values)
 constexpr int BLOCK1K = 1024;
-__device__ __forceinline__ unsigned block_exclusive_scan_1024(unsigned x, unsigned* lds, unsigned& total) {
+template <int NT>
+__device__ __forceinline__ unsigned block_exclusive_scan_nt(unsigned x, unsigned* lds, unsigned& total) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   unsigned v = x;
 #pragma unroll
@@ -101,7 +102,7 @@ __device__ __forceinline__ unsigned block_exclusive_scan_1024(unsigned x, unsign
   __syncthreads();
   unsigned off = 0, tot = 0;
 #pragma unroll
-  for (int w = 0; w < BLOCK1K / 64; w++) {
+  for (int w = 0; w < NT / 64; w++) {
     unsigned s = lds[w];
     if (w < wid) off += s;
     tot += s;
@@ -110,10 +111,12 @@ __device__ __forceinline__ unsigned block_exclusive_scan_1024(unsigned x, unsign
   total = tot;
   return off + v - x;
 }
-
+__device__ __forceinline__ unsigned block_exclusive_scan_1024(unsigned x, unsigned* lds, unsigned& total) {
+  return block_exclusive_scan_nt<BLOCK1K>(x, lds, total);
+}
 
 // ------------------------------------------------ one-pass exclusive scan
-// Decoupled look-back: tile t (one block, SCAN1_TILE ints) scans itself,
+// Decoupled look-back: tile t (one block, NT x ITEMS ints) scans itself,
 // publishes its aggregate, adds the aggregates/prefixes of earlier tiles
 // and publishes its inclusive prefix.  A tile's status and value travel in
 // one 64-bit word written and read with relaxed device-scope atomics, so no
@@ -125,41 +128,72 @@ __device__ __forceinline__ unsigned block_exclusive_scan_1024(unsigned x, unsign
 // done; the spin is capped anyway and reports through err.  Inputs are
 // non-negative (counts and flags); totals stay below 2^40.
 constexpr unsigned long long SCAN_ST_AGG = 1ull, SCAN_ST_PREFIX = 2ull;
+// Tile shapes (threads x items per thread).  A tile is one workgroup on one
+// CU, so short inputs take small tiles to spread over more CUs; long ones
+// take large tiles for fewer look-back steps.  Measured on the MI355X (edge
+// list of 16k faces: 7.9 us with 512x4 tiles, 18.8 us with 1024x8; at 2M
+// faces both 77 us; plain array scans gain only below 64k items).
 constexpr int SCAN1_ITEMS = 8, SCAN1_TILE = BLOCK1K * SCAN1_ITEMS;   // 8192 ints per tile
+constexpr int SCANS_NT = 512, SCANS_ITEMS = 4, SCANS_TILE = SCANS_NT * SCANS_ITEMS;   // 2048
+constexpr long long SCAN_SMALL_MAX = 65536;        // small tiles up to this length ...
+constexpr long long SCAN_SMALL_MAX_HEAVY = 1LL << 20;   // ... or this, for ops with gathers and stores per item
 __device__ __forceinline__ unsigned long long scan_word(unsigned epoch, unsigned long long st, long long v) {
   return ((unsigned long long)epoch << 42) | (st << 40) | ((unsigned long long)v & ((1ull << 40) - 1));
 }
-// The scan takes its input from op.count(i) and hands each exclusive prefix
-// to op.emit(i, pos), so a producer (counts from faces) and a consumer (a
-// scatter to the prefix position) can share the scan's launch.
-template <class Op>
-__global__ void __launch_bounds__(BLOCK1K) k_scan_op(Op op, long long n, const int* __restrict__ d_n, int add,
-                                                     unsigned long long* desc, unsigned epoch,
-                                                     int* __restrict__ total, int* err) {
-  __shared__ unsigned lds[BLOCK1K / 64];
-  __shared__ int tile[SCAN1_TILE];
+// The scan takes its input from the op and hands each exclusive prefix to
+// op.emit, so a producer (counts from faces) and a consumer (a scatter to
+// the prefix position) can share the scan's launch.  Per item the op
+// provides
+//   Rec load(i)             the item's record (its loads)
+//   int count(rec)          the value scanned
+//   void gather(i, rec)     further loads and atomics of the emit
+//   void emit(i, rec, pos)  the stores
+// The kernel runs each phase for all of a thread's items before the next,
+// so no item's loads wait behind another item's stores (which the compiler
+// must assume may alias them); the gathers are issued before the
+// look-back and overlap it.
+template <class Op, int NT = BLOCK1K, int ITEMS = SCAN1_ITEMS>
+__global__ void __launch_bounds__(NT) k_scan_op(Op op, long long n, const int* __restrict__ d_n, int add,
+                                                unsigned long long* desc, unsigned epoch, int* __restrict__ total,
+                                                int* err) {
+  __shared__ unsigned lds[NT / 64];
+  constexpr int TILE = NT * ITEMS;
+  __shared__ int tile[TILE];
   __shared__ long long excl_s;
   const long long live = d_n ? min(n, (long long)*d_n + add) : n;
   const long long t = blockIdx.x;
-  const long long base = t * SCAN1_TILE;
+  const long long base = t * TILE;
   if (live <= 0) {
     if (t == 0 && threadIdx.x == 0 && total) *total = 0;
     return;
   }
   if (base >= live) return;   // tiles past the live length contribute nothing
-  // unrolled so the items' loads are in flight together
+  typename Op::Rec rec[ITEMS];
+  // unrolled and branch-free (items past the end load the last live item
+  // and count 0), so the items' loads are in flight together
 #pragma unroll
-  for (int i = 0; i < SCAN1_ITEMS; i++) {
-    long long idx = base + (long long)i * BLOCK1K + threadIdx.x;
-    tile[i * BLOCK1K + threadIdx.x] = idx < live ? op.count(idx) : 0;
+  for (int i = 0; i < ITEMS; i++) {
+    long long idx = base + (long long)i * NT + threadIdx.x;
+    rec[i] = op.load(idx < live ? idx : live - 1);
+  }
+#pragma unroll
+  for (int i = 0; i < ITEMS; i++) {
+    long long idx = base + (long long)i * NT + threadIdx.x;
+    const int cnt = op.count(rec[i]);
+    tile[i * NT + threadIdx.x] = idx < live ? cnt : 0;
+  }
+#pragma unroll
+  for (int i = 0; i < ITEMS; i++) {
+    long long idx = base + (long long)i * NT + threadIdx.x;
+    if (idx < live) op.gather(idx, rec[i]);
   }
   __syncthreads();
-  int loc[SCAN1_ITEMS];
+  int loc[ITEMS];
   int s = 0;
 #pragma unroll
-  for (int i = 0; i < SCAN1_ITEMS; i++) { loc[i] = s; s += tile[threadIdx.x * SCAN1_ITEMS + i]; }
+  for (int i = 0; i < ITEMS; i++) { loc[i] = s; s += tile[threadIdx.x * ITEMS + i]; }
   unsigned utot;
-  int off = (int)block_exclusive_scan_1024((unsigned)s, lds, utot);
+  int off = (int)block_exclusive_scan_nt<NT>((unsigned)s, lds, utot);
   const int tot = (int)utot;
   if (threadIdx.x == 0) {
     long long ex = 0;
@@ -186,17 +220,17 @@ __global__ void __launch_bounds__(BLOCK1K) k_scan_op(Op op, long long n, const i
       }
     }
     excl_s = ex;
-    if (total && base + SCAN1_TILE >= live) *total = (int)(ex + tot);
+    if (total && base + TILE >= live) *total = (int)(ex + tot);
   }
   __syncthreads();
   off += (int)excl_s;
 #pragma unroll
-  for (int i = 0; i < SCAN1_ITEMS; i++) tile[threadIdx.x * SCAN1_ITEMS + i] = off + loc[i];
+  for (int i = 0; i < ITEMS; i++) tile[threadIdx.x * ITEMS + i] = off + loc[i];
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < SCAN1_ITEMS; i++) {
-    long long idx = base + (long long)i * BLOCK1K + threadIdx.x;
-    if (idx < live) op.emit(idx, tile[i * BLOCK1K + threadIdx.x]);
+  for (int i = 0; i < ITEMS; i++) {
+    long long idx = base + (long long)i * NT + threadIdx.x;
+    if (idx < live) op.emit(idx, rec[i], tile[i * NT + threadIdx.x]);
   }
 }
 // the plain scan of an int array, and optionally the compaction of 0/1
@@ -205,10 +239,13 @@ struct ScanArray {
   const int* in;
   int* out;
   int* list;
-  __device__ int count(long long i) const { return in[i]; }
-  __device__ void emit(long long i, int pos) const {
+  struct Rec { int x; };
+  __device__ Rec load(long long i) const { return {in[i]}; }
+  __device__ int count(const Rec& r) const { return r.x; }
+  __device__ void gather(long long, Rec&) const {}
+  __device__ void emit(long long i, const Rec& r, int pos) const {
     out[i] = pos;
-    if (list && in[i]) list[pos] = (int)i;
+    if (list && r.x) list[pos] = (int)i;
   }
 };
 

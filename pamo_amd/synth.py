@@ -103,6 +103,30 @@ def noisy_torus(nu, nv, seed):
     return pts.astype(np.float32), f.astype(np.int32)
 
 
+def uv_sphere(n_lon, n_lat, seed, noise=0.02):
+    """Latitude-longitude sphere with radial noise: two poles of degree
+    n_lon (long fans), 2 * n_lon * (n_lat - 1) faces, outward orientation."""
+    rng = np.random.default_rng(seed)
+    th = np.arange(1, n_lat) * (np.pi / n_lat)
+    ph = np.arange(n_lon) * (2 * np.pi / n_lon)
+    ring = np.stack([np.sin(th)[:, None] * np.cos(ph)[None], np.sin(th)[:, None] * np.sin(ph)[None],
+                     np.cos(th)[:, None] * np.ones(n_lon)[None]], -1).reshape(-1, 3)
+    v = np.concatenate([[[0.0, 0.0, 1.0]], ring, [[0.0, 0.0, -1.0]]])
+    v = v * (1.0 + noise * rng.standard_normal((v.shape[0], 1)))
+    nr = n_lat - 1
+    south = 1 + nr * n_lon
+    j = np.arange(n_lon)
+    j1 = (j + 1) % n_lon
+    faces = [np.stack([np.zeros(n_lon, np.int64), 1 + j, 1 + j1], 1)]
+    for i in range(nr - 1):
+        a, b = 1 + i * n_lon + j, 1 + i * n_lon + j1
+        c, d = 1 + (i + 1) * n_lon + j1, 1 + (i + 1) * n_lon + j
+        faces += [np.stack([a, d, c], 1), np.stack([a, c, b], 1)]
+    last = 1 + (nr - 1) * n_lon
+    faces.append(np.stack([np.full(n_lon, south), last + j1, last + j], 1))
+    return v.astype(np.float32), np.concatenate(faces).astype(np.int32)
+
+
 # ------------------------------------------------------------- SDF union
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _SYN = os.path.join(_HERE, "libpamo_synth.so")

@@ -166,6 +166,28 @@ def test_pamo_simplify_host_abi(cuda):
         assert _same(hv, ov) and _same(hf, of)
 
 
+def test_long_fans_match_oracle(cuda):
+    """Vertices of degree above the fan kernel's LDS capacity (the poles of
+    latitude-longitude spheres, 80 and 200 faces; the second keeps a
+    187-face fan to the end) take the global-memory sort path; results equal
+    the oracle's, alone and batched."""
+    meshes = [synth.uv_sphere(80, 40, 21), synth.uv_sphere(200, 24, 22)]
+    for v, f in meshes:
+        assert np.bincount(f.ravel()).max() > 64
+        verts, scale, _ = oracle.preprocess(v, f)
+        target = int(0.01 * f.shape[0])
+        ov, of, st = oracle.run(verts, f, scale, target)
+        gv, gf, gst = simplify_stage2(torch.from_numpy(verts).to(cuda), torch.from_numpy(f).to(cuda), scale, target)
+        print("long fans", f.shape, st, gst["rounds"], flush=True)
+        assert gst["rounds"] == st["rounds"]
+        assert _same(gv.cpu().numpy(), ov) and _same(gf.cpu().numpy(), of)
+    res = batch.simplify_batch(meshes + [synth.displaced_icosphere(3, 12)], [0.01, 0.01, 0.05])
+    for (v, f), r, (bv, bf, info) in zip(meshes, [0.01, 0.01], res):
+        ov, of, st = _oracle_result(v, f, r)
+        assert info["rounds"] == st["rounds"]
+        assert _same(bv, ov) and _same(bf, of)
+
+
 @pytest.mark.parametrize("guard", [0, 1, 2])
 def test_strict_guard_matches_oracle(cuda, guard):
     """The strict guard on the device equals the oracle's, bit for bit, with
