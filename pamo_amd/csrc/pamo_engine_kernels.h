@@ -1372,7 +1372,6 @@ __global__ void __launch_bounds__(BLOCK) k_traverse_emit(
   const long long qi = gid >> kshift;
   const int part = (int)(gid & ((1 << kshift) - 1));
   int nq = *nq_ptr;
-  unsigned cand = 0;
   int cnt = 0;
   int f = -1;
   bool full = false;   // the pair buffer ran out: walk this query in full instead
@@ -1415,7 +1414,6 @@ __global__ void __launch_bounds__(BLOCK) k_traverse_emit(
         if (c >= 0) { node = c; continue; }
         int g = ~c;
         if (g != f) {
-          cand++;
           if (cnt == CAND_CAP) {   // LDS list full: move it to the pair buffer
             long long b = (long long)atomicAdd(npairs, (unsigned long long)CAND_CAP);
             for (int j = 0; j < CAND_CAP; j++)
@@ -1451,12 +1449,6 @@ __global__ void __launch_bounds__(BLOCK) k_traverse_emit(
   if (f >= 0 && !fits) ovf[atomicAdd(novf, 1)] = f;
   for (int j = 0; j < mine; j++)
     if (at + j < pair_cap) pairs[at + j] = make_int2(f, cl[j * BLOCK + t]);
-  if (stats) {
-    unsigned long long c64 = cand;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) c64 += __shfl_xor(c64, o, 64);
-    if (lane == 0 && c64) atomicAdd(&stats[0], c64);
-  }
 }
 
 // One candidate pair per lane (grid-stride over the pairs the walk wrote):
@@ -1467,9 +1459,15 @@ __device__ void pair_test_blocks(long long bid, long long nb, const int2* __rest
                                  const unsigned long long* __restrict__ npairs, long long pair_cap,
                                  const float* __restrict__ P, const int* __restrict__ T, uint8_t* __restrict__ S,
                                  unsigned long long* __restrict__ stats) {
-  long long n = (long long)*npairs;
+  const unsigned long long n_found = *npairs;   // every candidate the search reserved
+  long long n = (long long)n_found;
   if (n > pair_cap) n = pair_cap;
-  if (stats && bid == 0 && threadIdx.x == 0) atomicAdd(&stats[2], (unsigned long long)n);
+  // the check's candidate and pair counts, once per check (the searches do
+  // not count: one address updated by every workgroup serialises them)
+  if (stats && bid == 0 && threadIdx.x == 0) {
+    atomicAdd(&stats[0], n_found);
+    atomicAdd(&stats[2], (unsigned long long)n);
+  }
   unsigned hits = 0;
   for (long long i = bid * blockDim.x + threadIdx.x; i < n; i += nb * blockDim.x) {
     int2 pr = pairs[i];
@@ -1514,7 +1512,7 @@ __global__ void __launch_bounds__(BLOCK) k_pair_test(const int2* __restrict__ pa
 // touching included, with the boxes of k_leaf_boxes_b), so the flags are
 // the same.  Faces and queries are taken in the spatial order (positions
 // foff[k] .. foff[k+1] hold mesh k; dead faces have empty boxes and id -1);
-// each mesh gets ceil(queries/BLOCK) query blocks (bx), and grid row by
+// each mesh gets ceil(queries/BF_NT) query blocks (bx), and grid row by
 // covers positions [by*BF_TILE, (by+1)*BF_TILE) of the block's mesh.  The
 // tile's boxes sit in LDS and every lane of a wave reads the same box at the
 // same time.  Candidates go through the same LDS lists, flushes and pair
@@ -1528,7 +1526,8 @@ __global__ void __launch_bounds__(BLOCK) k_pair_test(const int2* __restrict__ pa
 // unbounded.
 constexpr int BF_TILE = 256;
 constexpr int BF_GRP = 16;
-static_assert(BF_TILE == BLOCK, "one tile box per thread at load");
+constexpr int BF_NT = 64;   // queries (threads) per workgroup: one wave (many small workgroups hide latency)
+static_assert(BF_TILE % BF_NT == 0 && BF_NT % BF_GRP == 0 && BF_NT == 64, "one wave, tile load by groups");
 __device__ __forceinline__ bool box_has_nan(float4 lo, float4 hi) {
   return (lo.x != lo.x) | (lo.y != lo.y) | (lo.z != lo.z) | (hi.x != hi.x) | (hi.y != hi.y) | (hi.z != hi.z);
 }
@@ -1552,53 +1551,53 @@ __global__ void __launch_bounds__(BLOCK) k_bf_blocks(int M, const long long* __r
                                                      long long* __restrict__ boff) {
   for (int k = threadIdx.x; k < M; k += BLOCK) {
     int nq = qpos[foff[k + 1]] - qpos[foff[k]];
-    nb[k] = (nq + BLOCK - 1) / BLOCK;
+    nb[k] = (nq + BF_NT - 1) / BF_NT;
   }
   __syncthreads();
   block_small_scan(nb, M, boff, 1, nullptr, 0);   // query blocks -> block offsets per mesh
 }
-__global__ void __launch_bounds__(BLOCK) k_bf_emit(
+__global__ void __launch_bounds__(BF_NT) k_bf_emit(
     const int* __restrict__ qlist, const int* __restrict__ qpos, const long long* __restrict__ foff,
     const long long* __restrict__ boff, int M, const int* __restrict__ sid, const float4* __restrict__ slo,
     const float4* __restrict__ shi, const float* __restrict__ P, const int* __restrict__ T, int2* __restrict__ pairs,
     long long pair_cap, unsigned long long* __restrict__ npairs, int* __restrict__ ovf, int* __restrict__ novf,
-    int* __restrict__ oflag, unsigned long long* __restrict__ stats) {
-  __shared__ int cl[CAND_CAP * BLOCK];
+    int* __restrict__ oflag) {
+  __shared__ int cl[CAND_CAP * BF_NT];
   __shared__ float4 blo[BF_TILE], bhi[BF_TILE];
   __shared__ int bid[BF_TILE];
   __shared__ float4 glo[BF_TILE / BF_GRP], ghi[BF_TILE / BF_GRP];
-  const int t = threadIdx.x;
-  const int lane = t & 63;
+  const int t = threadIdx.x;   // == lane
   const long long bx = blockIdx.x;
-  if (bx >= boff[M]) return;                     // block-uniform exits
+  if (bx >= boff[M]) return;   // workgroup-uniform exits
   const int k = find_seg(boff, M, bx);
   const long long a0 = foff[k], a1 = foff[k + 1];
   const long long tile0 = a0 + (long long)blockIdx.y * BF_TILE;
   if (tile0 >= a1) return;
   const int tn = (int)min((long long)BF_TILE, a1 - tile0);
-  {
+#pragma unroll
+  for (int r = 0; r < BF_TILE / BF_NT; r++) {
+    const int j = r * BF_NT + t;
     float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.f), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
     int id = -1;   // padding: never a candidate, empty in the union
-    if (t < tn) {
-      lo = slo[tile0 + t];
-      hi = shi[tile0 + t];
-      id = sid[tile0 + t];
+    if (j < tn) {
+      lo = slo[tile0 + j];
+      hi = shi[tile0 + j];
+      id = sid[tile0 + j];
     }
-    blo[t] = lo;
-    bhi[t] = hi;
-    bid[t] = id;
+    blo[j] = lo;
+    bhi[j] = hi;
+    bid[j] = id;
     float l0 = lo.x, l1 = lo.y, l2 = lo.z, h0 = hi.x, h1 = hi.y, h2 = hi.z;
     if (box_has_nan(lo, hi)) l0 = l1 = l2 = -INFINITY, h0 = h1 = h2 = INFINITY;
     lane_union(l0, l1, l2, h0, h1, h2, BF_GRP);
-    if ((t & (BF_GRP - 1)) == 0) {
-      glo[t / BF_GRP] = make_float4(l0, l1, l2, 0.f);
-      ghi[t / BF_GRP] = make_float4(h0, h1, h2, 0.f);
+    if ((j & (BF_GRP - 1)) == 0) {
+      glo[j / BF_GRP] = make_float4(l0, l1, l2, 0.f);
+      ghi[j / BF_GRP] = make_float4(h0, h1, h2, 0.f);
     }
   }
   __syncthreads();
-  const long long q1 = qpos[foff[k + 1]];
-  const long long qi = qpos[foff[k]] + (bx - boff[k]) * BLOCK + t;
-  unsigned cand = 0;
+  const long long q1 = qpos[a1];
+  const long long qi = qpos[a0] + (bx - boff[k]) * BF_NT + t;
   int cnt = 0;
   int f = -1;
   bool full = false;
@@ -1618,24 +1617,39 @@ __global__ void __launch_bounds__(BLOCK) k_bf_emit(
   lane_union(w0, w1, w2, x0, x1, x2, 64);
   w0 = wave_uniform(w0), w1 = wave_uniform(w1), w2 = wave_uniform(w2);
   x0 = wave_uniform(x0), x1 = wave_uniform(x1), x2 = wave_uniform(x2);
-  if (qi < q1) {
-    // the six comparisons of the overlap test, evaluated without short
-    // circuits (they have no side effects) so the box loads are not
-    // serialised behind branches; four boxes per step.  The append is
-    // predicated: every box writes the lane's next list slot and only a hit
-    // advances the count, so the step has no branch; the list is moved to
-    // the pair buffer before a step that could overrun it.
+  // The six comparisons of the overlap test, evaluated without short
+  // circuits (they have no side effects) so the box loads are not
+  // serialised behind branches; four boxes per step.  The append is
+  // predicated: every box writes the lane's next list slot and only a hit
+  // advances the count, so the step has no branch.  Every lane runs the
+  // loop (lanes without a query hold an empty box and never hit), so the
+  // flushes are wave-wide: when any lane's list could overrun, the wave
+  // moves all lists to the pair buffer with one prefix sum and one atomic.
+  // If the buffer is full the wave stops, and all its queries take the
+  // overflow walk.
+  {
     const int ng = (tn + BF_GRP - 1) / BF_GRP;
     for (int gi = 0; gi < ng && !full; gi++) {
       const float4 gl = glo[gi], gh = ghi[gi];
       if ((x0 < gl.x) | (gh.x < w0) | (x1 < gl.y) | (gh.y < w1) | (x2 < gl.z) | (gh.z < w2)) continue;
       for (int j0 = gi * BF_GRP; j0 < gi * BF_GRP + BF_GRP; j0 += 4) {
-        if (cnt > CAND_CAP - 4) {
-          long long b0 = (long long)atomicAdd(npairs, (unsigned long long)cnt);
+        if (__any(cnt > CAND_CAP - 4)) {
+          int incl = cnt;
+#pragma unroll
+          for (int o = 1; o < 64; o <<= 1) {
+            int y = __shfl_up(incl, o, 64);
+            if (t >= o) incl += y;
+          }
+          const int wtot = __shfl(incl, 63, 64);
+          unsigned long long b0 = 0;
+          if (t == 0) b0 = atomicAdd(npairs, (unsigned long long)wtot);
+          b0 = ((unsigned long long)(unsigned)__shfl((int)(b0 >> 32), 0, 64) << 32) |
+               (unsigned long long)(unsigned)__shfl((int)(unsigned)b0, 0, 64);
+          const long long at = (long long)b0 + (incl - cnt);
           for (int q = 0; q < cnt; q++)
-            if (b0 + q < pair_cap) pairs[b0 + q] = make_int2(f, cl[q * BLOCK + t]);
-          if (b0 + cnt > pair_cap) { full = true; break; }
+            if (at + q < pair_cap) pairs[at + q] = make_int2(f, cl[q * BF_NT + t]);
           cnt = 0;
+          if ((long long)b0 + wtot > pair_cap) { full = true; break; }
         }
         unsigned sep[4];
         int gg[4];
@@ -1649,53 +1663,33 @@ __global__ void __launch_bounds__(BLOCK) k_bf_emit(
 #pragma unroll
         for (int u = 0; u < 4; u++) {
           const int g = gg[u];
-          const int hit = (int)(sep[u] == 0u) & (int)(g >= 0) & (int)(g != f);
-          cl[cnt * BLOCK + t] = g;
+          const int hit = (int)(sep[u] == 0u) & (int)(g >= 0) & (int)(g != f) & (int)(f >= 0);
+          cl[cnt * BF_NT + t] = g;
           cnt += hit;
-          cand += (unsigned)hit;
         }
       }
     }
   }
-  // one reservation per workgroup: lane prefix within the wave, wave
-  // prefix within the block, one atomic
-  __shared__ int wtot[BLOCK / 64];
-  __shared__ unsigned long long wbase;
-  __shared__ unsigned wcand[BLOCK / 64];
-  const int wid = t >> 6;
+  // the wave's final reservation: one prefix sum, one atomic
   const bool small = f >= 0 && !full;
-  int mine = small ? cnt : 0;
+  const int mine = small ? cnt : 0;
   int incl = mine;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
     int y = __shfl_up(incl, o, 64);
-    if (lane >= o) incl += y;
+    if (t >= o) incl += y;
   }
-  unsigned c32 = cand;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) c32 += __shfl_xor(c32, o, 64);
-  if (lane == 63) wtot[wid] = incl;
-  if (lane == 0) wcand[wid] = c32;
-  __syncthreads();
-  int wpre = 0, btot = 0;
-  unsigned bcand = 0;
-#pragma unroll
-  for (int w = 0; w < BLOCK / 64; w++) {
-    if (w < wid) wpre += wtot[w];
-    btot += wtot[w];
-    bcand += wcand[w];
-  }
-  if (t == 0) {
-    wbase = btot ? atomicAdd(npairs, (unsigned long long)btot) : 0ull;
-    if (stats && bcand) atomicAdd(&stats[0], (unsigned long long)bcand);
-  }
-  __syncthreads();
-  long long at = (long long)wbase + wpre + (incl - mine);
+  const int wtot = __shfl(incl, 63, 64);
+  unsigned long long base = 0;
+  if (t == 0 && wtot) base = atomicAdd(npairs, (unsigned long long)wtot);
+  base = ((unsigned long long)(unsigned)__shfl((int)(base >> 32), 0, 64) << 32) |
+         (unsigned long long)(unsigned)__shfl((int)(unsigned)base, 0, 64);
+  const long long at = (long long)base + (incl - mine);
   const bool fits = small && at + mine <= pair_cap;
   // a query may overflow in several tiles; it enters the list once
   if (f >= 0 && !fits && atomicExch(&oflag[f], 1) == 0) ovf[atomicAdd(novf, 1)] = f;
   for (int j = 0; j < mine; j++)
-    if (at + j < pair_cap) pairs[at + j] = make_int2(f, cl[j * BLOCK + t]);
+    if (at + j < pair_cap) pairs[at + j] = make_int2(f, cl[j * BF_NT + t]);
 }
 // overflow queries of the exhaustive search: every alive face of the mesh,
 // with the narrow phase inline, S[f] = 1 on the first hit
