@@ -98,8 +98,8 @@ __global__ void k_init_mesh_state(int M, const int* __restrict__ nust, int* __re
 __global__ void k_round_init(long long V, long long F, int M, int* __restrict__ counters, int n_counters,
                              int c_v, int c_f, unsigned* __restrict__ cstats, int n_cstats,
                              const int* __restrict__ nust, int* __restrict__ first, int* __restrict__ nvu,
-                             int* __restrict__ lcnt, int* __restrict__ cnt, int* __restrict__ fcur,
-                             int* __restrict__ ucnt, int* __restrict__ occ, u64* __restrict__ vmin) {
+                             int* __restrict__ lcnt, int* __restrict__ cnt, int* __restrict__ ucnt,
+                             int* __restrict__ occ, u64* __restrict__ vmin) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n_counters) counters[i] = i == c_v ? (int)V : (i == c_f ? (int)F : 0);
   if (i < n_cstats) cstats[i] = 0;
@@ -110,14 +110,17 @@ __global__ void k_round_init(long long V, long long F, int M, int* __restrict__ 
   }
   if (i <= V) {
     cnt[i] = 0;
-    fcur[i] = 0;
     ucnt[i] = 0;
     occ[i] = i < V ? 1 : 0;
     if (i < V) vmin[i] = ~0ull;
   }
 }
-// fan counts, plus the round-start face snapshot (the undo restore source)
-__global__ void k_fan_count_snap(const int* __restrict__ T, int F, int* cnt, int* __restrict__ T0) {
+// fan counts, plus the round-start face snapshot (the undo restore source).
+// The count's atomic returns the face's slot in each corner's fan (in an
+// arbitrary order: k_fan_slots sorts every fan), kept in frank so the fill
+// is plain stores.
+__global__ void k_fan_count_snap(const int* __restrict__ T, int F, int* cnt, int* __restrict__ T0,
+                                 int* __restrict__ frank) {
   int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= F) return;
   I3 t = ldt(T, f);
@@ -125,7 +128,22 @@ __global__ void k_fan_count_snap(const int* __restrict__ T, int F, int* cnt, int
   T0[3LL * f + 1] = t.j;
   T0[3LL * f + 2] = t.k;
   if (dead3(t)) return;
-  atomicAdd(&cnt[t.i], 1); atomicAdd(&cnt[t.j], 1); atomicAdd(&cnt[t.k], 1);
+  const int ri = atomicAdd(&cnt[t.i], 1), rj = atomicAdd(&cnt[t.j], 1), rk = atomicAdd(&cnt[t.k], 1);
+  frank[3LL * f] = ri;
+  frank[3LL * f + 1] = rj;
+  frank[3LL * f + 2] = rk;
+}
+// the fans from the counts' slots
+__global__ void k_fan_place(const int* __restrict__ T, int F, const int* __restrict__ fan_off,
+                            const int* __restrict__ frank, int* __restrict__ fan) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  I3 t = ldt(T, f);
+  if (dead3(t)) return;
+  const I3 r = ldt(frank, f);
+  fan[fan_off[t.i] + r.i] = f;
+  fan[fan_off[t.j] + r.j] = f;
+  fan[fan_off[t.k] + r.k] = f;
 }
 // Per-check resets in one launch: query and hit flags of every face, the
 // per-mesh leaf ranges, the root's parent, the mark counter and the pair

@@ -600,11 +600,14 @@ int pamo_ctx::forward(const float* d_pts, const int* d_tris, const int* d_undo, 
 static void eng_free(pamo_ctx* c) {
   if (!c->eng) return;
   Engine& g = *c->eng;
+  // every Buf member of Engine
   Buf* all[] = {&g.P, &g.P0, &g.T, &g.T0, &g.vmesh, &g.fmesh, &g.nP, &g.nT, &g.nvm, &g.nfm, &g.undo_in, &g.undo_in2,
-                &g.undo_out, &g.tmp, &g.marked, &g.pm, &g.table, &g.ecost, &g.mp, &g.voff, &g.foff, &g.eoffm,
-                &g.uioff, &g.uioff2, &g.nustk, &g.h, &g.lcnt, &g.ncoll, &g.cut, &g.cutoff, &g.uobase, &g.hm, &g.part,
-                &g.first, &g.nvu, &g.nvoff, &g.nfoff, &g.seg_a, &g.seg_b, &g.root, &g.root_end, &g.scene,
-                &g.flat_cost, &g.g, &g.keep, &g.kpos, &g.newk, &g.vkeep, &g.vpos, &g.fsucc, &g.sface, &g.splane, &g.slen, &g.erank, &g.vedge, &g.uoff, &g.vmin, &g.fcur, &g.ucnt, &g.oflag, &g.bfnb, &g.bfoff};
+                &g.undo_out, &g.tmp, &g.marked, &g.pm, &g.table, &g.ecost, &g.pm2, &g.table2, &g.ecost2, &g.roff,
+                &g.mp, &g.voff, &g.foff, &g.eoffm, &g.uioff, &g.uioff2, &g.nustk, &g.h, &g.lcnt, &g.ncoll, &g.cut,
+                &g.cutoff, &g.uobase, &g.hm, &g.part, &g.first, &g.nvu, &g.nvoff, &g.nfoff, &g.seg_a, &g.seg_b,
+                &g.root, &g.root_end, &g.scene, &g.flat_cost, &g.g, &g.keep, &g.kpos, &g.newk, &g.vkeep, &g.vpos,
+                &g.fsucc, &g.sface, &g.splane, &g.slen, &g.erank, &g.vedge, &g.uoff, &g.vmin, &g.wlist, &g.frank,
+                &g.ucnt, &g.oflag, &g.bfnb, &g.bfoff, &g.pk, &g.sorder, &g.sorder2};
   for (Buf* b : all)
     if (b->p) (void)hipFree(b->p);
   for (auto& s : g.slots) {
