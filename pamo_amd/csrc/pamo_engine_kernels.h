@@ -961,6 +961,29 @@ __global__ void k_sorder_keys(const int* __restrict__ T0, const float* __restric
   keys[f] = mesh_morton_key(fmesh[f], mp, lo, hi);
   vals[f] = f;
 }
+// keys of an admitted mesh's faces [f0, f0 + n): the Morton code of the box
+// centre in the mesh's vertex box (the mesh is appended last, so its
+// faces follow every earlier mesh's in the order)
+__global__ void k_sorder_keys_new(const int* __restrict__ T, const float* __restrict__ P, long long f0, int n,
+                                  float lo0, float lo1, float lo2, float hi0, float hi1, float hi2,
+                                  u64* __restrict__ keys, int* __restrict__ vals) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 lo, hi;
+  face_box(P, ldt(T, (int)(f0 + i)), lo, hi);
+  const float c[3] = {0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
+  const float bl[3] = {lo0, lo1, lo2}, bh[3] = {hi0, hi1, hi2};
+  uint32_t q[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    float ext = bh[a] - bl[a];
+    float t = ext > 0.0f ? (c[a] - bl[a]) / ext : 0.0f;
+    t = fminf(fmaxf(t * 1024.0f, 0.0f), 1023.0f);
+    q[a] = (uint32_t)t;
+  }
+  keys[i] = (u64)(expand10(q[0]) * 4 + expand10(q[1]) * 2 + expand10(q[2]));
+  vals[i] = (int)(f0 + i);
+}
 // per position of the order: the face's box and index if it takes part in
 // the check (alive, in a participating mesh), else an empty box and -1
 __global__ void __launch_bounds__(BLOCK) k_sorder_boxes(const int* __restrict__ sorder, int F,

@@ -166,6 +166,45 @@ def test_pamo_simplify_host_abi(cuda):
         assert _same(hv, ov) and _same(hf, of)
 
 
+def test_region_packing_after_retirement(cuda):
+    """A large mesh with a mild ratio retires after a few rounds while a
+    small one runs on: the retired regions then exceed the kept ones, so the
+    kept mesh's persistent regions (vertex map, invalid-vertex table, edge
+    costs) are moved mid-run. Both results equal the oracle's; a third
+    mesh admitted after the move also does."""
+    big, small, late = synth.sdf_union(20, 80, 9), synth.displaced_icosphere(3, 12), synth.noisy_torus(32, 12, 4)
+    assert big[1].shape[0] > 8 * small[1].shape[0]
+    eng = batch.BatchEngine()
+    meta = {}
+
+    def submit(mesh, ratio, job):
+        v, t, s, m = batch.prepare(*mesh, cuda)
+        meta[eng.submit(v, t, s, int(ratio * mesh[1].shape[0]), job=job)] = (job, m)
+
+    submit(big, 0.5, 0)
+    submit(small, 0.01, 1)
+    got = {}
+    small_running = None
+    for _ in range(2000):
+        _, active = eng.step(1)
+        for h, _ in eng.poll():
+            v, f, info = eng.take(h)
+            job, m = meta[h]
+            got[job] = (v.cpu().numpy() + m, f.cpu().numpy(), info["rounds"])
+        if 0 in got and small_running is None:
+            small_running = 1 not in got
+            submit(late, 0.01, 2)
+        if active == 0 and len(got) == 3:
+            break
+    assert small_running, "the small mesh must still run when the large one retires"
+    for job, (v, f), r in ((0, big, 0.5), (1, small, 0.01), (2, late, 0.01)):
+        ov, of, st = _oracle_result(v, f, r)
+        gv, gf, rounds = got[job]
+        print("packing", job, f.shape, rounds, st["rounds"], flush=True)
+        assert rounds == st["rounds"]
+        assert _same(gv, ov) and _same(gf, of), job
+
+
 def test_long_fans_match_oracle(cuda):
     """Vertices of degree above the fan kernel's LDS capacity (the poles of
     latitude-longitude spheres, 80 and 200 faces; the second keeps a
