@@ -985,20 +985,50 @@ __global__ void k_sorder_keys_new(const int* __restrict__ T, const float* __rest
   vals[i] = (int)(f0 + i);
 }
 // per position of the order: the face's box and index if it takes part in
-// the check (alive, in a participating mesh), else an empty box and -1
+// the check (alive, in a participating mesh), else an empty box and -1;
+// and per tile of BLOCK positions (one workgroup, aligned to the position
+// grid) the union of its boxes, unbounded if a box has a NaN coordinate
 __global__ void __launch_bounds__(BLOCK) k_sorder_boxes(const int* __restrict__ sorder, int F,
                                                         const int* __restrict__ lf, const float* __restrict__ P,
                                                         const int* __restrict__ T, int* __restrict__ sid,
-                                                        float4* __restrict__ slo, float4* __restrict__ shi) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= F) return;
-  const int f = sorder[p];
+                                                        float4* __restrict__ slo, float4* __restrict__ shi,
+                                                        float4* __restrict__ tlo, float4* __restrict__ thi) {
+  __shared__ float4 wlo[BLOCK / 64], whi[BLOCK / 64];
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
   float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.0f), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
-  const bool in = lf[f] != 0;
-  if (in) face_box(P, ldt(T, f), lo, hi);
-  sid[p] = in ? f : -1;
-  slo[p] = lo;
-  shi[p] = hi;
+  if (p < F) {
+    const int f = sorder[p];
+    const bool in = lf[f] != 0;
+    if (in) face_box(P, ldt(T, f), lo, hi);
+    sid[p] = in ? f : -1;
+    slo[p] = lo;
+    shi[p] = hi;
+  }
+  float l0 = lo.x, l1 = lo.y, l2 = lo.z, h0 = hi.x, h1 = hi.y, h2 = hi.z;
+  if ((l0 != l0) | (l1 != l1) | (l2 != l2) | (h0 != h0) | (h1 != h1) | (h2 != h2))
+    l0 = l1 = l2 = -INFINITY, h0 = h1 = h2 = INFINITY;
+  for (int o = 1; o < 64; o <<= 1) {
+    l0 = fminf(l0, __shfl_xor(l0, o, 64));
+    l1 = fminf(l1, __shfl_xor(l1, o, 64));
+    l2 = fminf(l2, __shfl_xor(l2, o, 64));
+    h0 = fmaxf(h0, __shfl_xor(h0, o, 64));
+    h1 = fmaxf(h1, __shfl_xor(h1, o, 64));
+    h2 = fmaxf(h2, __shfl_xor(h2, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    wlo[threadIdx.x >> 6] = make_float4(l0, l1, l2, 0.0f);
+    whi[threadIdx.x >> 6] = make_float4(h0, h1, h2, 0.0f);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float4 a = wlo[0], c = whi[0];
+    for (int w = 1; w < BLOCK / 64; w++) {
+      a = make_float4(fminf(a.x, wlo[w].x), fminf(a.y, wlo[w].y), fminf(a.z, wlo[w].z), 0.0f);
+      c = make_float4(fmaxf(c.x, whi[w].x), fmaxf(c.y, whi[w].y), fmaxf(c.z, whi[w].z), 0.0f);
+    }
+    tlo[blockIdx.x] = a;
+    thi[blockIdx.x] = c;
+  }
 }
 // queries (flagged faces) listed in the order; qpos per position
 struct SorderQueryOp {
@@ -1552,22 +1582,28 @@ __global__ void __launch_bounds__(BLOCK) k_pair_test(const int2* __restrict__ pa
 // the same (alive faces g != f whose float AABB overlaps the query's,
 // touching included, with the boxes of k_leaf_boxes_b), so the flags are
 // the same.  Faces and queries are taken in the spatial order (positions
-// foff[k] .. foff[k+1] hold mesh k; dead faces have empty boxes and id -1);
-// each mesh gets ceil(queries/BF_NT) query blocks (bx), and grid row by
-// covers positions [by*BF_TILE, (by+1)*BF_TILE) of the block's mesh.  The
-// tile's boxes sit in LDS and every lane of a wave reads the same box at the
-// same time.  Candidates go through the same LDS lists, flushes and pair
-// buffer as the walk.
+// foff[k] .. foff[k+1] hold mesh k; dead faces have empty boxes and id -1).
 //
-// Culling: the tile is cut into groups of BF_GRP consecutive boxes, each
-// with the union of its boxes, and a wave skips a group when the union of
-// its lanes' query boxes misses the group's union.  The unions contain
-// every member box, so no overlapping pair is skipped; a box or query with
-// a NaN coordinate (which the overlap test never separates) makes its union
-// unbounded.
+// Work unit: one wave of BF_NT queries of one mesh (a query wave, bx)
+// against one tile of BF_TILE positions, tiles aligned to the global
+// position grid (a tile at a mesh boundary takes only the mesh's
+// positions).  The culling is exact at every level, since every union
+// contains its members' boxes and the overlap test never separates a NaN
+// coordinate (a union with one is unbounded):
+//   - k_bf_waves: each query wave stores its queries' boxes and their
+//     union, tests the union against the union of every tile of its mesh
+//     (from k_sorder_boxes), and lists the (wave, tile) pairs that meet;
+//   - k_bf_emit: persistent workgroups take listed pairs; the tile's boxes
+//     go to LDS in groups of BF_GRP with their unions, groups that miss the
+//     wave's union are skipped, the rest are tested box by box.
+// The spatial order keeps a wave's queries and a tile's faces local, so
+// most pairs are never listed, and no workgroup is spent on them.
+// Candidates go through per-lane LDS lists, wave-wide flushes and the pair
+// buffer.
 constexpr int BF_TILE = 256;
 constexpr int BF_GRP = 16;
-constexpr int BF_NT = 64;   // queries (threads) per workgroup: one wave (many small workgroups hide latency)
+constexpr int BF_NT = 64;   // queries (threads) per workgroup: one wave
+static_assert(BF_TILE == BLOCK, "tile unions come from one k_sorder_boxes workgroup");
 static_assert(BF_TILE % BF_NT == 0 && BF_NT % BF_GRP == 0 && BF_NT == 64, "one wave, tile load by groups");
 __device__ __forceinline__ bool box_has_nan(float4 lo, float4 hi) {
   return (lo.x != lo.x) | (lo.y != lo.y) | (lo.z != lo.z) | (hi.x != hi.x) | (hi.y != hi.y) | (hi.z != hi.z);
@@ -1589,88 +1625,145 @@ __device__ __forceinline__ float wave_uniform(float x) {
 }
 __global__ void __launch_bounds__(BLOCK) k_bf_blocks(int M, const long long* __restrict__ foff,
                                                      const int* __restrict__ qpos, int* __restrict__ nb,
-                                                     long long* __restrict__ boff) {
+                                                     long long* __restrict__ boff, int* __restrict__ npair) {
   for (int k = threadIdx.x; k < M; k += BLOCK) {
     int nq = qpos[foff[k + 1]] - qpos[foff[k]];
     nb[k] = (nq + BF_NT - 1) / BF_NT;
   }
+  if (threadIdx.x == 0) *npair = 0;   // the (wave, tile) list of this check
   __syncthreads();
-  block_small_scan(nb, M, boff, 1, nullptr, 0);   // query blocks -> block offsets per mesh
+  block_small_scan(nb, M, boff, 1, nullptr, 0);   // query waves -> wave offsets per mesh
+}
+// One workgroup per query wave: its queries' boxes, their union, and the
+// listed tiles of its mesh whose union meets it.
+__global__ void __launch_bounds__(BF_NT) k_bf_waves(
+    const int* __restrict__ qlist, const int* __restrict__ qpos, const long long* __restrict__ foff,
+    const long long* __restrict__ boff, int M, const float* __restrict__ P, const int* __restrict__ T,
+    const float4* __restrict__ tlo, const float4* __restrict__ thi, float4* __restrict__ qlo,
+    float4* __restrict__ qhi, float4* __restrict__ wlo, float4* __restrict__ whi, int4* __restrict__ wrec,
+    int2* __restrict__ plist, int* __restrict__ npair) {
+  const int t = threadIdx.x;   // == lane
+  const long long bx = blockIdx.x;
+  if (bx >= boff[M]) return;   // workgroup-uniform exits
+  const int k = find_seg(boff, M, bx);
+  const long long a0 = foff[k], a1 = foff[k + 1];
+  const long long q1 = qpos[a1];
+  const long long qi = qpos[a0] + (bx - boff[k]) * BF_NT + t;
+  float l0 = INFINITY, l1 = INFINITY, l2 = INFINITY, h0 = -INFINITY, h1 = -INFINITY, h2 = -INFINITY;
+  if (qi < q1) {
+    I3 tf = ldt(T, qlist[qi]);
+    V3 a = ldp(P, tf.i), b = ldp(P, tf.j), c = ldp(P, tf.k);
+    l0 = fminf(a.x, fminf(b.x, c.x)), h0 = fmaxf(a.x, fmaxf(b.x, c.x));
+    l1 = fminf(a.y, fminf(b.y, c.y)), h1 = fmaxf(a.y, fmaxf(b.y, c.y));
+    l2 = fminf(a.z, fminf(b.z, c.z)), h2 = fmaxf(a.z, fmaxf(b.z, c.z));
+    qlo[qi] = make_float4(l0, l1, l2, 0.f);
+    qhi[qi] = make_float4(h0, h1, h2, 0.f);
+  }
+  if ((l0 != l0) | (l1 != l1) | (l2 != l2) | (h0 != h0) | (h1 != h1) | (h2 != h2))
+    l0 = l1 = l2 = -INFINITY, h0 = h1 = h2 = INFINITY;
+  lane_union(l0, l1, l2, h0, h1, h2, 64);
+  const float w0 = wave_uniform(l0), w1 = wave_uniform(l1), w2 = wave_uniform(l2);
+  const float x0 = wave_uniform(h0), x1 = wave_uniform(h1), x2 = wave_uniform(h2);
+  if (t == 0) {
+    wlo[bx] = make_float4(w0, w1, w2, 0.f);
+    whi[bx] = make_float4(x0, x1, x2, 0.f);
+    const long long q0 = qi;   // lane 0's query: the wave's first
+    wrec[bx] = make_int4((int)q0, (int)min(q0 + BF_NT, q1), (int)a0, (int)a1);   // queries, mesh positions
+  }
+  const long long tt0 = a0 / BF_TILE, tt1 = (a1 - 1) / BF_TILE;   // a1 > a0: the mesh has queries
+  for (long long c0 = tt0; c0 <= tt1; c0 += BF_NT) {
+    const long long tt = c0 + t;
+    bool ov = false;
+    if (tt <= tt1) {
+      const float4 ul = tlo[tt], uh = thi[tt];
+      ov = !((x0 < ul.x) | (uh.x < w0) | (x1 < ul.y) | (uh.y < w1) | (x2 < ul.z) | (uh.z < w2));
+    }
+    const unsigned long long m = __ballot(ov);
+    if (!m) continue;
+    int base = 0;
+    if (t == 0) base = atomicAdd(npair, __popcll(m));
+    base = __shfl(base, 0, 64);
+    if (ov) plist[base + __popcll(m & ((1ull << t) - 1))] = make_int2((int)bx, (int)tt);
+  }
 }
 __global__ void __launch_bounds__(BF_NT) k_bf_emit(
-    const int* __restrict__ qlist, const int* __restrict__ qpos, const long long* __restrict__ foff,
-    const long long* __restrict__ boff, int M, const int* __restrict__ sid, const float4* __restrict__ slo,
-    const float4* __restrict__ shi, const float* __restrict__ P, const int* __restrict__ T, int2* __restrict__ pairs,
-    long long pair_cap, unsigned long long* __restrict__ npairs, int* __restrict__ ovf, int* __restrict__ novf,
+    const int2* __restrict__ plist, const int* __restrict__ npair, const int4* __restrict__ wrec,
+    const float4* __restrict__ wlo, const float4* __restrict__ whi, const float4* __restrict__ qlo,
+    const float4* __restrict__ qhi, const int* __restrict__ qlist, int F, const int* __restrict__ sid,
+    const float4* __restrict__ slo, const float4* __restrict__ shi, int2* __restrict__ pairs, long long pair_cap,
+    unsigned long long* __restrict__ npairs, int* __restrict__ ovf, int* __restrict__ novf,
     int* __restrict__ oflag) {
   __shared__ int cl[CAND_CAP * BF_NT];
   __shared__ float4 blo[BF_TILE], bhi[BF_TILE];
   __shared__ int bid[BF_TILE];
   __shared__ float4 glo[BF_TILE / BF_GRP], ghi[BF_TILE / BF_GRP];
   const int t = threadIdx.x;   // == lane
-  const long long bx = blockIdx.x;
-  if (bx >= boff[M]) return;   // workgroup-uniform exits
-  const int k = find_seg(boff, M, bx);
-  const long long a0 = foff[k], a1 = foff[k + 1];
-  const long long tile0 = a0 + (long long)blockIdx.y * BF_TILE;
-  if (tile0 >= a1) return;
-  const int tn = (int)min((long long)BF_TILE, a1 - tile0);
+  const int np = *npair;
+  for (int it = blockIdx.x; it < np; it += gridDim.x) {   // workgroup-uniform loop
+    // the pair, then the wave's record and the tile together (the tile's
+    // loads wait only for the pair; the mesh range masks them afterwards)
+    const int2 pe = plist[it];
+    const long long bx = pe.x, tile0 = (long long)pe.y * BF_TILE;
+    const int4 wr = wrec[bx];
+    const long long a0 = wr.z, a1 = wr.w;
+    __syncthreads();   // the previous pair's boxes are no longer read
+    float4 tl[BF_TILE / BF_NT], th[BF_TILE / BF_NT];
+    int ti[BF_TILE / BF_NT];
 #pragma unroll
-  for (int r = 0; r < BF_TILE / BF_NT; r++) {
-    const int j = r * BF_NT + t;
-    float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.f), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
-    int id = -1;   // padding: never a candidate, empty in the union
-    if (j < tn) {
-      lo = slo[tile0 + j];
-      hi = shi[tile0 + j];
-      id = sid[tile0 + j];
+    for (int r = 0; r < BF_TILE / BF_NT; r++) {
+      const long long p = min(tile0 + r * BF_NT + t, (long long)F - 1);
+      tl[r] = slo[p];
+      th[r] = shi[p];
+      ti[r] = sid[p];
     }
-    blo[j] = lo;
-    bhi[j] = hi;
-    bid[j] = id;
-    float l0 = lo.x, l1 = lo.y, l2 = lo.z, h0 = hi.x, h1 = hi.y, h2 = hi.z;
-    if (box_has_nan(lo, hi)) l0 = l1 = l2 = -INFINITY, h0 = h1 = h2 = INFINITY;
-    lane_union(l0, l1, l2, h0, h1, h2, BF_GRP);
-    if ((j & (BF_GRP - 1)) == 0) {
-      glo[j / BF_GRP] = make_float4(l0, l1, l2, 0.f);
-      ghi[j / BF_GRP] = make_float4(h0, h1, h2, 0.f);
+#pragma unroll
+    for (int r = 0; r < BF_TILE / BF_NT; r++) {
+      const int j = r * BF_NT + t;
+      const long long p = tile0 + j;
+      float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.f), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+      int id = -1;   // other meshes' positions: never a candidate, empty in the union
+      if (p >= a0 && p < a1) {
+        lo = tl[r];
+        hi = th[r];
+        id = ti[r];
+      }
+      blo[j] = lo;
+      bhi[j] = hi;
+      bid[j] = id;
+      float l0 = lo.x, l1 = lo.y, l2 = lo.z, h0 = hi.x, h1 = hi.y, h2 = hi.z;
+      if (box_has_nan(lo, hi)) l0 = l1 = l2 = -INFINITY, h0 = h1 = h2 = INFINITY;
+      lane_union(l0, l1, l2, h0, h1, h2, BF_GRP);
+      if ((j & (BF_GRP - 1)) == 0) {
+        glo[j / BF_GRP] = make_float4(l0, l1, l2, 0.f);
+        ghi[j / BF_GRP] = make_float4(h0, h1, h2, 0.f);
+      }
     }
-  }
-  __syncthreads();
-  const long long q1 = qpos[a1];
-  const long long qi = qpos[a0] + (bx - boff[k]) * BF_NT + t;
-  int cnt = 0;
-  int f = -1;
-  bool full = false;
-  float qlo0 = INFINITY, qlo1 = INFINITY, qlo2 = INFINITY, qhi0 = -INFINITY, qhi1 = -INFINITY, qhi2 = -INFINITY;
-  if (qi < q1) {
-    f = qlist[qi];
-    I3 tf = ldt(T, f);
-    V3 a = ldp(P, tf.i), b = ldp(P, tf.j), c = ldp(P, tf.k);
-    qlo0 = fminf(a.x, fminf(b.x, c.x)), qhi0 = fmaxf(a.x, fmaxf(b.x, c.x));
-    qlo1 = fminf(a.y, fminf(b.y, c.y)), qhi1 = fmaxf(a.y, fmaxf(b.y, c.y));
-    qlo2 = fminf(a.z, fminf(b.z, c.z)), qhi2 = fmaxf(a.z, fmaxf(b.z, c.z));
-  }
-  // the wave's query union, in scalar registers
-  float w0 = qlo0, w1 = qlo1, w2 = qlo2, x0 = qhi0, x1 = qhi1, x2 = qhi2;
-  if ((w0 != w0) | (w1 != w1) | (w2 != w2) | (x0 != x0) | (x1 != x1) | (x2 != x2))
-    w0 = w1 = w2 = -INFINITY, x0 = x1 = x2 = INFINITY;
-  lane_union(w0, w1, w2, x0, x1, x2, 64);
-  w0 = wave_uniform(w0), w1 = wave_uniform(w1), w2 = wave_uniform(w2);
-  x0 = wave_uniform(x0), x1 = wave_uniform(x1), x2 = wave_uniform(x2);
-  // The six comparisons of the overlap test, evaluated without short
-  // circuits (they have no side effects) so the box loads are not
-  // serialised behind branches; four boxes per step.  The append is
-  // predicated: every box writes the lane's next list slot and only a hit
-  // advances the count, so the step has no branch.  Every lane runs the
-  // loop (lanes without a query hold an empty box and never hit), so the
-  // flushes are wave-wide: when any lane's list could overrun, the wave
-  // moves all lists to the pair buffer with one prefix sum and one atomic.
-  // If the buffer is full the wave stops, and all its queries take the
-  // overflow walk.
-  {
-    const int ng = (tn + BF_GRP - 1) / BF_GRP;
-    for (int gi = 0; gi < ng && !full; gi++) {
+    __syncthreads();
+    const long long qi = (long long)wr.x + t;
+    const long long q1 = wr.y;
+    int cnt = 0;
+    int f = -1;
+    bool full = false;
+    float qlo0 = INFINITY, qlo1 = INFINITY, qlo2 = INFINITY, qhi0 = -INFINITY, qhi1 = -INFINITY, qhi2 = -INFINITY;
+    if (qi < q1) {
+      f = qlist[qi];
+      const float4 ql = qlo[qi], qh = qhi[qi];
+      qlo0 = ql.x, qlo1 = ql.y, qlo2 = ql.z, qhi0 = qh.x, qhi1 = qh.y, qhi2 = qh.z;
+    }
+    const float4 wl = wlo[bx], wh = whi[bx];
+    const float w0 = wave_uniform(wl.x), w1 = wave_uniform(wl.y), w2 = wave_uniform(wl.z);
+    const float x0 = wave_uniform(wh.x), x1 = wave_uniform(wh.y), x2 = wave_uniform(wh.z);
+    // The six comparisons of the overlap test, evaluated without short
+    // circuits (they have no side effects) so the box loads are not
+    // serialised behind branches; four boxes per step.  The append is
+    // predicated: every box writes the lane's next list slot and only a hit
+    // advances the count, so the step has no branch.  Every lane runs the
+    // loop (lanes without a query hold an empty box and never hit), so the
+    // flushes are wave-wide: when any lane's list could overrun, the wave
+    // moves all lists to the pair buffer with one prefix sum and one
+    // atomic.  If the buffer is full the wave stops, and all its queries
+    // take the overflow walk.
+    for (int gi = 0; gi < BF_TILE / BF_GRP && !full; gi++) {
       const float4 gl = glo[gi], gh = ghi[gi];
       if ((x0 < gl.x) | (gh.x < w0) | (x1 < gl.y) | (gh.y < w1) | (x2 < gl.z) | (gh.z < w2)) continue;
       for (int j0 = gi * BF_GRP; j0 < gi * BF_GRP + BF_GRP; j0 += 4) {
@@ -1710,27 +1803,27 @@ __global__ void __launch_bounds__(BF_NT) k_bf_emit(
         }
       }
     }
-  }
-  // the wave's final reservation: one prefix sum, one atomic
-  const bool small = f >= 0 && !full;
-  const int mine = small ? cnt : 0;
-  int incl = mine;
+    // the wave's final reservation for this pair: one prefix sum, one atomic
+    const bool small = f >= 0 && !full;
+    const int mine = small ? cnt : 0;
+    int incl = mine;
 #pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    int y = __shfl_up(incl, o, 64);
-    if (t >= o) incl += y;
+    for (int o = 1; o < 64; o <<= 1) {
+      int y = __shfl_up(incl, o, 64);
+      if (t >= o) incl += y;
+    }
+    const int wtot = __shfl(incl, 63, 64);
+    unsigned long long base = 0;
+    if (t == 0 && wtot) base = atomicAdd(npairs, (unsigned long long)wtot);
+    base = ((unsigned long long)(unsigned)__shfl((int)(base >> 32), 0, 64) << 32) |
+           (unsigned long long)(unsigned)__shfl((int)(unsigned)base, 0, 64);
+    const long long at = (long long)base + (incl - mine);
+    const bool fits = small && at + mine <= pair_cap;
+    // a query may overflow in several tiles; it enters the list once
+    if (f >= 0 && !fits && atomicExch(&oflag[f], 1) == 0) ovf[atomicAdd(novf, 1)] = f;
+    for (int j = 0; j < mine; j++)
+      if (at + j < pair_cap) pairs[at + j] = make_int2(f, cl[j * BF_NT + t]);
   }
-  const int wtot = __shfl(incl, 63, 64);
-  unsigned long long base = 0;
-  if (t == 0 && wtot) base = atomicAdd(npairs, (unsigned long long)wtot);
-  base = ((unsigned long long)(unsigned)__shfl((int)(base >> 32), 0, 64) << 32) |
-         (unsigned long long)(unsigned)__shfl((int)(unsigned)base, 0, 64);
-  const long long at = (long long)base + (incl - mine);
-  const bool fits = small && at + mine <= pair_cap;
-  // a query may overflow in several tiles; it enters the list once
-  if (f >= 0 && !fits && atomicExch(&oflag[f], 1) == 0) ovf[atomicAdd(novf, 1)] = f;
-  for (int j = 0; j < mine; j++)
-    if (at + j < pair_cap) pairs[at + j] = make_int2(f, cl[j * BF_NT + t]);
 }
 // overflow queries of the exhaustive search: every alive face of the mesh,
 // with the narrow phase inline, S[f] = 1 on the first hit

@@ -607,7 +607,8 @@ static void eng_free(pamo_ctx* c) {
                 &g.cutoff, &g.uobase, &g.hm, &g.part, &g.first, &g.nvu, &g.nvoff, &g.nfoff, &g.seg_a, &g.seg_b,
                 &g.root, &g.root_end, &g.scene, &g.flat_cost, &g.g, &g.keep, &g.kpos, &g.newk, &g.vkeep, &g.vpos,
                 &g.fsucc, &g.sface, &g.splane, &g.slen, &g.erank, &g.vedge, &g.uoff, &g.vmin, &g.wlist, &g.frank,
-                &g.ucnt, &g.oflag, &g.bfnb, &g.bfoff, &g.pk, &g.sorder, &g.sorder2, &g.nsort};
+                &g.ucnt, &g.oflag, &g.bfnb, &g.bfoff, &g.pk, &g.sorder, &g.sorder2, &g.nsort, &g.tbox, &g.bfq,
+                &g.bfw, &g.bfl, &g.bfn};
   for (Buf* b : all)
     if (b->p) (void)hipFree(b->p);
   for (auto& s : g.slots) {
