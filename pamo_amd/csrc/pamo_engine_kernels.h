@@ -986,13 +986,15 @@ __global__ void k_sorder_keys_new(const int* __restrict__ T, const float* __rest
 }
 // per position of the order: the face's box and index if it takes part in
 // the check (alive, in a participating mesh), else an empty box and -1;
-// and per tile of BLOCK positions (one workgroup, aligned to the position
-// grid) the union of its boxes, unbounded if a box has a NaN coordinate
+// and the union of the boxes of each group of 16 positions and of each tile
+// of BLOCK positions (one workgroup), aligned to the position grid and
+// unbounded if a box has a NaN coordinate
 __global__ void __launch_bounds__(BLOCK) k_sorder_boxes(const int* __restrict__ sorder, int F,
                                                         const int* __restrict__ lf, const float* __restrict__ P,
                                                         const int* __restrict__ T, int* __restrict__ sid,
                                                         float4* __restrict__ slo, float4* __restrict__ shi,
-                                                        float4* __restrict__ tlo, float4* __restrict__ thi) {
+                                                        float4* __restrict__ tlo, float4* __restrict__ thi,
+                                                        float4* __restrict__ glo, float4* __restrict__ ghi) {
   __shared__ float4 wlo[BLOCK / 64], whi[BLOCK / 64];
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.0f), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
@@ -1008,6 +1010,10 @@ __global__ void __launch_bounds__(BLOCK) k_sorder_boxes(const int* __restrict__ 
   if ((l0 != l0) | (l1 != l1) | (l2 != l2) | (h0 != h0) | (h1 != h1) | (h2 != h2))
     l0 = l1 = l2 = -INFINITY, h0 = h1 = h2 = INFINITY;
   for (int o = 1; o < 64; o <<= 1) {
+    if (o == 16 && (threadIdx.x & 15) == 0) {   // the union of each group of 16 positions
+      glo[p >> 4] = make_float4(l0, l1, l2, 0.0f);
+      ghi[p >> 4] = make_float4(h0, h1, h2, 0.0f);
+    }
     l0 = fminf(l0, __shfl_xor(l0, o, 64));
     l1 = fminf(l1, __shfl_xor(l1, o, 64));
     l2 = fminf(l2, __shfl_xor(l2, o, 64));
@@ -1605,9 +1611,6 @@ constexpr int BF_GRP = 16;
 constexpr int BF_NT = 64;   // queries (threads) per workgroup: one wave
 static_assert(BF_TILE == BLOCK, "tile unions come from one k_sorder_boxes workgroup");
 static_assert(BF_TILE % BF_NT == 0 && BF_NT % BF_GRP == 0 && BF_NT == 64, "one wave, tile load by groups");
-__device__ __forceinline__ bool box_has_nan(float4 lo, float4 hi) {
-  return (lo.x != lo.x) | (lo.y != lo.y) | (lo.z != lo.z) | (hi.x != hi.x) | (hi.y != hi.y) | (hi.z != hi.z);
-}
 // union of the box over `width` consecutive lanes (width a power of two <= 64)
 __device__ __forceinline__ void lane_union(float& l0, float& l1, float& l2, float& h0, float& h1, float& h2,
                                            int width) {
@@ -1690,7 +1693,8 @@ __global__ void __launch_bounds__(BF_NT) k_bf_emit(
     const int2* __restrict__ plist, const int* __restrict__ npair, const int4* __restrict__ wrec,
     const float4* __restrict__ wlo, const float4* __restrict__ whi, const float4* __restrict__ qlo,
     const float4* __restrict__ qhi, const int* __restrict__ qlist, int F, const int* __restrict__ sid,
-    const float4* __restrict__ slo, const float4* __restrict__ shi, int2* __restrict__ pairs, long long pair_cap,
+    const float4* __restrict__ slo, const float4* __restrict__ shi, const float4* __restrict__ gulo,
+    const float4* __restrict__ guhi, int2* __restrict__ pairs, long long pair_cap,
     unsigned long long* __restrict__ npairs, int* __restrict__ ovf, int* __restrict__ novf,
     int* __restrict__ oflag) {
   __shared__ int cl[CAND_CAP * BF_NT];
@@ -1716,27 +1720,20 @@ __global__ void __launch_bounds__(BF_NT) k_bf_emit(
       th[r] = shi[p];
       ti[r] = sid[p];
     }
+    // group unions (precomputed over whole groups; at a mesh boundary they
+    // also cover the other mesh's boxes, which only weakens the culling)
+    if (t < BF_TILE / BF_GRP) {
+      glo[t] = gulo[tile0 / BF_GRP + t];
+      ghi[t] = guhi[tile0 / BF_GRP + t];
+    }
 #pragma unroll
     for (int r = 0; r < BF_TILE / BF_NT; r++) {
       const int j = r * BF_NT + t;
       const long long p = tile0 + j;
-      float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.f), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
-      int id = -1;   // other meshes' positions: never a candidate, empty in the union
-      if (p >= a0 && p < a1) {
-        lo = tl[r];
-        hi = th[r];
-        id = ti[r];
-      }
-      blo[j] = lo;
-      bhi[j] = hi;
-      bid[j] = id;
-      float l0 = lo.x, l1 = lo.y, l2 = lo.z, h0 = hi.x, h1 = hi.y, h2 = hi.z;
-      if (box_has_nan(lo, hi)) l0 = l1 = l2 = -INFINITY, h0 = h1 = h2 = INFINITY;
-      lane_union(l0, l1, l2, h0, h1, h2, BF_GRP);
-      if ((j & (BF_GRP - 1)) == 0) {
-        glo[j / BF_GRP] = make_float4(l0, l1, l2, 0.f);
-        ghi[j / BF_GRP] = make_float4(h0, h1, h2, 0.f);
-      }
+      const bool in = p >= a0 && p < a1;   // other meshes' positions: never a candidate
+      blo[j] = in ? tl[r] : make_float4(INFINITY, INFINITY, INFINITY, 0.f);
+      bhi[j] = in ? th[r] : make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+      bid[j] = in ? ti[r] : -1;
     }
     __syncthreads();
     const long long qi = (long long)wr.x + t;
