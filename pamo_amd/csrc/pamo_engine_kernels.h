@@ -1600,8 +1600,9 @@ __global__ void __launch_bounds__(BLOCK) k_pair_test(const int2* __restrict__ pa
 //     union, tests the union against the union of every tile of its mesh
 //     (from k_sorder_boxes), and lists the (wave, tile) pairs that meet;
 //   - k_bf_emit: persistent workgroups take listed pairs; the tile's boxes
-//     go to LDS in groups of BF_GRP with their unions, groups that miss the
-//     wave's union are skipped, the rest are tested box by box.
+//     go to LDS, groups of BF_GRP positions whose union (also from
+//     k_sorder_boxes) misses the wave's union are skipped, the rest are
+//     tested box by box.
 // The spatial order keeps a wave's queries and a tile's faces local, so
 // most pairs are never listed, and no workgroup is spent on them.
 // Candidates go through per-lane LDS lists, wave-wide flushes and the pair
