@@ -45,19 +45,6 @@ __device__ __forceinline__ int find_seg(const long long* __restrict__ off, int M
 // ------------------------------------------------------- step 1 (undo in)
 // clear the table of meshes that bring an undo list and are not stuck
 // (cusimp_free.cu:905-908); one block per mesh
-// Only the entries of the mesh's current vertices are cleared: the table is
-// read at local vertex indices, which stay below the current vertex count
-// for the rest of the run (it never grows), so entries past it are never
-// read again.
-__global__ void k_clear_tables(const MeshParam* __restrict__ mp, const long long* __restrict__ uioff,
-                               const long long* __restrict__ voff, int M, uint8_t* __restrict__ table) {
-  int k = blockIdx.y;
-  if (k >= M) return;
-  if (mp[k].stuck || uioff[k + 1] == uioff[k]) return;
-  const long long n = min(voff[k + 1] - voff[k], mp[k].pmcap);
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    table[mp[k].pmo + i] = 0;
-}
 // every incoming undo entry marks its mesh's table (cusimp_free.cu:910-916);
 // bounds-checked (the reference is not)
 __global__ void k_undo_table(const int* __restrict__ undo_in, const long long* __restrict__ uioff, int M,
@@ -95,12 +82,23 @@ __global__ void k_init_mesh_state(int M, const int* __restrict__ nust, int* __re
 // statistics, per-mesh undo state, the three per-vertex counters of the fan
 // and edge builds, the occupancy flags (1 per vertex, 0 at V) and the
 // per-vertex claim minima.
+// It also clears the invalid-vertex table of every mesh that receives an
+// undo list and is not stuck (cusimp_free.cu:905-907), one entry per vertex:
+// the table is read at local vertex indices, which stay below the mesh's
+// current vertex count for the rest of the run (it never grows), so entries
+// past it are never read again.
 __global__ void k_round_init(long long V, long long F, int M, int* __restrict__ counters, int n_counters,
                              int c_v, int c_f, unsigned* __restrict__ cstats, int n_cstats,
                              const int* __restrict__ nust, int* __restrict__ first, int* __restrict__ nvu,
                              int* __restrict__ lcnt, int* __restrict__ cnt, int* __restrict__ ucnt,
-                             int* __restrict__ occ, u64* __restrict__ vmin) {
+                             int* __restrict__ occ, u64* __restrict__ vmin, const int* __restrict__ vmesh,
+                             const MeshParam* __restrict__ mp, const long long* __restrict__ uioff,
+                             const long long* __restrict__ voff, uint8_t* __restrict__ table) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < V) {
+    const int k = vmesh[i];
+    if (!mp[k].stuck && uioff[k + 1] != uioff[k]) table[mp[k].pmo + (i - voff[k])] = 0;
+  }
   if (i < n_counters) counters[i] = i == c_v ? (int)V : (i == c_f ? (int)F : 0);
   if (i < n_cstats) cstats[i] = 0;
   if (i < M) {
@@ -436,9 +434,13 @@ struct FaceCompactOp {
 // fans.  The edge emit counts edges per u (the atomic's return value is
 // the edge's rank within u; any rank order gives the same results, since
 // every output depends only on the edge index).
+// (threads 0..M also write each mesh's first edge, eoffm)
 __global__ void k_edge_vorder(const int* __restrict__ edges, const int* __restrict__ E_ptr,
-                              const int* __restrict__ uoff, const int* __restrict__ erank, int4* __restrict__ vedge) {
+                              const int* __restrict__ uoff, const int* __restrict__ erank, int4* __restrict__ vedge,
+                              int M, const long long* __restrict__ foff, const int* __restrict__ eoff_face,
+                              long long* __restrict__ eoffm) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e <= M) eoffm[e] = eoff_face[foff[e]];
   if (e >= *E_ptr) return;
   int u = edges[2 * e], v = edges[2 * e + 1];
   vedge[uoff[u] + erank[e]] = make_int4(e, u, v, 0);
@@ -736,12 +738,6 @@ __global__ void __launch_bounds__(BLOCK) k_collapse_b(const int* __restrict__ wl
 }
 
 // per-mesh edge offsets (faces emit their edges in order)
-__global__ void k_edge_offsets(int M, const long long* __restrict__ foff, const int* __restrict__ eoff_face,
-                               long long* __restrict__ eoffm) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k > M) return;
-  eoffm[k] = eoff_face[foff[k]];
-}
 
 // After the collapse, in one launch: blocks below nbf remove invalid faces
 // (cusimp_free.cu:218-232), the rest count zero-length edges per mesh (the
@@ -1994,19 +1990,22 @@ __global__ void k_advance_first(int M, const int* __restrict__ hm, int* __restri
   if (k < M) first[k] += 2 * hm[k];
 }
 // unwritten tail of each undo list (D6) -> 0
-__global__ void k_undo_tail(int M, const long long* __restrict__ uobase, const int* __restrict__ first,
-                            const int* __restrict__ nvu, int* __restrict__ undo_out) {
-  int k = blockIdx.x;
-  if (k >= M) return;
-  for (int j = first[k] + threadIdx.x; j < nvu[k]; j += blockDim.x) undo_out[uobase[k] + j] = 0;
-}
 
 // ---------------------------------------------------- end of round
 // local vertex maps into the mesh regions (the scan over occupancy is
 // global; local = global - value at the mesh start)
+// Workgroups from nbv on write the undo-list tails: per mesh, zeros from
+// the first entry the undo passes left free up to nvu.
 __global__ void k_local_map(const int* __restrict__ g, const long long* __restrict__ voff, int M,
                             const int* __restrict__ vmesh, const MeshParam* __restrict__ mp, const int* __restrict__ V_ptr,
-                            int* __restrict__ pm) {
+                            int* __restrict__ pm, int nbv, const long long* __restrict__ uobase,
+                            const int* __restrict__ first, const int* __restrict__ nvu, int* __restrict__ undo_out) {
+  if ((int)blockIdx.x >= nbv) {
+    const int k = blockIdx.x - nbv;
+    if (k >= M) return;
+    for (int j = first[k] + threadIdx.x; j < nvu[k]; j += blockDim.x) undo_out[uobase[k] + j] = 0;
+    return;
+  }
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   int V = *V_ptr;
   if (v >= V) return;
