@@ -743,10 +743,32 @@ __global__ void __launch_bounds__(BLOCK) k_collapse_b(const int* __restrict__ wl
 // (cusimp_free.cu:218-232), the rest count zero-length edges per mesh (the
 // line-edge pass runs only for meshes that have one).  The two read and
 // write disjoint data.
+// The face workgroups also do the first check's initialisation
+// (k_check_init's work: nothing between here and the check reads those
+// arrays); nbf covers max(Fl, M) threads.
 __global__ void k_invalid_line_b(int* T, int F, int nbf, const float* __restrict__ P, const int* __restrict__ edges,
-                                 const int* __restrict__ E_ptr, const int* __restrict__ vmesh, int* lcnt) {
+                                 const int* __restrict__ E_ptr, const int* __restrict__ vmesh, int* lcnt, long long Fl,
+                                 int M, uint8_t* __restrict__ qflag, uint8_t* __restrict__ S,
+                                 int* __restrict__ seg_a, int* __restrict__ seg_b, int* __restrict__ kparent,
+                                 int* __restrict__ nmark, int* __restrict__ novf,
+                                 unsigned long long* __restrict__ npairs, int* __restrict__ oflag) {
   if ((int)blockIdx.x < nbf) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < Fl) {
+      qflag[f] = 0;
+      S[f] = 0;
+      oflag[f] = 0;
+    }
+    if (f < M) {
+      seg_a[f] = -1;
+      seg_b[f] = -1;
+    }
+    if (f == 0) {
+      kparent[0] = -1;
+      *nmark = 0;
+      *novf = 0;
+      *npairs = 0;
+    }
     if (f >= F) return;
     I3 t = ldt(T, f);
     if (dead3(t)) return;
@@ -986,7 +1008,8 @@ __global__ void k_sorder_keys_new(const int* __restrict__ T, const float* __rest
 // of BLOCK positions (one workgroup), aligned to the position grid and
 // unbounded if a box has a NaN coordinate
 __global__ void __launch_bounds__(BLOCK) k_sorder_boxes(const int* __restrict__ sorder, int F,
-                                                        const int* __restrict__ lf, const float* __restrict__ P,
+                                                        const int* __restrict__ fmesh, const int* __restrict__ part,
+                                                        const float* __restrict__ P,
                                                         const int* __restrict__ T, int* __restrict__ sid,
                                                         float4* __restrict__ slo, float4* __restrict__ shi,
                                                         float4* __restrict__ tlo, float4* __restrict__ thi,
@@ -996,8 +1019,9 @@ __global__ void __launch_bounds__(BLOCK) k_sorder_boxes(const int* __restrict__ 
   float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.0f), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
   if (p < F) {
     const int f = sorder[p];
-    const bool in = lf[f] != 0;
-    if (in) face_box(P, ldt(T, f), lo, hi);
+    const I3 tf = ldt(T, f);
+    const bool in = !dead3(tf) && part[fmesh[f]];   // alive, in a participating mesh
+    if (in) face_box(P, tf, lo, hi);
     sid[p] = in ? f : -1;
     slo[p] = lo;
     shi[p] = hi;
@@ -1033,18 +1057,23 @@ __global__ void __launch_bounds__(BLOCK) k_sorder_boxes(const int* __restrict__ 
   }
 }
 // queries (flagged faces) listed in the order; qpos per position
+// (a query: alive, in a participating mesh, and flagged)
 struct SorderQueryOp {
   const int* sorder;
   int F;   // >= 1
-  const int* qf;
+  const int* T;
+  const int* fmesh;
+  const int* part;
+  const uint8_t* qflag;
   int* qpos;
   int* qlist;
   struct Rec { int f, q; };
   __device__ Rec load(long long p) const {
     const int f = sorder[min(p, (long long)F - 1)];
+    const I3 t = ldt(T, f);
     Rec r;
     r.f = f;
-    r.q = p < F ? qf[f] : 0;
+    r.q = (p < F && !dead3(t) && qflag[f] && part[fmesh[f]]) ? 1 : 0;
     return r;
   }
   __device__ int count(const Rec& r) const { return r.q; }
