@@ -2028,7 +2028,9 @@ __global__ void k_advance_first(int M, const int* __restrict__ hm, int* __restri
 __global__ void k_local_map(const int* __restrict__ g, const long long* __restrict__ voff, int M,
                             const int* __restrict__ vmesh, const MeshParam* __restrict__ mp, const int* __restrict__ V_ptr,
                             int* __restrict__ pm, int nbv, const long long* __restrict__ uobase,
-                            const int* __restrict__ first, const int* __restrict__ nvu, int* __restrict__ undo_out) {
+                            const int* __restrict__ first, const int* __restrict__ nvu, int* __restrict__ undo_out,
+                            const int* __restrict__ gate) {
+  if (gate && *gate) return;   // the round's end waits for the undo passes
   if ((int)blockIdx.x >= nbv) {
     const int k = blockIdx.x - nbv;
     if (k >= M) return;
@@ -2048,7 +2050,8 @@ __global__ void __launch_bounds__(BLOCK) k_new_offsets(int M, const long long* _
                                                        const long long* __restrict__ foff, const int* __restrict__ g,
                                                        const int* __restrict__ fpos, long long* __restrict__ nvoff,
                                                        long long* __restrict__ nfoff, const int* __restrict__ nvu,
-                                                       long long* __restrict__ uioff_new) {
+                                                       long long* __restrict__ uioff_new, const int* __restrict__ gate) {
+  if (gate && *gate) return;   // the round's end waits for the undo passes
   for (int k = threadIdx.x; k <= M; k += BLOCK) {
     nvoff[k] = g[voff[k]];
     nfoff[k] = fpos[foff[k]];
@@ -2057,7 +2060,8 @@ __global__ void __launch_bounds__(BLOCK) k_new_offsets(int M, const long long* _
 }
 // next round's packed undo input from the per-mesh undo outputs
 __global__ void k_pack_undo(int M, const long long* __restrict__ uioff_new, const long long* __restrict__ uobase,
-                            const int* __restrict__ undo_out, int* __restrict__ undo_in) {
+                            const int* __restrict__ undo_out, int* __restrict__ undo_in, const int* __restrict__ gate) {
+  if (gate && *gate) return;   // the round's end waits for the undo passes
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= uioff_new[M]) return;
   int k = find_seg(uioff_new, M, i);

@@ -216,9 +216,11 @@ struct pamo_ctx {
     return scan_op(ScanArray{in, out, list}, n, d_n, add, total);
   }
   // one-pass scan whose input and output are the op's count and emit
-  // (inputs up to small_max take the small tile shape)
+  // (inputs up to small_max take the small tile shape; a non-zero *gate on
+  // the device skips the scan)
   template <class Op>
-  bool scan_op(const Op& op, long long n, const int* d_n, int add, int* total, long long small_max = SCAN_SMALL_MAX) {
+  bool scan_op(const Op& op, long long n, const int* d_n, int add, int* total, long long small_max = SCAN_SMALL_MAX,
+               const int* gate = nullptr) {
     if (n <= 0) return true;
     const bool small = n <= small_max;
     const long long tile = small ? SCANS_TILE : SCAN1_TILE;
@@ -233,10 +235,10 @@ struct pamo_ctx {
     }
     if (small)
       k_scan_op<Op, SCANS_NT, SCANS_ITEMS><<<(unsigned)nb, SCANS_NT, 0, st>>>(
-          op, n, d_n, add, ptr<unsigned long long>(scan_desc), scan_epoch, total, cnt_dev(C_ERR));
+          op, n, d_n, add, ptr<unsigned long long>(scan_desc), scan_epoch, total, cnt_dev(C_ERR), gate);
     else
       k_scan_op<Op><<<(unsigned)nb, BLOCK1K, 0, st>>>(op, n, d_n, add, ptr<unsigned long long>(scan_desc), scan_epoch,
-                                                      total, cnt_dev(C_ERR));
+                                                      total, cnt_dev(C_ERR), gate);
     return check_launch("scan");
   }
   bool read_counters() {

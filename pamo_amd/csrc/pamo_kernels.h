@@ -152,15 +152,17 @@ __device__ __forceinline__ unsigned long long scan_word(unsigned epoch, unsigned
 // so no item's loads wait behind another item's stores (which the compiler
 // must assume may alias them); the gathers are issued before the
 // look-back and overlap it.
+// A non-zero *gate skips the scan (live length 0; only the total, 0, is
+// written), so work can be enqueued behind a decision the device makes.
 template <class Op, int NT = BLOCK1K, int ITEMS = SCAN1_ITEMS>
 __global__ void __launch_bounds__(NT) k_scan_op(Op op, long long n, const int* __restrict__ d_n, int add,
                                                 unsigned long long* desc, unsigned epoch, int* __restrict__ total,
-                                                int* err) {
+                                                int* err, const int* __restrict__ gate) {
   __shared__ unsigned lds[NT / 64];
   constexpr int TILE = NT * ITEMS;
   __shared__ int tile[TILE];
   __shared__ long long excl_s;
-  const long long live = d_n ? min(n, (long long)*d_n + add) : n;
+  const long long live = (gate && *gate) ? 0 : (d_n ? min(n, (long long)*d_n + add) : n);
   const long long t = blockIdx.x;
   const long long base = t * TILE;
   if (live <= 0) {
