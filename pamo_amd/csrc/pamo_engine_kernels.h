@@ -509,12 +509,11 @@ __device__ __forceinline__ void ec_fan(const float* __restrict__ P, const int4* 
 }
 
 // the cost's destinations: the mesh's persistent region (when the
-// reference writes it), the flat array the claim reads by global edge index,
-// and the claim's per-vertex minima of (cost << 32 | edge)
-__device__ __forceinline__ void ec_finish(uint32_t* c, uint32_t* __restrict__ flat, u64* vmin, int e, int u, int v,
-                                          uint32_t val, bool write_region) {
+// reference writes it) and the claim's per-vertex minima of
+// (cost << 32 | edge)
+__device__ __forceinline__ void ec_finish(uint32_t* c, u64* vmin, int e, int u, int v, uint32_t val,
+                                          bool write_region) {
   if (write_region) *c = val;
-  flat[e] = val;
   const u64 key = ((u64)val << 32) | (u64)(uint32_t)e;
   atomicMin(&vmin[u], key);
   atomicMin(&vmin[v], key);
@@ -529,8 +528,7 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__
                                                        const int* __restrict__ vmesh,
                                                        const long long* __restrict__ voff,
                                                        const long long* __restrict__ eoffm,
-                                                       const MeshParam* __restrict__ mp,
-                                                       uint32_t* __restrict__ flat, u64* vmin) {
+                                                       const MeshParam* __restrict__ mp, u64* vmin) {
   // every lane of a group has the same i, so each return below is taken by
   // the whole group and the width-EC_G shuffles always see all its lanes
   const int i = (blockIdx.x * blockDim.x + threadIdx.x) / EC_G;
@@ -544,27 +542,33 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__
   uint32_t* c = cost + mp[k].eco + (e - eoffm[k]);
   // where the reference leaves the cost unchanged, the kept value is used
   if (table[pmo + lu] && table[pmo + lv]) {
-    if (g == 0) ec_finish(c, flat, vmin, e, u, v, 0xFFFFFFFFu, true);
+    if (g == 0) ec_finish(c, vmin, e, u, v, 0xFFFFFFFFu, true);
     return;
   }
   if (mp[k].stuck) {
-    if (g == 0) ec_finish(c, flat, vmin, e, u, v, *c, false);
+    if (g == 0) ec_finish(c, vmin, e, u, v, *c, false);
     return;
   }
   int ua = fan_off[u], ub = fan_off[u + 1], va = fan_off[v], vb = fan_off[v + 1];
-  // link condition: equal successor pairs; more than 2 -> maximum, not 2 -> unchanged
+  // link condition: equal successor pairs; more than 2 -> maximum, not 2 ->
+  // unchanged.  The group loads EC_G slots of each fan at once and compares
+  // every pair through shuffles (successors are vertex indices, >= 0).
   int dup = 0;
-  for (int a = ua + g; a < ub; a += EC_G) {
-    int iu = fsucc[a];
-    for (int b = va; b < vb; b++) dup += fsucc[b] == iu;
+  for (int a0 = ua; a0 < ub; a0 += EC_G) {
+    const int iu = a0 + g < ub ? fsucc[a0 + g] : -1;
+    for (int b0 = va; b0 < vb; b0 += EC_G) {
+      const int iv = b0 + g < vb ? fsucc[b0 + g] : -2;
+      const int n = vb - b0 < EC_G ? vb - b0 : EC_G;
+      for (int j = 0; j < n; j++) dup += __shfl(iv, j, EC_G) == iu;
+    }
   }
   dup = ec_group_sum(dup);
   if (dup > 2) {
-    if (g == 0) ec_finish(c, flat, vmin, e, u, v, 0xFFFFFFFFu, true);
+    if (g == 0) ec_finish(c, vmin, e, u, v, 0xFFFFFFFFu, true);
     return;
   }
   if (dup != 2) {
-    if (g == 0) ec_finish(c, flat, vmin, e, u, v, *c, false);
+    if (g == 0) ec_finish(c, vmin, e, u, v, *c, false);
     return;
   }
   const float scale = mp[k].scale, tau = mp[k].tau;
@@ -578,7 +582,7 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__
   nt = ec_group_sum(nt);
   flip = ec_group_sum(flip);
   if (g != 0) return;
-  if (flip) { ec_finish(c, flat, vmin, e, u, v, 0xFFFFFFFFu, true); return; }
+  if (flip) { ec_finish(c, vmin, e, u, v, 0xFFFFFFFFu, true); return; }
   el = el / (float)ne / scale * tau;
   el = el + vnorm(vsub(v0, v1)) / scale * tau;
   Qa = 5.0f * Qa;
@@ -593,7 +597,7 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__
   float t3 = q[3] * m.x + q[6] * m.y + q[8] * m.z + q[9] * 1.0f;
   float qe = t0 * m.x + t1 * m.y + t2 * m.z + t3 * 1.0f;
   float cc = qe / (scale * scale);
-  ec_finish(c, flat, vmin, e, u, v,
+  ec_finish(c, vmin, e, u, v,
             u32sat(clamp_ref(cc + el + Qa / (float)nt * tau, 0.0f, COST_RANGE) / COST_RANGE * U32F), true);
 }
 
