@@ -458,7 +458,7 @@ __global__ void k_edge_vorder(const int* __restrict__ edges, const int* __restri
 // +0 or more (a norm, or 1 - clamp(x, 0, 1)) or NaN.  The link count and the
 // flip test are an integer sum and an OR, exact in any order; a flip
 // anywhere makes the cost the maximum, as the reference's early return.
-constexpr int EC_G = 8;
+constexpr int EC_G = 4;
 
 __device__ __forceinline__ int ec_group_sum(int x) {
 #pragma unroll
