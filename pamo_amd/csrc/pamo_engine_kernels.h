@@ -1793,6 +1793,12 @@ __global__ void __launch_bounds__(BF_NT) k_bf_emit(
     for (int gi = 0; gi < BF_TILE / BF_GRP && !full; gi++) {
       const float4 gl = glo[gi], gh = ghi[gi];
       if ((x0 < gl.x) | (gh.x < w0) | (x1 < gl.y) | (gh.y < w1) | (x2 < gl.z) | (gh.z < w2)) continue;
+      // the group is skipped when no lane's own box meets its union (lanes
+      // without a query hold an empty box); a wave's queries can be spread
+      // over the mesh, so this culls more than the wave's union
+      if (!__any(!((qhi0 < gl.x) | (gh.x < qlo0) | (qhi1 < gl.y) | (gh.y < qlo1) | (qhi2 < gl.z) |
+                   (gh.z < qlo2))))
+        continue;
       for (int j0 = gi * BF_GRP; j0 < gi * BF_GRP + BF_GRP; j0 += 4) {
         if (__any(cnt > CAND_CAP - 4)) {
           int incl = cnt;
