@@ -32,6 +32,17 @@ struct MeshParam {
                           // restored original, so the box bounds the mesh
 };
 
+// Undo passes per round: the reference repeats the check at most 5 times
+// (cusimp_free.cu:1014-1040); the strict guard repeats until nothing is
+// marked, at most STRICT_PASSES times (only a round whose input already
+// intersects needs more than a few).
+constexpr int REF_PASSES = 5;
+constexpr int STRICT_PASSES = 64;
+__host__ __device__ constexpr int undo_passes_cap(int strict) { return strict ? STRICT_PASSES : REF_PASSES; }
+// undo-list entries a mesh may write per scanned collapsed entry in one
+// round: 2 per marked entry for the first check and for every pass
+__host__ __device__ constexpr long long undo_slots_per_cut(int strict) { return 2LL * (undo_passes_cap(strict) + 1); }
+
 __device__ __forceinline__ int find_seg(const long long* __restrict__ off, int M, long long i) {
   int lo = 0, hi = M - 1;   // largest k with off[k] <= i
   while (lo < hi) {
@@ -93,7 +104,8 @@ __global__ void k_round_init(long long V, long long F, int M, int* __restrict__ 
                              int* __restrict__ lcnt, int* __restrict__ cnt, int* __restrict__ ucnt,
                              int* __restrict__ occ, u64* __restrict__ vmin, const int* __restrict__ vmesh,
                              const MeshParam* __restrict__ mp, const long long* __restrict__ uioff,
-                             const long long* __restrict__ voff, uint8_t* __restrict__ table) {
+                             const long long* __restrict__ voff, uint8_t* __restrict__ table,
+                             int* __restrict__ npass) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < V) {
     const int k = vmesh[i];
@@ -105,6 +117,7 @@ __global__ void k_round_init(long long V, long long F, int M, int* __restrict__ 
     first[i] = nust[i];
     nvu[i] = nust[i];
     lcnt[i] = 0;
+    npass[i] = 0;
   }
   if (i <= V) {
     cnt[i] = 0;
@@ -818,8 +831,10 @@ __device__ void block_small_scan(const int* __restrict__ x, int M, long long* __
 // collapse, sequential per mesh (D5), appending to the mesh's collapsed
 // region; the scan cut
 // (cusimp_free.cu:1016); the flat offsets of the cut entries and of the undo
-// outputs (12 per cut entry plus the incoming list of stuck meshes); and
-// which meshes take part in the check.
+// outputs; and which meshes take part in the check.  Every check of a round
+// (the first and one per undo pass) adds 2 undo entries per marked cut entry,
+// so a mesh's undo region holds the incoming list of a stuck mesh plus
+// 2 (passes + 1) entries per cut entry, passes being the mode's cap.
 __global__ void __launch_bounds__(BLOCK) k_line_cut_tail(int M, const int* __restrict__ lcnt, float* P, int* T,
                                                          int* occ, const int* __restrict__ edges,
                                                          const long long* __restrict__ eoffm,
@@ -854,7 +869,7 @@ __global__ void __launch_bounds__(BLOCK) k_line_cut_tail(int M, const int* __res
   }
   __syncthreads();
   block_small_scan(cut, M, cutoff, 1, nullptr, 0);
-  block_small_scan(cut, M, uobase, 12, nustk, 1);
+  block_small_scan(cut, M, uobase, undo_slots_per_cut(strict), nustk, 1);
 }
 
 // ------------------------------------------------- flat views over meshes
@@ -2022,11 +2037,17 @@ __global__ void k_undo_b(int nbu, const long long* __restrict__ cutoff, int M, c
   int r = 0;
   if (x >= 0 && (long long)x < mp[k].pmcap) r = pm[mp[k].pmo + x];
   else atomicAdd(err, 1);
-  undo_out[uobase[k] + first[k] + j] = r;
+  const long long o = uobase[k] + first[k] + j;
+  if (o < uobase[k + 1]) undo_out[o] = r;   // the region holds every pass of the mode's cap
+  else atomicAdd(err, 1);
 }
-__global__ void k_advance_first(int M, const int* __restrict__ hm, int* __restrict__ first) {
+// after an undo pass: the next entry of each mesh's undo list, and the
+// passes each mesh took part in (those with marks)
+__global__ void k_advance_first(int M, const int* __restrict__ hm, int* __restrict__ first, int* __restrict__ npass) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < M) first[k] += 2 * hm[k];
+  if (k >= M || hm[k] == 0) return;
+  first[k] += 2 * hm[k];
+  npass[k] += 1;
 }
 // unwritten tail of each undo list (D6) -> 0
 
@@ -2044,7 +2065,8 @@ __global__ void k_local_map(const int* __restrict__ g, const long long* __restri
   if ((int)blockIdx.x >= nbv) {
     const int k = blockIdx.x - nbv;
     if (k >= M) return;
-    for (int j = first[k] + threadIdx.x; j < nvu[k]; j += blockDim.x) undo_out[uobase[k] + j] = 0;
+    const long long n = min((long long)nvu[k], uobase[k + 1] - uobase[k]);
+    for (long long j = first[k] + threadIdx.x; j < n; j += blockDim.x) undo_out[uobase[k] + j] = 0;
     return;
   }
   int v = blockIdx.x * blockDim.x + threadIdx.x;

@@ -607,7 +607,7 @@ static void eng_free(pamo_ctx* c) {
                 &g.undo_out, &g.tmp, &g.marked, &g.pm, &g.table, &g.ecost, &g.pm2, &g.table2, &g.ecost2, &g.roff,
                 &g.mp, &g.voff, &g.foff, &g.eoffm, &g.uioff, &g.uioff2, &g.nustk, &g.h, &g.lcnt, &g.ncoll, &g.cut,
                 &g.cutoff, &g.uobase, &g.hm, &g.part, &g.first, &g.nvu, &g.nvoff, &g.nfoff, &g.seg_a, &g.seg_b,
-                &g.root, &g.root_end, &g.scene, &g.g, &g.keep, &g.kpos, &g.newk, &g.vkeep, &g.vpos,
+                &g.root, &g.root_end, &g.scene, &g.g, &g.keep, &g.kpos, &g.newk, &g.vkeep, &g.vpos, &g.npass,
                 &g.fsucc, &g.sface, &g.splane, &g.slen, &g.erank, &g.vedge, &g.uoff, &g.vmin, &g.wlist, &g.frank,
                 &g.ucnt, &g.oflag, &g.bfnb, &g.bfoff, &g.pk, &g.sorder, &g.sorder2, &g.nsort, &g.tbox, &g.bfq,
                 &g.bfw, &g.bfl, &g.bfn};

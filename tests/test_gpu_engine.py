@@ -247,3 +247,27 @@ def test_strict_guard_matches_oracle(cuda, guard):
         assert gst["rounds"] == st["rounds"]
         assert _same(gv.cpu().numpy(), ov) and _same(gf.cpu().numpy(), of)
         assert verify(ov, of)["ok"]
+
+
+def test_strict_batch_equals_alone(cuda):
+    """Strict guard, batched: a mesh whose input already intersects (two
+    overlapping spheres) re-marks the same collapses on every undo pass up to
+    the strict cap of 64, so its undo list grows far past what the
+    reference's 5 passes write. Packed next to other meshes, every mesh's
+    result equals its run alone and the oracle's, bit for bit (an undo region
+    sized for fewer passes would spill into the next mesh's list)."""
+    meshes = [synth.displaced_icosphere(4, 11), _two_spheres(), synth.sdf_union(10, 50, 7)]
+    ratios = [0.01, 0.01, 0.01]
+    res = batch.simplify_batch(meshes, ratios, engine=batch.BatchEngine(strict=True))
+    for (v, f), r, (bv, bf, info) in zip(meshes, ratios, res):
+        verts, scale, mean = oracle.preprocess(v, f)
+        target = int(r * f.shape[0])
+        ov, of, st = oracle.run(verts, f, scale, target, strict=True)
+        gv, gf, gst = simplify_stage2(torch.from_numpy(verts).to(cuda), torch.from_numpy(f).to(cuda), scale, target,
+                                      strict=True)
+        print("strict batch", f.shape, info["rounds"], info["max_undo_passes"], st, flush=True)
+        assert info["rounds"] == st["rounds"] == gst["rounds"]
+        assert info["max_undo_passes"] == st["max_undo_passes"] == gst["max_undo_passes"]
+        assert _same(gv.cpu().numpy(), ov) and _same(gf.cpu().numpy(), of)
+        assert _same(bv, ov + mean) and _same(bf, of)
+    assert res[1][2]["max_undo_passes"] == 64
