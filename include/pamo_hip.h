@@ -83,6 +83,30 @@ int pamo_run_stage2(pamo_ctx* ctx, void* stream, const float* d_verts, const int
                     float scale, long long target, float threshold, int tolerance, long long iters, int* out_V,
                     int* out_F, long long* stats);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc): the loop of
+ * pamo_run_stage2, with the result copied into caller buffers d_verts_out
+ * [n_verts,3] and d_faces_out [n_faces,3] (sizes as pamo_run_stage2 reports
+ * them in out_V / out_F), and every working buffer carved from one fixed
+ * device region of PAMO_NOALLOC_SCRATCH_BYTES that the library holds on each
+ * GPU it is loaded on until the process ends.  The call allocates no device
+ * or pinned host memory.  Meshes above PAMO_NOALLOC_MAX_VERTS vertices or
+ * PAMO_NOALLOC_MAX_FACES faces are rejected (-4, message in
+ * pamo_last_error), as is a context with the strict guard set.  ctx supplies
+ * the device, the shuffle mode and the error message; its own buffers are
+ * not used.  All calls on one GPU share its region: calls from threads of
+ * one process are serialised by a lock, and the call synchronises the
+ * stream before it returns; work of the caller's that still reads or writes
+ * the region (there is none unless the caller takes its address) would race
+ * with the next call. */
+#define PAMO_NOALLOC_SCRATCH_BYTES (128u << 20)
+#define PAMO_NOALLOC_MAX_FACES 65536
+#define PAMO_NOALLOC_MAX_VERTS 65536
+int pamo_run_stage2_noalloc(pamo_ctx* ctx, void* stream, const float* d_verts, const int* d_faces, int n_verts,
+                            int n_faces, float scale, long long target, float threshold, int tolerance,
+                            long long iters, float* d_verts_out, int* d_faces_out, int* out_V, int* out_F,
+                            long long* stats);
+
 /* The same loop as pamo_run_stage2 driven round by round from the host over
  * pamo_free_forward (the exact call sequence of simp_cuda/pamo/__init__.py:
  * 131-158: forward, then the compaction of lines 140-148).  Same arguments,

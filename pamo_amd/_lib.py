@@ -54,6 +54,9 @@ def lib():
     L.pamo_free_copy_outputs.restype = I
     L.pamo_run_stage2.argtypes = [P, P, P, P, I, I, F, LL, F, I, LL, ctypes.POINTER(I), ctypes.POINTER(I), P]
     L.pamo_run_stage2.restype = I
+    L.pamo_run_stage2_noalloc.argtypes = [P, P, P, P, I, I, F, LL, F, I, LL, P, P, ctypes.POINTER(I),
+                                          ctypes.POINTER(I), P]
+    L.pamo_run_stage2_noalloc.restype = I
     L.pamo_run_rounds.argtypes = [P, P, I]
     L.pamo_run_rounds.restype = I
     L.pamo_run_fetch.argtypes = [P, P, P, P]

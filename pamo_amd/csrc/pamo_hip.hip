@@ -84,6 +84,14 @@ struct pamo_ctx {
 
   int* hc = nullptr;   // pinned host mirror of the counters
 
+  // No-allocation mode: every buffer is carved, 256-byte aligned, from one
+  // fixed device region (arena) instead of hipMalloc; nothing is freed, and
+  // the region is reused from its start by the next context set up on it.
+  // The counter mirror is then the pageable hc_host.
+  char* arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
+  int hc_host[C_COUNT] = {0};
+
   // optional per-kernel timing with HIP events (mask of PAMO_K_* bits)
   unsigned prof_mask = 0;
   struct ProfRec { int k; hipEvent_t a, b; double bytes; };
@@ -144,13 +152,26 @@ struct pamo_ctx {
   }
   bool check_launch(const char* what) { return hip(hipGetLastError(), what); }
 
+  // nb bytes from the arena (no-allocation mode); the buffer's previous
+  // slice, if any, stays unused until the region is set up again
+  bool carve(Buf& b, size_t nb) {
+    const size_t off = (arena_used + 255) & ~(size_t)255;
+    if (off > arena_cap || nb > arena_cap - off)
+      return fail("scratch region exhausted: %zu bytes in use, %zu more requested, %zu available", arena_used, nb,
+                  arena_cap);
+    b.p = arena + off;
+    b.bytes = nb;
+    arena_used = off + nb;
+    return true;
+  }
   // grow without preserving contents
   bool ensure(Buf& b, size_t bytes) {
     if (bytes <= b.bytes && b.p) return true;
+    size_t nb = std::max<size_t>(bytes + bytes / 8, 256);
+    if (arena) return carve(b, nb);
     if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.bytes = 0;
-    size_t nb = std::max<size_t>(bytes + bytes / 8, 256);
     if (!hip(hipMalloc(&b.p, nb), "hipMalloc")) return false;
     b.bytes = nb;
     return true;
@@ -160,12 +181,16 @@ struct pamo_ctx {
     if (bytes <= b.bytes && b.p) return true;
     Buf nbuf;
     size_t nb = std::max<size_t>(bytes + bytes / 2, 256);
-    if (!hip(hipMalloc(&nbuf.p, nb), "hipMalloc")) return false;
-    nbuf.bytes = nb;
+    if (arena) {
+      if (!carve(nbuf, nb)) return false;
+    } else {
+      if (!hip(hipMalloc(&nbuf.p, nb), "hipMalloc")) return false;
+      nbuf.bytes = nb;
+    }
     if (b.p) {
       if (!hip(hipMemcpyAsync(nbuf.p, b.p, b.bytes, hipMemcpyDeviceToDevice, st), "copy")) return false;
       if (!hip(hipStreamSynchronize(st), "sync")) return false;
-      (void)hipFree(b.p);
+      if (!arena) (void)hipFree(b.p);
     }
     b = nbuf;
     return true;
@@ -225,13 +250,14 @@ struct pamo_ctx {
     const bool small = n <= small_max;
     const long long tile = small ? SCANS_TILE : SCAN1_TILE;
     long long nb = (n + tile - 1) / tile;
+    // descriptors start at zero (unpublished); cleared by a fill kernel
     if (scan_desc.bytes < (size_t)nb * 8) {
       if (!ensure(scan_desc, (size_t)nb * 8)) return false;
-      if (!hip(hipMemsetAsync(scan_desc.p, 0, scan_desc.bytes, st), "scan desc")) return false;
+      if (!fill_i32(scan_desc.p, 0, (long long)(scan_desc.bytes / 4))) return false;
     }
     if (++scan_epoch >= (1u << 22)) {   // epoch wrap: clear the descriptors
       scan_epoch = 1;
-      if (!hip(hipMemsetAsync(scan_desc.p, 0, scan_desc.bytes, st), "scan desc")) return false;
+      if (!fill_i32(scan_desc.p, 0, (long long)(scan_desc.bytes / 4))) return false;
     }
     if (small)
       k_scan_op<Op, SCANS_NT, SCANS_ITEMS><<<(unsigned)nb, SCANS_NT, 0, st>>>(
@@ -278,6 +304,10 @@ static void eng_free(pamo_ctx* c);
 
 pamo_ctx::~pamo_ctx() {
   (void)hipSetDevice(device);
+  if (arena) {   // no-allocation mode: the region is not ours to free
+    eng_free(this);
+    return;
+  }
   (void)hipDeviceSynchronize();
   eng_free(this);
   Buf* all[] = {&pts_map, &table, &edge_cost, &undo_list, &tmp_undo, &P, &P0, &occ, &T, &T0, &cnt, &fan_off, &fan,
@@ -611,13 +641,15 @@ static void eng_free(pamo_ctx* c) {
                 &g.fsucc, &g.sface, &g.splane, &g.slen, &g.erank, &g.vedge, &g.uoff, &g.vmin, &g.wlist, &g.frank,
                 &g.ucnt, &g.oflag, &g.bfnb, &g.bfoff, &g.pk, &g.sorder, &g.sorder2, &g.nsort, &g.tbox, &g.bfq,
                 &g.bfw, &g.bfl, &g.bfn};
-  for (Buf* b : all)
-    if (b->p) (void)hipFree(b->p);
-  for (auto& s : g.slots) {
-    if (s.outV.p) (void)hipFree(s.outV.p);
-    if (s.outF.p) (void)hipFree(s.outF.p);
+  if (!c->arena) {
+    for (Buf* b : all)
+      if (b->p) (void)hipFree(b->p);
+    for (auto& s : g.slots) {
+      if (s.outV.p) (void)hipFree(s.outV.p);
+      if (s.outF.p) (void)hipFree(s.outF.p);
+    }
+    if (g.hb) (void)hipHostFree(g.hb);
   }
-  if (g.hb) (void)hipHostFree(g.hb);
   delete c->eng;
   c->eng = nullptr;
 }
@@ -710,8 +742,10 @@ int pamo_batch_take(pamo_ctx* ctx, void* stream, int handle, float* d_verts_out,
       !c.hip(hipMemcpyAsync(d_faces_out, s.outF.p, (size_t)s.rF * 12, hipMemcpyDeviceToDevice, c.st), "take"))
     return -1;
   if (!c.hip(hipStreamSynchronize(c.st), "sync")) return -1;
-  (void)hipFree(s.outV.p);
-  (void)hipFree(s.outF.p);
+  if (!c.arena) {
+    (void)hipFree(s.outV.p);
+    (void)hipFree(s.outF.p);
+  }
   s.outV = Buf();
   s.outF = Buf();
   s.taken = true;
@@ -1027,6 +1061,79 @@ int pamo_run_fetch(pamo_ctx* ctx, void* stream, float* d_verts_out, int* d_faces
       !c.hip(hipMemcpyAsync(d_faces_out, c.RF[c.run_cur].p, (size_t)c.run_F * 12, hipMemcpyDeviceToDevice, c.st), "fetch"))
     return -1;
   return c.hip(hipStreamSynchronize(c.st), "sync") ? 0 : -1;
+}
+
+// ------------------------------------------------ no-allocation variant
+// Device region of pamo_run_stage2_noalloc: every engine buffer of a call
+// is carved from it.  Each GPU the library is loaded on holds its own copy
+// until the process ends.  Host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device (na_region), never
+// through the array's name.
+static __device__ __align__(256) unsigned char scratch[PAMO_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int NA_MAX_DEVICES = 64;
+struct NoAllocDevice {
+  std::mutex mu;            // one call at a time per device (in this process)
+  char* region = nullptr;   // this device's address of scratch
+  pamo_ctx* c = nullptr;    // host state of the last call (its buffers point into the region)
+};
+NoAllocDevice na_dev[NA_MAX_DEVICES];
+}  // namespace
+
+int pamo_run_stage2_noalloc(pamo_ctx* ctx, void* stream, const float* d_verts, const int* d_faces, int n_verts,
+                            int n_faces, float scale, long long target, float threshold, int tolerance,
+                            long long iters, float* d_verts_out, int* d_faces_out, int* out_V, int* out_F,
+                            long long* stats) {
+  if (!ctx) return -1;
+  ctx->err.clear();
+  if (n_verts < 0 || n_faces < 0 || n_verts > PAMO_NOALLOC_MAX_VERTS || n_faces > PAMO_NOALLOC_MAX_FACES) {
+    ctx->fail("pamo_run_stage2_noalloc takes at most %d vertices and %d faces (got %d and %d)",
+              PAMO_NOALLOC_MAX_VERTS, PAMO_NOALLOC_MAX_FACES, n_verts, n_faces);
+    return -4;
+  }
+  if (ctx->strict) {
+    ctx->fail("pamo_run_stage2_noalloc runs the reference guard only (strict mode is set on this context)");
+    return -4;
+  }
+  if ((n_verts && !d_verts_out) || (n_faces && !d_faces_out)) {
+    ctx->fail("pamo_run_stage2_noalloc: output buffers are required");
+    return -1;
+  }
+  if (ctx->device < 0 || ctx->device >= NA_MAX_DEVICES) { ctx->fail("device %d out of range", ctx->device); return -1; }
+  NoAllocDevice& nd = na_dev[ctx->device];
+  std::lock_guard<std::mutex> lock(nd.mu);
+  if (!ctx->hip(hipSetDevice(ctx->device), "hipSetDevice")) return -1;
+  if (!nd.region) {   // set-up, once per device, with that device current
+    void* p = nullptr;
+    if (!ctx->hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)")) return -1;
+    nd.region = (char*)p;
+  }
+  // fresh host state over the same region: every buffer is carved again
+  // from the region's start, and nothing of the last call is read before
+  // this call writes it (counters and scan descriptors are cleared by
+  // kernels, the engine's persistent regions are zero-filled at admission)
+  delete nd.c;
+  nd.c = new pamo_ctx();
+  pamo_ctx& c = *nd.c;
+  c.device = ctx->device;
+  c.shuffle_mode = ctx->shuffle_mode;
+  c.guard_mode = ctx->guard_mode;
+  c.pair_cap_limit = ctx->pair_cap_limit;
+  c.arena = nd.region;
+  c.arena_cap = PAMO_NOALLOC_SCRATCH_BYTES;
+  c.hc = c.hc_host;
+  int oV = 0, oF = 0;
+  int rc = pamo_run_stage2(&c, stream, d_verts, d_faces, n_verts, n_faces, scale, target, threshold, tolerance, iters,
+                           &oV, &oF, stats);
+  if (rc == 0) rc = pamo_run_fetch(&c, stream, d_verts_out, d_faces_out);
+  if (rc != 0) {
+    ctx->err = c.err;
+    return rc;
+  }
+  if (out_V) *out_V = oV;
+  if (out_F) *out_F = oF;
+  return 0;
 }
 
 int pamo_profile(pamo_ctx* ctx, unsigned kernel_mask) {
