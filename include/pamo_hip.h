@@ -214,6 +214,12 @@ int pamo_debug_pair_cap(pamo_ctx* ctx, long long cap);
  * exhaustive search.  Both give the same candidate set. */
 int pamo_debug_guard(pamo_ctx* ctx, int mode);
 
+/* Test hook (no counterpart in the reference): the engine's LBVH walk splits
+ * each query over 2^kshift lanes, one subtree each (by size: 3 up to 65,536
+ * alive faces, 2 up to 262,144, else 0).  kshift 0..3 forces a split, -1
+ * restores the choice by size.  The candidate set does not depend on it. */
+int pamo_debug_walk_split(pamo_ctx* ctx, int kshift);
+
 /* Strict guard for the engine (pamo_run_stage2, pamo_batch_*); no
  * counterpart in the reference, off by default.  For callers that need every
  * output to pass an exact self-intersection check: a candidate pair also

@@ -34,19 +34,34 @@
 //   collapse threshold conversion saturates like the device conversion (the
 //      host conversion at cusimp_free.cu:923 is undefined only for tau >= 10).
 //
-// What pins each part (tests/test_oracle.py unless noted); no part is left
-// without a check, so nothing here is "parity unpinned" beyond the
-// reference's own undefined behaviour that D0-D9 replace:
+// What pins each part (tests/test_oracle.py unless noted).  Beyond the
+// reference's own undefined behaviour that D0-D9 replace, parity is pinned
+// everywhere except where said below:
 //   shuffle_permutation   rocThrust's own permutations as golden vectors
 //                         (tests/golden/), bijection for many sizes
 //   build_quadrics        closed form on the octahedron; zero QEM on a plane
 //   edge_cost_one         closed form on the octahedron; quantization edge
-//                         cases (collapse_t(1e-3) = 429496, NaN, saturation)
+//                         cases (collapse_t(1e-3) = 429496, NaN, saturation);
+//                         a second restatement (numpy float32,
+//                         tests/cost_eval.py) equal on every edge of four
+//                         meshes; hand-built meshes (flat patch, exact QEM 0)
+//                         for the x5 skinny weight and the /num_tri divide
+//                         (closed form), the normal-flip -> UINT32_MAX branch,
+//                         and the unguarded quotient in double (D10: a case
+//                         where float evaluation differs); each of these
+//                         tests fails on an oracle with that term mutated
 //   claim                 brute-force per-face minimum over all edges, and the
 //                         winners' fans are disjoint
 //   collapse, restore,    per-round invariants over 12 rounds (closed,
 //   remove_invalid_faces  manifold, Euler characteristic, two faces per net
-//                         collapse, unchanged faces equal the snapshot)
+//                         collapse, unchanged faces equal the snapshot);
+//                         is_skinny's restore threshold formed in double:
+//                         collapses whose new face area sits exactly on, or
+//                         one ulp beside, thresholds that a float-formed
+//                         product would move by one ulp
+//   parity unpinned       CUDA's rounding of pow(float, 2) (D4) and the
+//                         overload CUDA picks for sqrt(int) (D10) are
+//                         definitions: no reference output fixes them
 //   remove_line_edges     sphere with one zero-length edge: the only collapse,
 //                         the two faces it kills, the faces it retargets and
 //                         the invariants of the compacted result

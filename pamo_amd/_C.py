@@ -121,3 +121,7 @@ class CUDSP:
         self._ctx.check(L.pamo_free_copy_outputs(self._ctx.handle, stream, verts.data_ptr(), tris.data_ptr(),
                                                  occ.data_ptr(), vmap.data_ptr(), None), "copy outputs")
         return verts, tris, occ, vmap
+
+    # the same debug exports as CUDSP_Free (fans, edges, quadrics, costs, claims)
+    _DEBUG = CUDSP_Free._DEBUG
+    debug = CUDSP_Free.debug

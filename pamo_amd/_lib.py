@@ -72,6 +72,8 @@ def lib():
     L.pamo_debug_pair_cap.argtypes = [P, LL]
     L.pamo_debug_pair_cap.restype = I
     L.pamo_debug_guard.argtypes = [P, I]
+    L.pamo_debug_walk_split.argtypes = [P, I]
+    L.pamo_debug_walk_split.restype = I
     L.pamo_set_strict.argtypes = [P, I]
     L.pamo_debug_guard.restype = I
     L.pamo_simplify.argtypes = [P, P, P, P, P, F]
@@ -128,6 +130,10 @@ class Context:
     def debug_guard(self, mode):
         """Candidate search of the engine's check: 0 by size, 1 LBVH, 2 exhaustive; test hook."""
         self.check(self._L.pamo_debug_guard(self.handle, int(mode)), "pamo_debug_guard")
+
+    def debug_walk_split(self, kshift):
+        """Lanes per query of the engine's LBVH walk, log2 (-1: by size); test hook."""
+        self.check(self._L.pamo_debug_walk_split(self.handle, int(kshift)), "pamo_debug_walk_split")
 
     def set_strict(self, strict):
         """Strict guard for the engine (off by default; not in the reference):

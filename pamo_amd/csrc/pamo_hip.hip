@@ -80,6 +80,7 @@ struct pamo_ctx {
   int run_handle = -1;          // engine slot of the last pamo_run_stage2
   long long pair_cap_limit = -1;  // test hook: caps the traversal's pair buffer (-1: none)
   int guard_mode = 0;             // test hook: 0 by mesh size, 1 tree only, 2 exhaustive only
+  int walk_split = -1;            // test hook: lanes per query of the LBVH walk, log2 (-1: by size)
   int strict = 0;                 // strict guard (engine; not in the reference)
 
   int* hc = nullptr;   // pinned host mirror of the counters
@@ -851,6 +852,12 @@ int pamo_debug_guard(pamo_ctx* ctx, int mode) {
   return 0;
 }
 
+int pamo_debug_walk_split(pamo_ctx* ctx, int kshift) {
+  if (!ctx || kshift < -1 || kshift > 3) return -1;
+  ctx->walk_split = kshift;
+  return 0;
+}
+
 int pamo_debug_pair_cap(pamo_ctx* ctx, long long cap) {
   if (!ctx) return -1;
   ctx->pair_cap_limit = cap;
@@ -1120,6 +1127,7 @@ int pamo_run_stage2_noalloc(pamo_ctx* ctx, void* stream, const float* d_verts, c
   c.shuffle_mode = ctx->shuffle_mode;
   c.guard_mode = ctx->guard_mode;
   c.pair_cap_limit = ctx->pair_cap_limit;
+  c.walk_split = ctx->walk_split;
   c.arena = nd.region;
   c.arena_cap = PAMO_NOALLOC_SCRATCH_BYTES;
   c.hc = c.hc_host;

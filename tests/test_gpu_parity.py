@@ -197,3 +197,53 @@ def test_pasp_run_matches_oracle_loop(cuda):
             break
     print("PaSP", f.shape, "->", of.shape, it + 1, flush=True)
     assert _same(gv.cpu().numpy(), ov) and _same(gf.cpu().numpy(), of)
+
+
+def _crafted_cases():
+    """Hand-built meshes of tests/cost_eval.py that each decide one branch:
+    (name, points, faces, scale, threshold, undo list or None, unguarded)."""
+    import cost_eval as ce
+    cases = [("skinny_penalty", *ce.flat_patch(), 1.0, 1.0, None, False),
+             ("normal_flip", *ce.flat_patch(lift=ce.FLIP_LIFT), 1.0, 1e-3, None, False),
+             ("d10_quotient", *ce.flat_patch(p=ce.D10_P), 1.0, 1.0, None, True)]
+    for i, (k, y, _) in enumerate(ce.RESTORE_CASES):
+        cases.append(("restore_threshold_%d" % i, *ce.restore_mesh(k, y), 1.0, 1e-3, ce.RESTORE_UNDO, False))
+    return cases
+
+
+@pytest.mark.parametrize("case", range(7))
+def test_crafted_branches_exact(cuda, case):
+    """The device's skinny weight and divide, normal flip, unguarded quotient
+    and skinny-restore threshold on the meshes that pin them in the oracle's
+    tests: every output and internal buffer bit for bit (identity shuffle;
+    the restore cases first size the tables with a call whose faces are all
+    removed, then pass the undo list)."""
+    name, P, F, s, tau, undo, sp = _crafted_cases()[case]
+    print("case", name, flush=True)
+    dv, df = torch.from_numpy(P).to(cuda), torch.from_numpy(F).to(cuda)
+    if sp:
+        g = _C.CUDSP(shuffle=oracle.SHUFFLE_IDENTITY)
+        o = oracle.Oracle(shuffle=oracle.SHUFFLE_IDENTITY)
+        gout = g.forward(dv, df, s, tau, True)
+        oout = o.forward_sp(P, F, s, tau, True)
+        for n, a, b in zip(NAMES, gout, oout):
+            assert _same(a.cpu().numpy(), b), n
+        assert _same(g.debug("edge_cost"), o.get("edge_cost"))
+        return
+    g = _C.CUDSP_Free(shuffle=oracle.SHUFFLE_IDENTITY)
+    o = oracle.Oracle(shuffle=oracle.SHUFFLE_IDENTITY)
+    none_g = torch.empty(0, dtype=torch.int32, device=cuda)
+    if undo is not None:
+        dead = np.full_like(F, -1)
+        g.forward(dv, torch.from_numpy(dead).to(cuda), none_g, 0, s, tau, False, True)
+        o.forward(P, dead, np.zeros(0, np.int32), s, tau, False, True)
+        ug = torch.from_numpy(undo).to(cuda)
+        gout = g.forward(dv, df, ug, ug.shape[0], s, tau, False, False)
+        oout = o.forward(P, F, undo, s, tau, False, False)
+    else:
+        gout = g.forward(dv, df, none_g, 0, s, tau, False, True)
+        oout = o.forward(P, F, np.zeros(0, np.int32), s, tau, False, True)
+    for n, a, b in zip(NAMES, gout, oout):
+        assert _same(a.cpu().numpy(), b), n
+    for n in ["edges", "Q", "edge_cost", "tri_min", "collapsed", "table"]:
+        assert _same(g.debug(n), o.get(n)), n

@@ -534,3 +534,149 @@ def test_strict_guard_removes_what_the_reference_keeps():
     a = oracle.run(verts, f, scale, int(0.01 * len(f)))
     b = oracle.run(verts, f, scale, int(0.01 * len(f)), strict=True)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+# ------------------------------------------- cost and collapse branches
+# Hand-built meshes (tests/cost_eval.py) that each decide one branch, so a
+# misreading shared by the oracle and the device would fail here.
+import warnings  # noqa: E402
+
+import cost_eval as ce  # noqa: E402
+
+
+def _identity_round(P, F, s, tau, undo=None, sp=False):
+    o = oracle.Oracle(shuffle=oracle.SHUFFLE_IDENTITY)
+    if sp:
+        o.forward_sp(P, F, s, tau, True)
+    elif undo is None:
+        o.forward(P, F, np.zeros(0, np.int32), s, tau, False, True)
+    else:
+        # a first call on the same vertices with every face removed sizes the
+        # invalid-vertex table, so the second call's undo list marks it (the
+        # table is written on the storage as it stands, cusimp_free.cu:895-916)
+        o.forward(P, np.full_like(F, -1), np.zeros(0, np.int32), s, tau, False, True)
+        o.forward(P, F, undo, s, tau, False, False)
+    return o
+
+
+def _cost_of(o, u, v):
+    E = o.get("edges")
+    idx = [i for i, (a, b) in enumerate(E) if (a, b) == (u, v)]
+    assert len(idx) == 1
+    return int(o.get("edge_cost")[idx[0]])
+
+
+def test_cost_restatement_matches_oracle_every_edge():
+    """A second restatement of the edge cost (numpy float32 scalars, written
+    separately from the C++ oracle) gives the oracle's quantized cost on every
+    edge whose cost the reference writes, guarded and unguarded."""
+    meshes = [ce.flat_patch(), ce.flat_patch(lift=ce.FLIP_LIFT), synth.displaced_icosphere(2, 5),
+              synth.noisy_torus(12, 8, 3)]
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")   # 0/0 of degenerate normals gives NaN, as in the reference
+        for P, F in meshes:
+            verts, scale, _ = oracle.preprocess(P, F)
+            for sp in (False, True):
+                o = _identity_round(verts, F, scale, 1e-3, sp=sp)
+                E, cost = o.get("edges"), o.get("edge_cost")
+                n = 0
+                for e, (a, b) in enumerate(E):
+                    want = ce.edge_cost(verts, F, int(a), int(b), scale, 1e-3, unguarded=sp)
+                    if want is None:
+                        continue
+                    n += 1
+                    assert int(cost[e]) == want, (F.shape, sp, a, b)
+                assert n >= 20
+
+
+def test_skinny_penalty_closed_form():
+    """Edge (u, v) of the flat patch, s = 1, tau = 1.  Both fans are flat, so
+    the QEM term is exactly 0.  Of the 8 moved faces (the fan faces without
+    the other endpoint: num_tri = 8) only (m, p, q) has a skinny term below 1:
+    with k = 1/8, m = (k/2, 0), p = (-k/2, k/10), q = (-k, 0), its area is
+    0.075 k^2, |m-p|^2 = 1.01 k^2, |p-q|^2 = 0.26 k^2, |q-m|^2 = 2.25 k^2, so
+    t = 4 sqrt3 * 0.075 / 1.01 + 2.51 k^2 + 1e-7 = 0.55369 (the reference's
+    precedence: only the area is divided by e01^2).  The other seven terms
+    are 1.76 to 3.51 and clamp to 1.  The cost is the length term plus
+    5 (1 - t) / 8 * tau (cusimp_free.cu:573-584); the penalty is about half
+    of it, so a lost x5 weight or /num_tri, or a wrong num_tri, moves the
+    cost far outside float rounding."""
+    k = 0.125
+    P, F = ce.flat_patch()
+    o = _identity_round(P, F, 1.0, 1.0)
+    got = _cost_of(o, 0, 1)
+    P64 = P.astype(np.float64)
+    el, ne = 0.0, 0
+    for f in [0, 1, 2, 3, 4, 5, 0, 5, 6, 7, 8, 9]:   # fan of u, then fan of v (ascending face ids)
+        i, j, kk = F[f]
+        for a, b in ((i, j), (j, kk), (kk, i)):
+            if a > b:
+                el += np.linalg.norm(P64[a] - P64[b])
+                ne += 1
+    L = el / ne + np.linalg.norm(P64[0] - P64[1])
+    t = 4 * np.sqrt(3) * 0.075 / 1.01 + (0.26 + 2.25) * k * k + 1e-7
+    penalty = 5.0 * (1.0 - t) / 8.0
+    want = (L + penalty) / 10.0 * 2.0 ** 32
+    assert abs(t - 0.55369) < 1e-5
+    assert penalty > 0.4 * (L + penalty)
+    assert abs(got - want) <= 1e-5 * want, (got, want)
+
+
+def test_normal_flip_sets_max_cost():
+    """A closed manifold mesh without intersections (verified) in which the
+    collapse of (u, v) to its midpoint turns the moved face (u, p, q) over:
+    the cost is UINT32_MAX (cusimp_free.cu:526-531,566-571).  The same
+    restatement without the flip test gives a finite cost, so the flip is
+    what decides it."""
+    P, F = ce.flat_patch(lift=ce.FLIP_LIFT)
+    assert pv.verify(P, F)["ok"]
+    o = _identity_round(P, F, 1.0, 1e-3)
+    assert _cost_of(o, 0, 1) == 0xFFFFFFFF
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        assert ce.edge_cost(P, F, 0, 1, 1.0, 1e-3) == 0xFFFFFFFF
+        assert ce.edge_cost(P, F, 0, 1, 1.0, 1e-3, flip_check=False) < 0xFFFFFFFF
+
+
+@pytest.mark.parametrize("case", range(len(ce.RESTORE_CASES)))
+def test_skinny_restore_threshold_formed_in_double(case):
+    """The winning collapse of (u, v) leaves the face (m, p, q) with a float
+    area exactly at, or one ulp beside, the restore threshold
+    1e-6 * max_len^2 (cusimp_free.cu:617-637).  1e-6 is a double literal, so
+    the threshold is formed in double and rounded once; at these scales a
+    float-formed threshold is one ulp away, and the face at the lower of the
+    two would be judged the other way.  The collapse is restored (v occupied
+    again, u back at its position) exactly when area < the double-formed
+    threshold."""
+    k, y, restored = ce.RESTORE_CASES[case]
+    P, F = ce.restore_mesh(k, y)
+    assert pv.verify(P, F)["ok"]
+    m = (P[0] + P[1]) / np.float32(2)
+    area, thr_d, thr_f = ce.skinny_thresholds(m, P[3], P[4])
+    assert thr_d != thr_f
+    assert (area < thr_d) == restored
+    if area in (thr_d, thr_f):
+        assert (area < thr_f) != restored     # a float-formed threshold decides the other way
+    o = _identity_round(P, F, 1.0, 1e-3, undo=ce.RESTORE_UNDO)
+    E = o.get("edges")
+    assert [tuple(int(c) for c in E[e]) for e in o.get("collapsed")] == [(0, 1)]
+    occ, pts = o.get("occ"), o.get("points")
+    assert int(occ[1]) == (1 if restored else 0)
+    assert pts[0].tobytes() == (P[0] if restored else m).tobytes()
+
+
+def test_unguarded_quotient_in_double():
+    """The unguarded variant writes sqrt(3) with an integer argument
+    (cusimp.cu:423,463), the double overload, so its skinny quotient is
+    formed in double and rounded to float once (D10).  At this geometry the
+    double and a float evaluation give quantized costs 16 apart; the oracle
+    gives the double one."""
+    P, F = ce.flat_patch(p=ce.D10_P)
+    o = _identity_round(P, F, 1.0, 1.0, sp=True)
+    got = _cost_of(o, 0, 1)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        want = ce.edge_cost(P, F, 0, 1, 1.0, 1.0, unguarded=True, d10=True)
+        alt = ce.edge_cost(P, F, 0, 1, 1.0, 1.0, unguarded=True, d10=False)
+    assert want != alt
+    assert got == want
