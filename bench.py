@@ -39,6 +39,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "input Mfaces/s to 0.1% (plain vs batched/served); p50/p99 job latency"
+EXIT_NAMES = {1: "target", 2: "10 faces", 3: "tolerance", 4: "iteration cap"}
 
 
 def log(*a):
@@ -70,6 +72,11 @@ def parse():
     p.add_argument("--no-classes", action="store_true",
                    help="skip the extra untimed run that times every kernel class (for kernel traces of one run)")
     p.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic.json"))
+    p.add_argument("--no-legs", action="store_true",
+                   help="sdf2m: skip the batched and served legs of the default line")
+    p.add_argument("--leg-meshes", type=int, default=512, help="batched leg: meshes of the many-mesh mix")
+    p.add_argument("--leg-jobs", type=int, default=1000, help="served leg: jobs of the Poisson stream")
+    p.add_argument("--leg-rate", type=float, default=300.0, help="served leg: arrival rate in jobs/s")
     p.add_argument("--copy-probe", type=int, default=0,
                    help="MiB copied device-to-device once before timing: a copy kernel of known size that "
                         "calibrates the FETCH_SIZE / WRITE_SIZE counters in a rocprofv3 --pmc run")
@@ -183,14 +190,19 @@ def run_plain(args):
         from pamo_amd import meshio
         meshio.save(args.save_mesh, out_v, out_f)
     round_faces = [int(x) for x in st["round_faces"]]
-    result = dict(metric="input Mfaces/s to %g%% (plain, one mesh)" % (100 * ratio), value=round(value, 3),
+    result = dict(metric=METRIC, value=round(value, 3),
                   unit="Mfaces/s",
                   n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=round(1e3 * sec, 3),
                   higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32",
                   data="synthetic (seeded SDF mesh generated in-process)",
                   config=dict(cfg, faces=F0, vertices=int(v.shape[0]), ratio=ratio, tolerance=4, threshold=1e-3,
                               min_verts=0, target_faces=target, rounds=st["rounds"], final_faces=int(out_f.shape[0]),
-                              exit_reason=st["exit_reason"], parallelism="single GPU", strict_guard=args.strict,
+                              exit_reason=st["exit_reason"],
+                              stop="target %d faces (%g%%) or %d rounds without change (tolerance); this run "
+                                   "ends by %s at %d faces (%.3f%%)" % (
+                                       target, 100 * ratio, 4, EXIT_NAMES.get(st["exit_reason"], "?"),
+                                       int(out_f.shape[0]), 100.0 * out_f.shape[0] / F0),
+                              parallelism="single GPU", strict_guard=args.strict,
                               guard_checks=st["checks"], query_faces=st["query_faces"],
                               candidate_pairs=st["candidate_pairs"]),
                   latency_ms=dict(p50=round(1e3 * float(np.percentile(times, 50)), 3),
@@ -217,7 +229,140 @@ def run_plain(args):
         result["verify"] = {k: r[k] for k in ("manifold", "watertight", "degenerate_faces", "intersecting_pairs",
                                                "ok")}
         result["verify"]["max_undo_passes"] = st["max_undo_passes"]
+    if args.workload == "sdf2m" and not args.no_legs and not args.strict:
+        del dv, df, ov, of
+        ctx.close()
+        torch.cuda.empty_cache()
+        result["batched"] = _batched_leg(args)
+        torch.cuda.empty_cache()
+        result["served"] = _served_leg(args)
     print(json.dumps(result), flush=True)
+
+
+def _plain_each(meshes, ctx, with_prep):
+    """The plain path (pamo_run_stage2, one mesh at a time) on every mesh:
+    per-mesh seconds and outputs.  with_prep: the timed region also holds
+    the job's host preparation and upload (what a one-job-at-a-time server
+    pays per job); otherwise the inputs are already on the device."""
+    import torch
+    from pamo_amd import batch, simplify_stage2
+    dev = torch.device("cuda:0")
+    out = []
+    t_log = time.perf_counter()
+    for n, (v, f, r) in enumerate(meshes):
+        if not with_prep:
+            dv, dt, sc, m = batch.prepare(v, f, dev)
+            torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if with_prep:
+            dv, dt, sc, m = batch.prepare(v, f, dev)
+        ov, of, st = simplify_stage2(dv, dt, sc, int(r * len(f)), ctx=ctx)
+        torch.cuda.synchronize()
+        out.append((time.perf_counter() - t0, ov, of, m))
+        if time.perf_counter() - t_log > 10:
+            t_log = time.perf_counter()
+            log("plain path: %d of %d meshes" % (n + 1, len(meshes)))
+    return out
+
+
+def _batched_leg(args):
+    """The many-mesh mix (512 meshes, log-uniform 2k-400k faces, three
+    families, ratios 0.01/0.005/0.001, seeds 4000000+i) in one packed
+    engine, inputs resident in HBM when timing starts; against the plain
+    path on every mesh, one at a time, with the same resident inputs.  Every
+    batched result must equal the plain one bit for bit."""
+    import torch
+    from pamo_amd import _lib, batch, synth
+    meshes = _gen(synth.batch_specs(args.leg_meshes))
+    F_total = sum(len(m[1]) for m in meshes)
+    dev = torch.device("cuda:0")
+    prepared = [batch.prepare(v, f, dev) for v, f, r in meshes]
+    torch.cuda.synchronize()
+    ctx = _lib.Context(0)
+    eng = batch.BatchEngine(ctx=ctx)
+    times, outs = [], {}
+    for s in range(2):   # the first run sizes the engine's buffers
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        handles = {}
+        for i, ((dv, dt, sc, m), (v, f, r)) in enumerate(zip(prepared, meshes)):
+            handles[eng.submit(dv, dt, sc, int(r * len(f)), job=i)] = i
+        pending = len(meshes)
+        while pending:
+            _, active = eng.step(1)
+            for h, _ in eng.poll():
+                outs[handles[h]] = eng.take(h)[:2]
+                pending -= 1
+            if active == 0 and pending:
+                raise RuntimeError("engine idle with %d meshes pending" % pending)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        log("batched leg run %d: %.1f ms" % (s, 1e3 * times[-1]))
+    del eng, prepared
+    plain = _plain_each(meshes, ctx, with_prep=False)
+    t_plain = sum(p[0] for p in plain)
+    same = sum(int(torch.equal(outs[i][0], plain[i][1]) and torch.equal(outs[i][1], plain[i][2]))
+               for i in range(len(meshes)))
+    sec = times[-1]
+    return dict(value=round(F_total / sec / 1e6, 3), unit="Mfaces/s", meshes=len(meshes), faces=F_total,
+                face_range=[int(min(len(m[1]) for m in meshes)), int(max(len(m[1]) for m in meshes))],
+                seconds=round(sec, 4), workload="batch512: 512 meshes, families in thirds, seeds 4000000+i",
+                plain=dict(value=round(F_total / t_plain / 1e6, 3), unit="Mfaces/s", seconds=round(t_plain, 3),
+                           meshes=len(meshes)),
+                speedup_over_plain=round(t_plain / sec, 2), results_equal_plain="%d of %d" % (same, len(meshes)))
+
+
+def _served_leg(args):
+    """A Poisson stream (arrival seed 5) of the serving workload's jobs
+    (seeds 5000000+i) through the continuous-batching server, latency from
+    arrival to result on the host.  The plain path serves the same stream
+    one job at a time: every job's service time (host preparation, upload,
+    pamo_run_stage2) is measured, and its latencies are those of a FIFO
+    queue over the same arrivals with those times.  Every served result
+    must equal the plain one bit for bit."""
+    import numpy as np
+    import torch
+    from pamo_amd import _lib, batch, synth
+    meshes = _gen(synth.batch_specs(args.leg_jobs, seed_base=5_000_000))
+    F_total = sum(len(m[1]) for m in meshes)
+    rate = args.leg_rate
+    rng = np.random.default_rng(5)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(meshes)))
+    jobs = [dict(points=v, faces=f, ratio=r, arrival=float(t)) for (v, f, r), t in zip(meshes, arrivals)]
+    ctx = _lib.Context(0)
+    # warm-up: a few jobs through the plain path load every kernel
+    _plain_each(meshes[:4], ctx, with_prep=True)
+    got = {}
+    srv = batch.Server(engine=batch.BatchEngine(ctx=ctx))
+    t0 = time.perf_counter()
+    recs = srv.run(jobs, on_result=lambda i, v, f, info: got.__setitem__(i, (v, f)))
+    log("served leg: %d jobs in %.1fs" % (len(jobs), time.perf_counter() - t0))
+    lat = np.array([r["latency"] for r in recs])
+    span = max(r["finished"] for r in recs) - float(arrivals[0])
+    plain = _plain_each(meshes, ctx, with_prep=True)
+    svc = np.array([p[0] for p in plain])
+    done, seq_lat = 0.0, []
+    for t_arr, s_i in zip(arrivals, svc):
+        done = max(done, t_arr) + s_i
+        seq_lat.append(done - t_arr)
+    seq_lat = np.array(seq_lat)
+    same = sum(int(got[i][0].tobytes() == (plain[i][1].cpu().numpy() + plain[i][3]).tobytes() and
+                   got[i][1].tobytes() == plain[i][2].cpu().numpy().tobytes()) for i in range(len(meshes)))
+    served = F_total / span / 1e6
+    cap = F_total / float(svc.sum()) / 1e6
+    return dict(value=round(served, 3), unit="Mfaces/s", rate_jobs_s=rate, jobs=len(jobs), faces=F_total,
+                jobs_per_s=round(len(jobs) / span, 2), workload="serve: Poisson arrivals of the batch mix, "
+                "seeds 5000000+i, arrival seed 5, HBM budget 200 GB",
+                latency_ms=dict(p50=round(1e3 * float(np.percentile(lat, 50)), 2),
+                                p99=round(1e3 * float(np.percentile(lat, 99)), 2),
+                                max=round(1e3 * float(lat.max()), 2)),
+                plain=dict(capacity=round(cap, 3), unit="Mfaces/s", capacity_jobs_s=round(len(jobs) / svc.sum(), 2),
+                           latency_ms=dict(p50=round(1e3 * float(np.percentile(seq_lat, 50)), 2),
+                                           p99=round(1e3 * float(np.percentile(seq_lat, 99)), 2)),
+                           service="measured for every job (preparation, upload, pamo_run_stage2); FIFO queue over "
+                                   "the same arrivals"),
+                served_over_plain=round(served / cap, 2), results_equal_plain="%d of %d" % (same, len(meshes)),
+                max_live_jobs=srv.stats.get("max_live"))
 
 
 def _cpu_plain(verts, f, scale, target, round_faces, threads, seconds):

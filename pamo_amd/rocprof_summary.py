@@ -4,6 +4,7 @@ tables: per kernel name, dispatches, total and average duration; with
 summaries under profiles/; reads only the database it is given.
 
     python3 rocprof_summary.py DB [TOP] [--rounds]
+    python3 rocprof_summary.py --pmc AGENT_CSV COUNTER_CSV [COUNTER_CSV ...]
 """
 import sqlite3
 import sys
@@ -65,8 +66,66 @@ def rounds(db, top=3):
     return "\n".join(out)
 
 
+def _agent(agent_csv):
+    import csv
+    for row in csv.DictReader(open(agent_csv)):
+        if row["Agent_Type"] == "GPU":
+            return dict(cu=int(row["Cu_Count"]), xcc=int(row["Num_Xcc"]), waves_per_cu=int(row["Max_Waves_Per_Cu"]))
+    raise ValueError("no GPU agent in %s" % agent_csv)
+
+
+def pmc(counter_csvs, agent_csv):
+    """Per kernel, from rocprofv3 --pmc counter CSVs (one or more passes of
+    the same run): dispatches, summed counters and the derived metrics
+    with AMD's expressions.  GRBM_GUI_ACTIVE is summed over the XCCs in the
+    CSV, so the kernel's own cycles are that sum / XCC count.
+      occupancy %       = 4 SQ_WAVE_CYCLES / (cycles x CUs x max waves per CU)
+                          (SQ_WAVE_CYCLES is in quad-cycles)
+      VALU busy %       = SQ_ACTIVE_INST_VALU / CUs / cycles
+      VALU utilization % = SQ_THREAD_CYCLES_VALU / (SQ_ACTIVE_INST_VALU x 64)
+      LDS bank conflict = SQ_LDS_BANK_CONFLICT / (SQ_LDS_IDX_ACTIVE - SQ_LDS_BANK_CONFLICT)
+                          (conflict cycles per conflict-free LDS cycle)"""
+    import csv
+    ag = _agent(agent_csv)
+    k = {}
+    for path in counter_csvs:
+        for row in csv.DictReader(open(path)):
+            name = row["Kernel_Name"].split("(")[0]
+            d = k.setdefault(name, dict(dispatch=set(), c={}, vgpr=row["VGPR_Count"], agpr=row["Accum_VGPR_Count"],
+                                        lds=row["LDS_Block_Size"], wg=row["Workgroup_Size"]))
+            d["dispatch"].add(row["Dispatch_Id"])
+            cn = row["Counter_Name"]
+            key = (cn, path)   # the same counter in two passes is kept apart
+            d["c"][key] = d["c"].get(key, 0.0) + float(row["Counter_Value"])
+    out = ["| kernel | dispatches | VGPR | LDS B | occupancy % | VALU busy % | VALU util % | VALU inst / wave | "
+           "LDS bank conflict | wait / wave cycles |", "|---|---|---|---|---|---|---|---|---|---|"]
+    for name, d in sorted(k.items()):
+        def c(cn):
+            vals = [v for (n, _), v in d["c"].items() if n == cn]
+            return vals[0] if vals else None
+        def ratio(a, b, scale=1.0):
+            return None if a is None or not b else scale * a / b
+        cycles = {p: v / ag["xcc"] for (n, p), v in d["c"].items() if n == "GRBM_GUI_ACTIVE"}
+        cyc_for = lambda cn: next((cycles[p] for (n, p) in d["c"] if n == cn and p in cycles), None)
+        occ = ratio(c("SQ_WAVE_CYCLES"), (cyc_for("SQ_WAVE_CYCLES") or 0) * ag["cu"] * ag["waves_per_cu"], 400.0)
+        busy = ratio(c("SQ_ACTIVE_INST_VALU"), (cyc_for("SQ_ACTIVE_INST_VALU") or 0) * ag["cu"], 100.0)
+        util = ratio(c("SQ_THREAD_CYCLES_VALU"), (c("SQ_ACTIVE_INST_VALU") or 0) * 64, 100.0)
+        ipw = ratio(c("SQ_INSTS_VALU"), c("SQ_WAVES"))
+        bc, idx = c("SQ_LDS_BANK_CONFLICT"), c("SQ_LDS_IDX_ACTIVE")
+        lbc = None if bc is None or idx is None or idx - bc <= 0 else bc / (idx - bc)
+        wait = ratio(c("SQ_WAIT_INST_ANY"), c("SQ_WAVE_CYCLES"))
+        f = lambda x, fmt="%.1f": "-" if x is None else fmt % x
+        out.append("| %s | %d | %s | %s | %s | %s | %s | %s | %s | %s |" % (
+            name.split("::")[-1], len(d["dispatch"]), d["vgpr"], d["lds"], f(occ), f(busy), f(util), f(ipw, "%.0f"),
+            f(lbc, "%.3f"), f(wait, "%.2f")))
+    return "\n".join(out)
+
+
 if __name__ == "__main__":
     args = [a for a in sys.argv[1:] if not a.startswith("--")]
+    if "--pmc" in sys.argv:   # --pmc AGENT_CSV COUNTER_CSV [COUNTER_CSV ...]
+        print(pmc(args[1:], args[0]))
+        sys.exit(0)
     print(summary(args[0], int(args[1]) if len(args) > 1 else 30))
     if "--rounds" in sys.argv:
         print()

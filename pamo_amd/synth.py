@@ -213,9 +213,17 @@ def mesh_for_faces(kind, target_faces, seed, threads=0):
     if kind == "sdf":
         rng = np.random.default_rng(seed)
         ns = int(rng.integers(5, 40))
-        # faces grow roughly with the square of the grid resolution
+        # faces grow roughly with the square of the grid resolution; a union
+        # of few small spheres falls short of that estimate, so the lattice
+        # is refined (faces ~ grid^2) until the mesh has at least 0.7 x target
         grid = int(np.clip(np.sqrt(target_faces / (2.2 * max(ns, 1) ** 0.5)) * 1.0, 24, 700))
-        return sdf_union(ns, grid, seed, threads=threads)
+        v, f = sdf_union(ns, grid, seed, threads=threads)
+        for _ in range(4):
+            if f.shape[0] >= 0.7 * target_faces or grid >= 700:
+                break
+            grid = int(min(700, np.ceil(grid * np.sqrt(target_faces / max(f.shape[0], 1)))))
+            v, f = sdf_union(ns, grid, seed, threads=threads)
+        return v, f
     raise ValueError(kind)
 
 
