@@ -1734,7 +1734,20 @@ __global__ void __launch_bounds__(BF_NT) k_bf_waves(
     if (ov) plist[base + __popcll(m & ((1ull << t) - 1))] = make_int2((int)bx, (int)tt);
   }
 }
-__global__ void __launch_bounds__(BF_NT) k_bf_emit(
+// A listed pair's tile may be split over WAVES waves: each wave takes every
+// WAVES-th group of the tile for the same BF_NT queries (lane = query), with
+// its own candidate lists, so a pair's serial chain of group tests is WAVES
+// times shorter.  Small checks list few pairs and wait on their longest
+// chain (one wave per pair: 40 us for a 20k-face check, 13-17 us with four
+// waves); large checks list many and are bound by throughput, where one wave
+// per pair pays each pair's fixed costs (tile load, reservation) once.  The
+// candidate set is the same either way: each group is tested by one wave.
+constexpr int BF_WAVES_SMALL = 4;
+constexpr long long BF_SPLIT_MAX_FACES = 65536;   // alive faces of a check that take the split
+template <int WAVES> struct BfCap { static constexpr int value = WAVES == 1 ? 32 : 16; };
+static_assert((BF_TILE / BF_GRP) % BF_WAVES_SMALL == 0, "groups split evenly over the waves");
+template <int WAVES>
+__global__ void __launch_bounds__(BF_NT * WAVES) k_bf_emit(
     const int2* __restrict__ plist, const int* __restrict__ npair, const int4* __restrict__ wrec,
     const float4* __restrict__ wlo, const float4* __restrict__ whi, const float4* __restrict__ qlo,
     const float4* __restrict__ qhi, const int* __restrict__ qlist, int F, const int* __restrict__ sid,
@@ -1742,11 +1755,15 @@ __global__ void __launch_bounds__(BF_NT) k_bf_emit(
     const float4* __restrict__ guhi, int2* __restrict__ pairs, long long pair_cap,
     unsigned long long* __restrict__ npairs, int* __restrict__ ovf, int* __restrict__ novf,
     int* __restrict__ oflag) {
-  __shared__ int cl[CAND_CAP * BF_NT];
+  constexpr int BF_CAP = BfCap<WAVES>::value;   // candidate slots per lane and wave before a flush
+  __shared__ int cls[WAVES][BF_CAP * BF_NT];
   __shared__ float4 blo[BF_TILE], bhi[BF_TILE];
   __shared__ int bid[BF_TILE];
   __shared__ float4 glo[BF_TILE / BF_GRP], ghi[BF_TILE / BF_GRP];
-  const int t = threadIdx.x;   // == lane
+  const int tid = threadIdx.x;
+  const int t = tid & (BF_NT - 1);   // lane == query of the wave
+  const int wv = tid / BF_NT;        // this wave's share of the tile's groups
+  int* cl = cls[wv];
   const int np = *npair;
   for (int it = blockIdx.x; it < np; it += gridDim.x) {   // workgroup-uniform loop
     // the pair, then the wave's record and the tile together (the tile's
@@ -1756,29 +1773,33 @@ __global__ void __launch_bounds__(BF_NT) k_bf_emit(
     const int4 wr = wrec[bx];
     const long long a0 = wr.z, a1 = wr.w;
     __syncthreads();   // the previous pair's boxes are no longer read
-    float4 tl[BF_TILE / BF_NT], th[BF_TILE / BF_NT];
-    int ti[BF_TILE / BF_NT];
+    {
+      // the tile's boxes, all loads issued before the LDS stores
+      constexpr int R = BF_TILE / (BF_NT * WAVES);
+      float4 tl[R], th[R];
+      int ti[R];
 #pragma unroll
-    for (int r = 0; r < BF_TILE / BF_NT; r++) {
-      const long long p = min(tile0 + r * BF_NT + t, (long long)F - 1);
-      tl[r] = slo[p];
-      th[r] = shi[p];
-      ti[r] = sid[p];
-    }
-    // group unions (precomputed over whole groups; at a mesh boundary they
-    // also cover the other mesh's boxes, which only weakens the culling)
-    if (t < BF_TILE / BF_GRP) {
-      glo[t] = gulo[tile0 / BF_GRP + t];
-      ghi[t] = guhi[tile0 / BF_GRP + t];
-    }
+      for (int r = 0; r < R; r++) {
+        const long long pc = min(tile0 + r * BF_NT * WAVES + tid, (long long)F - 1);
+        tl[r] = slo[pc];
+        th[r] = shi[pc];
+        ti[r] = sid[pc];
+      }
+      // group unions (precomputed over whole groups; at a mesh boundary they
+      // also cover the other mesh's boxes, which only weakens the culling)
+      if (tid < BF_TILE / BF_GRP) {
+        glo[tid] = gulo[tile0 / BF_GRP + tid];
+        ghi[tid] = guhi[tile0 / BF_GRP + tid];
+      }
 #pragma unroll
-    for (int r = 0; r < BF_TILE / BF_NT; r++) {
-      const int j = r * BF_NT + t;
-      const long long p = tile0 + j;
-      const bool in = p >= a0 && p < a1;   // other meshes' positions: never a candidate
-      blo[j] = in ? tl[r] : make_float4(INFINITY, INFINITY, INFINITY, 0.f);
-      bhi[j] = in ? th[r] : make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
-      bid[j] = in ? ti[r] : -1;
+      for (int r = 0; r < R; r++) {
+        const int j = r * BF_NT * WAVES + tid;
+        const long long p = tile0 + j;
+        const bool in = p >= a0 && p < a1;   // other meshes' positions: never a candidate
+        blo[j] = in ? tl[r] : make_float4(INFINITY, INFINITY, INFINITY, 0.f);
+        bhi[j] = in ? th[r] : make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+        bid[j] = in ? ti[r] : -1;
+      }
     }
     __syncthreads();
     const long long qi = (long long)wr.x + t;
@@ -1805,7 +1826,7 @@ __global__ void __launch_bounds__(BF_NT) k_bf_emit(
     // moves all lists to the pair buffer with one prefix sum and one
     // atomic.  If the buffer is full the wave stops, and all its queries
     // take the overflow walk.
-    for (int gi = 0; gi < BF_TILE / BF_GRP && !full; gi++) {
+    for (int gi = wv; gi < BF_TILE / BF_GRP && !full; gi += WAVES) {
       const float4 gl = glo[gi], gh = ghi[gi];
       if ((x0 < gl.x) | (gh.x < w0) | (x1 < gl.y) | (gh.y < w1) | (x2 < gl.z) | (gh.z < w2)) continue;
       // the group is skipped when no lane's own box meets its union (lanes
@@ -1815,7 +1836,7 @@ __global__ void __launch_bounds__(BF_NT) k_bf_emit(
                    (gh.z < qlo2))))
         continue;
       for (int j0 = gi * BF_GRP; j0 < gi * BF_GRP + BF_GRP; j0 += 4) {
-        if (__any(cnt > CAND_CAP - 4)) {
+        if (__any(cnt > BF_CAP - 4)) {
           int incl = cnt;
 #pragma unroll
           for (int o = 1; o < 64; o <<= 1) {
