@@ -61,9 +61,14 @@ def parse():
                    help="SDF lattice (default: 300 for sdf2m, ~2.0M faces; 584 for sdf7m, ~7.05M faces)")
     p.add_argument("--cpu1-seconds", type=float, default=10.0,
                    help="budget of the single-thread CPU sample (0: skip)")
-    p.add_argument("--cpu-seconds", type=float, default=20.0)
+    p.add_argument("--cpu-seconds", type=float, default=20.0,
+                   help="budget of the all-cores CPU sample where the oracle does not run to completion (sdf7m)")
+    p.add_argument("--fp64-seconds", type=float, default=150.0,
+                   help="budget of the whole-run FP64 shadow (it stops after the round that passes it)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-verify", action="store_true")
+    p.add_argument("--placement", default="midpoint", choices=["midpoint", "optimal"],
+                   help="plain workloads: where a collapse places its vertex (optimal: not the reference's)")
     p.add_argument("--strict", action="store_true",
                    help="plain workloads: the strict guard (exact pair test, undo until clean; not the reference's)")
     p.add_argument("--save-mesh", default="",
@@ -140,6 +145,7 @@ def run_plain(args):
     h2d_ms = 1e3 * (time.perf_counter() - t_up)   # host buffers -> HBM (pageable), outside the timed region
     ctx = _lib.Context(0)
     ctx.set_strict(args.strict)
+    ctx.set_placement(args.placement)
     if args.copy_probe:
         src = torch.ones(args.copy_probe << 20, dtype=torch.uint8, device=dev)
         dst = torch.empty_like(src)
@@ -202,7 +208,7 @@ def run_plain(args):
                                    "ends by %s at %d faces (%.3f%%)" % (
                                        target, 100 * ratio, 4, EXIT_NAMES.get(st["exit_reason"], "?"),
                                        int(out_f.shape[0]), 100.0 * out_f.shape[0] / F0),
-                              parallelism="single GPU", strict_guard=args.strict,
+                              parallelism="single GPU", strict_guard=args.strict, placement=args.placement,
                               guard_checks=st["checks"], query_faces=st["query_faces"],
                               candidate_pairs=st["candidate_pairs"]),
                   latency_ms=dict(p50=round(1e3 * float(np.percentile(times, 50)), 3),
@@ -211,25 +217,36 @@ def run_plain(args):
                   h2d_ms=round(h2d_ms, 3))
     if not args.no_cpu:
         ncores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
-        result["cpu_baseline"] = _cpu_plain(verts, f, scale, target, round_faces, ncores, args.cpu_seconds)
+        if args.workload == "sdf7m":
+            result["cpu_baseline"] = _cpu_plain(verts, f, scale, target, round_faces, ncores, args.cpu_seconds)
+        else:
+            result["cpu_baseline"] = _cpu_full(verts, f, scale, target, round_faces, ncores, out_v, out_f,
+                                               args.placement)
         if args.cpu1_seconds > 0:
             one = _cpu_plain(verts, f, scale, target, round_faces, 1, args.cpu1_seconds)
             result["cpu_baseline"]["one_thread"] = {k: one[k] for k in ("value", "unit", "cores", "sample",
                                                                         "rounds_match_gpu")}
         # FP32 (the reference's arithmetic, which the GPU matches bit for bit)
-        # against the same expressions in FP64, on the first round
+        # against FP64: every round's costs and winners, and the output's
+        # positions against exact averaging (oracle.fp64_shadow)
         import oracle
         oracle.set_threads(ncores)
-        t0 = time.perf_counter()
-        result["fp64_error"] = dict(oracle.precision_report(verts, f, scale), scope="first round, unshuffled faces",
-                                    seconds=round(time.perf_counter() - t0, 2))
+        budget = args.fp64_seconds
+        log("fp64 shadow: %d threads, budget %.0fs" % (ncores, budget))
+        sh = oracle.fp64_shadow(verts, f, scale, target, max_seconds=budget)
+        result["fp64_error"] = dict(sh, scope="every round of the run" if sh["exit_reason"] != 5 else
+                                    "the first %d rounds (time budget %gs)" % (sh["rounds"], budget),
+                                    position_unit="fraction of the scale (largest extent)")
     if not args.no_verify:
         from pamo_amd.verify import verify
         r = verify(out_v, out_f)
         result["verify"] = {k: r[k] for k in ("manifold", "watertight", "degenerate_faces", "intersecting_pairs",
                                                "ok")}
         result["verify"]["max_undo_passes"] = st["max_undo_passes"]
-    if args.workload == "sdf2m" and not args.no_legs and not args.strict:
+    if args.workload == "sdf2m" and not args.no_legs and not args.strict and args.placement == "midpoint":
+        result["placement_optimal"] = _placement_leg(args, ctx, dv, df, scale, target, verts, out_v, out_f, sec,
+                                                     st["rounds"])
+    if args.workload == "sdf2m" and not args.no_legs and not args.strict and args.placement == "midpoint":
         del dv, df, ov, of
         ctx.close()
         torch.cuda.empty_cache()
@@ -237,6 +254,77 @@ def run_plain(args):
         torch.cuda.empty_cache()
         result["served"] = _served_leg(args)
     print(json.dumps(result), flush=True)
+
+
+def surface_distance(points, out_v, out_f, n=20000, seed=0):
+    """Distance from a seeded sample of n input points to the output surface
+    (closest point on each triangle: the plane projection when it falls
+    inside, else the nearest edge point), in float64 on the GPU.  Returns
+    mean and max, relative to the input's bounding-box diagonal."""
+    import numpy as np
+    import torch
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(seed)
+    idx = rng.choice(len(points), size=min(n, len(points)), replace=False)
+    p = torch.from_numpy(points[idx].astype(np.float64)).to(dev)
+    tri = torch.from_numpy(out_v.astype(np.float64)[out_f]).to(dev)   # [F,3,3]
+    a, b, c = tri[:, 0], tri[:, 1], tri[:, 2]
+    ab, ac = b - a, c - a
+    nrm = torch.cross(ab, ac, dim=1)
+    nn = (nrm * nrm).sum(1)
+    g00, g01, g11 = (ab * ab).sum(1), (ab * ac).sum(1), (ac * ac).sum(1)
+    det = g00 * g11 - g01 * g01
+    best = []
+    for i in range(0, len(p), 1024):
+        q = p[i:i + 1024, None, :]                                   # [n,1,3]
+        ap = q - a[None]
+        d1, d2 = (ap * ab[None]).sum(2), (ap * ac[None]).sum(2)
+        s_ = (g11 * d1 - g01 * d2) / det
+        t_ = (g00 * d2 - g01 * d1) / det
+        inside = (s_ >= 0) & (t_ >= 0) & (s_ + t_ <= 1) & (nn > 0)
+        dist = torch.where(inside, ((ap * nrm[None]).sum(2)).abs() / nn.sqrt(), torch.full_like(d1, float("inf")))
+        for u, v in ((a, b), (b, c), (c, a)):
+            e = v - u
+            h = (((q - u[None]) * e[None]).sum(2) / (e * e).sum(1)).clamp(0, 1)
+            dist = torch.minimum(dist, (q - (u[None] + h[..., None] * e[None])).norm(dim=2))
+        best.append(dist.min(1).values)
+    d = torch.cat(best)
+    diag = float(np.linalg.norm(points.max(0) - points.min(0)))
+    return dict(mean=round(float(d.mean()) / diag, 7), max=round(float(d.max()) / diag, 7))
+
+
+def _placement_leg(args, ctx, dv, df, scale, target, verts, out_v, out_f, sec_mid, rounds_mid):
+    """The opt-in optimal placement (not the reference's) on the same mesh and
+    target: its time, rounds and faces beside the midpoint run's, and both
+    outputs' distance to the input surface."""
+    import numpy as np
+    import torch
+    from pamo_amd import simplify_stage2
+    ctx.set_placement("optimal")
+    simplify_stage2(dv, df, scale, target, ctx=ctx)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ov, of, st = simplify_stage2(dv, df, scale, target, ctx=ctx)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    ctx.set_placement("midpoint")
+    sec = float(np.mean(times))
+    o_v, o_f = ov.cpu().numpy(), of.cpu().numpy()
+    out = dict(ms_per_step=round(1e3 * sec, 3), value=round(len(df) / sec / 1e6, 3), unit="Mfaces/s",
+               rounds=st["rounds"], final_faces=int(o_f.shape[0]),
+               midpoint=dict(ms_per_step=round(1e3 * sec_mid, 3), rounds=rounds_mid, final_faces=int(out_f.shape[0])),
+               surface_distance=dict(optimal=surface_distance(verts, o_v, o_f),
+                                     midpoint=surface_distance(verts, out_v, out_f),
+                                     measure="input-vertex sample (20,000, seed 0) to output surface, "
+                                             "/ bounding-box diagonal"))
+    if not args.no_verify:
+        from pamo_amd.verify import verify
+        r = verify(o_v, o_f)
+        out["verify"] = {k: r[k] for k in ("manifold", "watertight", "intersecting_pairs", "ok")}
+    return out
 
 
 def _plain_each(meshes, ctx, with_prep):
@@ -363,6 +451,23 @@ def _served_leg(args):
                                    "the same arrivals"),
                 served_over_plain=round(served / cap, 2), results_equal_plain="%d of %d" % (same, len(meshes)),
                 max_live_jobs=srv.stats.get("max_live"))
+
+
+def _cpu_full(verts, f, scale, target, round_faces, threads, out_v, out_f, placement):
+    """The oracle's whole loop on the host with the given threads, timed over
+    the stage-2 window; its output against the GPU's, bit for bit."""
+    import numpy as np
+    import oracle
+    threads = oracle.set_threads(threads)
+    log("cpu oracle: %d threads, full run" % threads)
+    t0 = time.perf_counter()
+    ov, of, st, orf = oracle.run(verts, f, scale, target, return_rounds=True,
+                                 placement=1 if placement == "optimal" else 0)
+    sec = time.perf_counter() - t0
+    same = ov.tobytes() == np.ascontiguousarray(out_v).tobytes() and of.tobytes() == np.ascontiguousarray(out_f).tobytes()
+    return dict(value=round(int(f.shape[0]) / sec / 1e6, 4), unit="Mfaces/s", cores=threads, kind="port",
+                sample="the whole run (%d rounds) in %.1fs" % (st["rounds"], sec),
+                rounds_match_gpu=[int(x) for x in orf] == round_faces, outputs_match_oracle=bool(same))
 
 
 def _cpu_plain(verts, f, scale, target, round_faces, threads, seconds):

@@ -232,6 +232,18 @@ int pamo_debug_walk_split(pamo_ctx* ctx, int kshift);
  * intersection.  The round kernels (pamo_free_forward) are unaffected. */
 int pamo_set_strict(pamo_ctx* ctx, int strict);
 
+/* Placement for the engine (pamo_run_stage2, pamo_batch_*,
+ * pamo_run_stage2_noalloc); no counterpart in the reference, which places
+ * and scores every collapse at the edge midpoint (simp_cuda/src/
+ * cusimp_free.cu:491,687-688).  0 (default): the midpoint.  1: the optimal
+ * vertex of Q_u + Q_v, for callers that want lower geometric error:
+ * the 3x3 system solved in FP32 when well conditioned, else in FP64, else
+ * (or when the solution lies farther from the midpoint than the edge is
+ * long) the one of midpoint, u and v with the lowest quadric error.  The
+ * cost is scored at that position and the collapse moves u there.  The
+ * round kernels (pamo_free_forward) are unaffected. */
+int pamo_set_placement(pamo_ctx* ctx, int mode);
+
 /* Parity hook: every ordered candidate pair (query, current) of one
  * self-intersection check over all alive faces of a mesh, with the
  * reference narrow-phase verdict (simp_cuda/src/bvh/self_intersect.cuh:151-392).

@@ -51,7 +51,13 @@ def _lib():
         _orc.orc_copy.argtypes = [P, ctypes.c_int, P, ctypes.c_longlong]
         _orc.orc_copy.restype = ctypes.c_longlong
         _orc.orc_run.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_longlong, ctypes.c_float,
-                                 ctypes.c_int, ctypes.c_longlong, ctypes.c_int, P, P, P, P, ctypes.c_int, ctypes.c_int]
+                                 ctypes.c_int, ctypes.c_longlong, ctypes.c_int, P, P, P, P, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_int]
+        _orc.orc_run_shadow.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_longlong,
+                                        ctypes.c_float, ctypes.c_int, ctypes.c_longlong, ctypes.c_int,
+                                        ctypes.c_double, P]
+        _orc.orc_place.argtypes = [P, P, P, P]
+        _orc.orc_place.restype = ctypes.c_int
         _orc.orc_shuffle_perm.argtypes = [ctypes.c_longlong, P]
         _orc.orc_narrow.argtypes = [P, P, P, P]
         _orc.orc_isect_raw.argtypes = [P, P, P, P]
@@ -147,10 +153,12 @@ class Oracle:
 
 
 def run(verts, faces, scale, target, threshold=1e-3, tolerance=4, iters=1000000, shuffle=SHUFFLE_FEISTEL,
-        return_rounds=False, strict=False):
+        return_rounds=False, strict=False, placement=0):
     """The stage-2 round loop of ``PaMO.run`` on centred vertices (C++).
     strict: the strict guard (not in the reference): exact pair test added
-    to the reference's, whole collapsed list scanned, undo until clean."""
+    to the reference's, whole collapsed list scanned, undo until clean.
+    placement: 0 the edge midpoint (the reference's), 1 the optimal vertex
+    (not in the reference; pamo_amd/csrc/placement.h)."""
     lib = _lib()
     verts = np.ascontiguousarray(verts, np.float32)
     faces = np.ascontiguousarray(faces, np.int32)
@@ -161,7 +169,8 @@ def run(verts, faces, scale, target, threshold=1e-3, tolerance=4, iters=1000000,
     cap = 100000
     rf = np.zeros(cap, np.int32)
     rc = lib.orc_run(_ptr(verts), _ptr(faces), V, F, float(scale), int(target), float(threshold), int(tolerance),
-                     int(iters), int(shuffle), _ptr(ov), _ptr(of), _ptr(st), _ptr(rf), cap, int(bool(strict)))
+                     int(iters), int(shuffle), _ptr(ov), _ptr(of), _ptr(st), _ptr(rf), cap, int(bool(strict)),
+                     int(placement))
     if rc != 0:
         raise RuntimeError("oracle run failed with code %d" % rc)
     stats = dict(rounds=int(st[0]), V=int(st[1]), F=int(st[2]), exit_reason=int(st[3]),
@@ -190,6 +199,39 @@ def run_timed(verts, faces, scale, target, max_seconds, threshold=1e-3, toleranc
         raise RuntimeError("oracle run failed with code %d" % rc)
     n = int(st[0])
     return dict(rounds=n, exit_reason=int(st[3])), rf[:n].copy(), rs[:n].copy()
+
+
+def fp64_shadow(verts, faces, scale, target, threshold=1e-3, tolerance=4, max_seconds=0.0,
+                shuffle=SHUFFLE_FEISTEL):
+    """The FP32 round loop with an FP64 shadow of every round: costs of all
+    edges from the FP64 twin on the round's start state, the winners those
+    costs would choose, and double positions that follow the FP32 run's
+    collapses (see orc_run_shadow).  max_seconds > 0 stops after the first
+    round that ends past it (exit_reason 5)."""
+    verts = np.ascontiguousarray(verts, np.float32)
+    faces = np.ascontiguousarray(faces, np.int32)
+    out = np.zeros(15, np.float64)
+    rc = _lib().orc_run_shadow(_ptr(verts), _ptr(faces), verts.shape[0], faces.shape[0], float(scale), int(target),
+                               float(threshold), int(tolerance), 1000000, int(shuffle), float(max_seconds),
+                               _ptr(out))
+    if rc != 0:
+        raise RuntimeError("oracle shadow run failed with code %d" % rc)
+    return dict(rounds=int(out[0]), edges_scored=int(out[1]), edges_valued=int(out[2]),
+                quantized_cost_differs=int(out[3]), winners_fp32=int(out[4]), winners_fp64=int(out[5]),
+                winners_differ=int(out[6]), cost_rel_err_median=float(out[7]), cost_rel_err_p99=float(out[8]),
+                cost_rel_err_max=float(out[9]), position_err_max=float(out[10]), position_err_mean=float(out[11]),
+                exit_reason=int(out[12]), final_faces=int(out[13]), seconds=round(float(out[14]), 2))
+
+
+def place(q, a, b):
+    """The optimal placement (pamo_amd/csrc/placement.h) of edge (a, b) under
+    the quadric q (10 unique entries): (position, branch taken)."""
+    q = np.ascontiguousarray(q, np.float32)
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    out = np.zeros(3, np.float32)
+    how = _lib().orc_place(_ptr(q), _ptr(a), _ptr(b), _ptr(out))
+    return out, int(how)
 
 
 def shuffle_perm(m):

@@ -95,6 +95,7 @@
 #include <vector>
 
 #include "../pamo_amd/csrc/exact_pred.h"
+#include "../pamo_amd/csrc/placement.h"
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -410,6 +411,19 @@ struct Ctx {
   // covers the whole collapsed list and the undo loop runs until nothing is
   // marked (at most STRICT_PASSES passes).
   bool strict = false;
+  // Placement (not in the reference): 0 the edge midpoint, as the reference
+  // (cusimp_free.cu:491,687-688); 1 the optimal vertex of Q_u + Q_v with the
+  // FP64 and midpoint/endpoint fallbacks of ../pamo_amd/csrc/placement.h,
+  // used by the cost and by the collapse alike.
+  int placement = 0;
+  V3 place_edge(int u, int v) const {
+    if (!placement) return vdivs(vadd(P[u], P[v]), 2.0f);
+    static const int idx[10] = {0, 1, 2, 3, 5, 6, 7, 10, 11, 15};
+    float q[10];
+    for (int i = 0; i < 10; i++) q[i] = Q[(size_t)16 * u + idx[i]] + Q[(size_t)16 * v + idx[i]];
+    pamo::place::P3 x = pamo::place::optimal(q, {P[u].x, P[u].y, P[u].z}, {P[v].x, P[v].y, P[v].z}, nullptr);
+    return {x.x, x.y, x.z};
+  }
   static const int STRICT_PASSES = 64;
 
   void ensure_pts(size_t n) {
@@ -558,7 +572,7 @@ struct Ctx {
     el = el / (float)ne / scale * tau;
     if (sp) el += el + vnorm(vsub(v0, v1)) / scale * tau;
     else el = el + vnorm(vsub(v0, v1)) / scale * tau;
-    V3 m = vdivs(vadd(v0, v1), 2.0f);
+    V3 m = sp ? vdivs(vadd(v0, v1), 2.0f) : place_edge(u, v);
     float Qa = 0; int nt = 0;
     if (!skinny_pass(u, v, m, Qa, nt, sp)) { edge_cost[e] = 0xFFFFFFFFu; return; }
     if (!skinny_pass(v, u, m, Qa, nt, sp)) { edge_cost[e] = 0xFFFFFFFFu; return; }
@@ -645,7 +659,7 @@ struct Ctx {
     for (int i = 0; i < n_winners; i++) {
       int e = collapsed[i];
       int u = edges[2 * e], v = edges[2 * e + 1];
-      V3 m = vdivs(vadd(P[u], P[v]), 2.0f);
+      V3 m = sp ? vdivs(vadd(P[u], P[v]), 2.0f) : place_edge(u, v);
       P[u] = m; P[v] = {0, 0, 0}; occ[v] = 0;
       surgery(u, v);
       if (sp) continue;   // the unguarded variant keeps every winner (cusimp.cu:511-584)
@@ -988,17 +1002,21 @@ template <class T> static inline T tclamp(T x, T a, T b) {
 
 template <class T> struct Twin {
   const Ctx& c;
+  const int* Tv;   // faces the twin reads: the current ones, or the round-start snapshot
   std::vector<W3<T>> P;
   std::vector<T> Q;
-  explicit Twin(const Ctx& ctx) : c(ctx) {
+  // snapshot: the round-start state of the last forward (P0, T0), on which
+  // that round's costs were computed
+  explicit Twin(const Ctx& ctx, bool snapshot = false) : c(ctx), Tv(snapshot ? ctx.T0.data() : ctx.T.data()) {
+    const std::vector<V3>& Ps = snapshot ? c.P0 : c.P;
     P.resize(c.V);
-    for (int v = 0; v < c.V; v++) P[v] = {(T)c.P[v].x, (T)c.P[v].y, (T)c.P[v].z};
+    for (int v = 0; v < c.V; v++) P[v] = {(T)Ps[v].x, (T)Ps[v].y, (T)Ps[v].z};
     Q.assign((size_t)16 * c.V, (T)0);
     for (int v = 0; v < c.V; v++) {
       T K[16] = {0};
       for (int s = c.fan_off[v]; s < c.fan_off[v + 1]; s++) {
         int f = c.fan[s];
-        W3<T> v0 = P[c.T[3 * f]], v1 = P[c.T[3 * f + 1]], v2 = P[c.T[3 * f + 2]];
+        W3<T> v0 = P[Tv[3 * f]], v1 = P[Tv[3 * f + 1]], v2 = P[Tv[3 * f + 2]];
         W3<T> n = wcross(wsub(v1, v0), wsub(v2, v0));
         n = wdivs(n, wnorm(n));
         T off = -wdot(n, v0);
@@ -1016,7 +1034,7 @@ template <class T> struct Twin {
   bool skinny(int moved, int other, W3<T> m, T& Qa, int& nt) const {
     for (int s = c.fan_off[moved]; s < c.fan_off[moved + 1]; s++) {
       int f = c.fan[s];
-      int i = c.T[3 * f], j = c.T[3 * f + 1], k = c.T[3 * f + 2];
+      int i = Tv[3 * f], j = Tv[3 * f + 1], k = Tv[3 * f + 2];
       if (i == other || j == other || k == other) continue;
       W3<T> o0 = P[i], o1 = P[j], o2 = P[k];
       W3<T> n0 = i == moved ? m : o0, n1 = j == moved ? m : o1, n2 = k == moved ? m : o2;
@@ -1038,9 +1056,9 @@ template <class T> struct Twin {
     int u = c.edges[2 * e], v = c.edges[2 * e + 1];
     int dup = 0;
     for (int a = c.fan_off[u]; a < c.fan_off[u + 1]; a++) {
-      int iu = Ctx::succ(&c.T[3 * c.fan[a]], u);
+      int iu = Ctx::succ(&Tv[3 * c.fan[a]], u);
       for (int b = c.fan_off[v]; b < c.fan_off[v + 1]; b++) {
-        if (iu == Ctx::succ(&c.T[3 * c.fan[b]], v)) dup++;
+        if (iu == Ctx::succ(&Tv[3 * c.fan[b]], v)) dup++;
         if (dup > 2) return 1;
       }
     }
@@ -1049,7 +1067,7 @@ template <class T> struct Twin {
     int ne = 0;
     for (int w : {u, v}) {
       for (int s = c.fan_off[w]; s < c.fan_off[w + 1]; s++) {
-        const int* t = &c.T[3 * c.fan[s]];
+        const int* t = &Tv[3 * c.fan[s]];
         if (t[0] > t[1]) { el += wnorm(wsub(P[t[0]], P[t[1]])); ne++; }
         if (t[1] > t[2]) { el += wnorm(wsub(P[t[1]], P[t[2]])); ne++; }
         if (t[2] > t[0]) { el += wnorm(wsub(P[t[2]], P[t[0]])); ne++; }
@@ -1073,6 +1091,140 @@ template <class T> struct Twin {
     return 0;
   }
 };
+
+// Whole-run FP64 shadow (measurement, not a restatement): the FP32 loop of
+// run_loop, and after every round's forward, on that round's start state:
+// each edge's cost from the FP64 twin (with the FP32 round's table and stuck
+// rules), quantized as the reference quantizes, and the claim and winner test
+// over those keys; and a double copy of every vertex position that follows
+// the FP32 round's collapses, restores and compaction, so the final output's
+// positions can be compared with exact averaging of the input.
+struct ShadowStats {
+  long long rounds = 0, edges = 0, valued = 0, qdiff = 0, w32 = 0, w64 = 0, wdiff = 0;
+  double rel_max = 0, pos_max = 0, pos_mean = 0;
+  std::vector<long long> hist = std::vector<long long>(2401, 0);   // log10(rel) in [-20, 4), 0.01 steps
+};
+
+static int run_shadow(Ctx& c, const float* verts, const int* faces, int V, int F, float scale, long long target,
+                      float threshold, int tolerance, long long iters, int shuffle_mode, double max_seconds,
+                      ShadowStats& ss, RunStats& st) {
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<float> cv(verts, verts + 3 * (size_t)V);
+  std::vector<int> cf(faces, faces + 3 * (size_t)F);
+  std::vector<double> pd(cv.begin(), cv.end());
+  std::vector<int> undo;
+  int count = 0, is_stuck = 0;
+  bool init = true;
+  memset(&st, 0, sizeof(st));
+  st.exit_reason = 4;
+  for (long long it = 0; it < iters; it++) {
+    int nprev = (int)(cf.size() / 3);
+    int rc = c.forward(cv.data(), cf.data(), undo.data(), (int)undo.size(), (int)(cv.size() / 3), nprev, scale,
+                       threshold, is_stuck != 0, init, shuffle_mode);
+    if (rc) return rc;
+    init = false;
+    st.rounds++;
+    // (1) the round's costs and winners in FP64
+    {
+      Twin<double> td(c, true);
+      const int E = c.E;
+      std::vector<uint32_t> q64(E);
+      std::vector<double> rel(E, -1.0);
+#pragma omp parallel for schedule(dynamic, 256)
+      for (int e = 0; e < E; e++) {
+        const int u = c.edges[2 * e], v = c.edges[2 * e + 1];
+        const uint32_t q32 = c.edge_cost[e];
+        if (c.table[u] && c.table[v]) { q64[e] = 0xFFFFFFFFu; continue; }
+        if (is_stuck) { q64[e] = q32; continue; }
+        double val = 0;
+        W3<double> m{0, 0, 0};
+        const int r = td.cost(e, val, m);
+        if (r == 1) { q64[e] = 0xFFFFFFFFu; continue; }
+        if (r == 2) { q64[e] = q32; continue; }
+        double x = std::floor(tclamp(val, 0.0, 10.0) / 10.0 * 4294967296.0);
+        q64[e] = x >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)x;
+        const double d32 = (double)q32 / 4294967296.0 * 10.0;   // the FP32 cost, dequantized
+        rel[e] = std::fabs(d32 - val) / std::max(std::fabs(val), 1e-300);
+      }
+      for (int e = 0; e < E; e++) {
+        ss.edges++;
+        if (q64[e] != c.edge_cost[e]) ss.qdiff++;
+        if (rel[e] < 0) continue;
+        ss.valued++;
+        ss.rel_max = std::max(ss.rel_max, rel[e]);
+        const double lg = rel[e] > 0 ? std::log10(rel[e]) : -20.0;
+        const int b = (int)std::min(2400.0, std::max(0.0, (lg + 20.0) * 100.0));
+        ss.hist[b]++;
+      }
+      // claim and winner test over the FP64 keys (cusimp_free.cu:587-681)
+      std::vector<u64> vmin(c.V, ~0ull);
+      for (int e = 0; e < E; e++) {
+        const u64 key = ((u64)q64[e] << 32) | (u64)(uint32_t)e;
+        for (int w : {c.edges[2 * e], c.edges[2 * e + 1]}) vmin[w] = std::min(vmin[w], key);
+      }
+      std::vector<uint8_t> w64(E, 0);
+#pragma omp parallel for schedule(static)
+      for (int e = 0; e < E; e++) {
+        if (q64[e] > c.collapse_t) continue;
+        const u64 key = ((u64)q64[e] << 32) | (u64)(uint32_t)e;
+        bool ok = true;
+        for (int w : {c.edges[2 * e], c.edges[2 * e + 1]})
+          for (int s2 = c.fan_off[w]; s2 < c.fan_off[w + 1] && ok; s2++) {
+            const int* t = &c.T0[3 * c.fan[s2]];
+            ok = std::min(vmin[t[0]], std::min(vmin[t[1]], vmin[t[2]])) == key;
+          }
+        w64[e] = ok ? 1 : 0;
+      }
+      std::vector<uint8_t> w32(E, 0);
+      for (int i = 0; i < c.n_winners; i++) w32[c.collapsed[i]] = 1;
+      for (int e = 0; e < E; e++) {
+        ss.w32 += w32[e];
+        ss.w64 += w64[e];
+        ss.wdiff += w32[e] != w64[e];
+      }
+    }
+    // (2) the double positions follow the round's kept winner collapses
+    // (line-edge collapses keep u, whose position equals v's)
+    for (int i = 0; i < c.n_winners; i++) {
+      const int e = c.collapsed[i];
+      const int u = c.edges[2 * e], v = c.edges[2 * e + 1];
+      if (c.occ[v]) continue;   // restored
+      for (int k = 0; k < 3; k++) pd[3 * (size_t)u + k] = (pd[3 * (size_t)u + k] + pd[3 * (size_t)v + k]) / 2.0;
+    }
+    undo.assign(c.undo_list.begin(), c.undo_list.begin() + c.n_vertices_undo);
+    std::vector<float> nv;
+    std::vector<double> npd;
+    for (int v = 0; v < c.V; v++)
+      if (c.occ[v]) {
+        nv.push_back(c.P[v].x); nv.push_back(c.P[v].y); nv.push_back(c.P[v].z);
+        for (int k = 0; k < 3; k++) npd.push_back(pd[3 * (size_t)v + k]);
+      }
+    std::vector<int> nf;
+    for (int f = 0; f < c.F; f++)
+      if (c.T[3 * f] >= 0)
+        for (int k = 0; k < 3; k++) nf.push_back(c.pts_map[c.T[3 * f + k]]);
+    cv.swap(nv); cf.swap(nf); pd.swap(npd);
+    int ncur = (int)(cf.size() / 3);
+    double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (max_seconds > 0 && el > max_seconds) { st.exit_reason = 5; break; }
+    if (target >= 0 && (ncur <= target || ncur <= 10)) { st.exit_reason = ncur <= target ? 1 : 2; break; }
+    if (ncur == nprev) count++;
+    else { count = 0; is_stuck = 0; }
+    if (count >= 2) is_stuck = 1;
+    if (count == tolerance) { st.exit_reason = 3; break; }
+  }
+  st.final_V = (int)(cv.size() / 3);
+  st.final_F = (int)(cf.size() / 3);
+  ss.rounds = st.rounds;
+  double sum = 0;
+  for (size_t i = 0; i < cv.size(); i++) {
+    const double d = std::fabs((double)cv[i] - pd[i]) / (double)scale;
+    ss.pos_max = std::max(ss.pos_max, d);
+    sum += d;
+  }
+  ss.pos_mean = cv.empty() ? 0.0 : sum / (double)cv.size();
+  return 0;
+}
 
 }  // namespace orc
 
@@ -1145,9 +1297,10 @@ long long orc_copy(void* ctx, int which, void* dst, long long cap_bytes) {
 
 int orc_run(const float* verts, const int* faces, int V, int F, float scale, long long target, float threshold,
             int tolerance, long long iters, int shuffle_mode, float* outV, int* outF, long long stats[8],
-            int* round_faces, int round_cap, int strict) {
+            int* round_faces, int round_cap, int strict, int placement) {
   orc::Ctx c;
   c.strict = strict != 0;
+  c.placement = placement;
   std::vector<float> ov; std::vector<int> of; orc::RunStats st;
   std::vector<int> rf;
   int rc = orc::run_loop(c, verts, faces, V, F, scale, target, threshold, tolerance, iters, shuffle_mode, ov, of, st, &rf);
@@ -1185,6 +1338,46 @@ void orc_shuffle_perm(long long m, long long* out) {
 }
 
 // narrow phase on one ordered pair (current, query); returns the outcome code
+// The whole-run FP64 shadow: out = [0] rounds [1] edges scored [2] edges
+// valued in both [3] quantized costs that differ [4] FP32 winners [5] FP64
+// winners [6] winners in one set only [7] cost relative error median [8] p99
+// [9] max [10] largest position difference / scale [11] mean [12] exit
+// reason (5: time budget) [13] final faces [14] seconds
+int orc_run_shadow(const float* verts, const int* faces, int V, int F, float scale, long long target,
+                   float threshold, int tolerance, long long iters, int shuffle_mode, double max_seconds,
+                   double out[15]) {
+  orc::Ctx c;
+  orc::ShadowStats ss;
+  orc::RunStats st;
+  auto t0 = std::chrono::steady_clock::now();
+  int rc = orc::run_shadow(c, verts, faces, V, F, scale, target, threshold, tolerance, iters, shuffle_mode,
+                           max_seconds, ss, st);
+  if (rc) return rc;
+  auto pct = [&](double q) {
+    long long need = (long long)std::ceil(q * (double)ss.valued), acc = 0;
+    for (int b = 0; b < (int)ss.hist.size(); b++) {
+      acc += ss.hist[b];
+      if (acc >= need && need > 0) return std::pow(10.0, b / 100.0 - 20.0);
+    }
+    return 0.0;
+  };
+  const double v[15] = {(double)ss.rounds, (double)ss.edges, (double)ss.valued, (double)ss.qdiff, (double)ss.w32,
+                        (double)ss.w64, (double)ss.wdiff, pct(0.5), pct(0.99), ss.rel_max, ss.pos_max, ss.pos_mean,
+                        (double)st.exit_reason, (double)st.final_F,
+                        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count()};
+  memcpy(out, v, sizeof(v));
+  return 0;
+}
+
+// the placement of ../pamo_amd/csrc/placement.h on a given quadric (10
+// unique entries) and edge; returns the branch taken
+int orc_place(const float q[10], const float a[3], const float b[3], float out[3]) {
+  int how = -1;
+  pamo::place::P3 x = pamo::place::optimal(q, {a[0], a[1], a[2]}, {b[0], b[1], b[2]}, &how);
+  out[0] = x.x; out[1] = x.y; out[2] = x.z;
+  return how;
+}
+
 int orc_narrow(const float cur[9], const float qry[9], const int ci[3], const int qi[3]) {
   return orc::narrow_pair(cur, qry, ci, qi);
 }

@@ -69,9 +69,11 @@ def _scale(verts):
 
 
 def simplify_stage2(verts, faces, scale, target, threshold=1e-3, tolerance=4, iters=1000000, ctx=None,
-                    shuffle=_lib.SHUFFLE_FEISTEL, strict=None):
+                    shuffle=_lib.SHUFFLE_FEISTEL, strict=None, placement=None):
     """Device-resident stage-2 loop on centred vertices (torch CUDA tensors).
     strict: True/False sets the context's strict guard (None keeps it).
+    placement: "midpoint" (the reference's) or "optimal" (QEM optimal vertex,
+    not in the reference) sets the context's placement (None keeps it).
 
     Returns (verts, faces, stats) as torch CUDA tensors and a dict."""
     import ctypes
@@ -84,6 +86,8 @@ def simplify_stage2(verts, faces, scale, target, threshold=1e-3, tolerance=4, it
         raise RuntimeError("verts must be float32 and faces int32")
     if strict is not None:
         ctx.set_strict(strict)
+    if placement is not None:
+        ctx.set_placement(placement)
     L = ctx._L
     stream = torch.cuda.current_stream().cuda_stream
     oV, oF = ctypes.c_int(), ctypes.c_int()
@@ -112,7 +116,7 @@ class PaMO(torch.nn.Module):
     results."""
 
     def __init__(self, input_mesh, use_stage1=True, use_stage3=True, verbose=True, engine="device",
-                 shuffle=_lib.SHUFFLE_FEISTEL, strict=False):
+                 shuffle=_lib.SHUFFLE_FEISTEL, strict=False, placement="midpoint"):
         super().__init__()
         if use_stage1:
             raise NotImplementedError("stage 1 (SDF remeshing) is not part of pamo_amd; pass use_stage1=False")
@@ -131,6 +135,9 @@ class PaMO(torch.nn.Module):
         if strict and engine != "device":
             raise ValueError("strict=True needs engine='device' (the round kernels follow the reference)")
         self.pamo._ctx.set_strict(strict)
+        if placement != "midpoint" and engine != "device":
+            raise ValueError("placement=%r needs engine='device' (the round kernels follow the reference)" % placement)
+        self.pamo._ctx.set_placement(placement)
         self.target_faces = None
         self.last_stats = None
         self.last_stage2_seconds = None

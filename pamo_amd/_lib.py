@@ -75,6 +75,8 @@ def lib():
     L.pamo_debug_walk_split.argtypes = [P, I]
     L.pamo_debug_walk_split.restype = I
     L.pamo_set_strict.argtypes = [P, I]
+    L.pamo_set_placement.argtypes = [P, I]
+    L.pamo_set_placement.restype = I
     L.pamo_debug_guard.restype = I
     L.pamo_simplify.argtypes = [P, P, P, P, P, F]
     L.pamo_simplify.restype = None
@@ -139,6 +141,15 @@ class Context:
         """Strict guard for the engine (off by default; not in the reference):
         exact pair test added, every collapse scanned, undo until clean."""
         self.check(self._L.pamo_set_strict(self.handle, 1 if strict else 0), "pamo_set_strict")
+
+    PLACEMENTS = {"midpoint": 0, "optimal": 1}
+
+    def set_placement(self, placement):
+        """Where the engine places a collapse: "midpoint" (the reference's,
+        the default) or "optimal" (the optimal vertex of Q_u + Q_v with FP64
+        and midpoint/endpoint fallbacks; not in the reference)."""
+        mode = self.PLACEMENTS[placement] if isinstance(placement, str) else int(placement)
+        self.check(self._L.pamo_set_placement(self.handle, mode), "pamo_set_placement")
 
     def check(self, rc, what):
         if rc < 0:

@@ -28,12 +28,14 @@ BYTES_PER_FACE = 420
 
 
 class BatchEngine:
-    def __init__(self, device=None, shuffle=_lib.SHUFFLE_FEISTEL, ctx=None, strict=None):
+    def __init__(self, device=None, shuffle=_lib.SHUFFLE_FEISTEL, ctx=None, strict=None, placement=None):
         if ctx is None:
             ctx = _lib.Context(torch.cuda.current_device() if device is None else device, shuffle)
         self.ctx = ctx
         if strict is not None:
             ctx.set_strict(strict)
+        if placement is not None:
+            ctx.set_placement(placement)
         L = ctx._L
         P, I, LL, Fl = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_float
         L.pamo_batch_submit.argtypes = [P, P, P, P, I, I, Fl, LL, Fl, I, LL, LL]

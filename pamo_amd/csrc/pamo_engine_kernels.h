@@ -16,6 +16,7 @@
 #pragma once
 #include "pamo_bvh.h"
 #include "pamo_kernels.h"
+#include "placement.h"
 
 namespace pamo {
 namespace eng {
@@ -459,6 +460,20 @@ __global__ void k_edge_vorder(const int* __restrict__ edges, const int* __restri
   vedge[uoff[u] + erank[e]] = make_int4(e, u, v, 0);
 }
 
+// The position an edge collapses to: the midpoint (the reference,
+// cusimp_free.cu:491,687-688), or with placement 1 the optimal vertex of
+// Q_u + Q_v (placement.h); the cost and the collapse call the same function.
+__device__ __forceinline__ V3 edge_position(const float* __restrict__ Q, int u, int v, V3 pu, V3 pv, int placement) {
+  if (!placement) return vdivs(vadd(pu, pv), 2.0f);
+  const float* qu = Q + (long long)10 * u;
+  const float* qv = Q + (long long)10 * v;
+  float q[10];
+#pragma unroll
+  for (int j = 0; j < 10; j++) q[j] = qu[j] + qv[j];
+  const place::P3 x = place::optimal(q, place::P3{pu.x, pu.y, pu.z}, place::P3{pv.x, pv.y, pv.z}, nullptr);
+  return V3{x.x, x.y, x.z};
+}
+
 // ----------------------------------------------------------- edge cost
 // cusimp_free.cu:382-585 with per-mesh parameters.  A group of EC_G lanes
 // takes one edge (edges in vertex order); lane g loads slots g, g+EC_G, ...
@@ -541,7 +556,7 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__
                                                        const int* __restrict__ vmesh,
                                                        const long long* __restrict__ voff,
                                                        const long long* __restrict__ eoffm,
-                                                       const MeshParam* __restrict__ mp, u64* vmin) {
+                                                       const MeshParam* __restrict__ mp, u64* vmin, int placement) {
   // every lane of a group has the same i, so each return below is taken by
   // the whole group and the width-EC_G shuffles always see all its lanes
   const int i = (blockIdx.x * blockDim.x + threadIdx.x) / EC_G;
@@ -586,7 +601,7 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__
   }
   const float scale = mp[k].scale, tau = mp[k].tau;
   V3 v0 = ldp(P, u), v1 = ldp(P, v);
-  V3 m = vdivs(vadd(v0, v1), 2.0f);
+  V3 m = edge_position(Q, u, v, v0, v1, placement);
   float el = 0.0f, Qa = 0.0f;
   int ne = 0, nt = 0, flip = 0;
   ec_fan(P, sface, splane, slen, ua, ub, g, u, v, m, el, Qa, ne, nt, flip);
@@ -685,11 +700,12 @@ __device__ void collapse_one(int i, int g, const int* __restrict__ wlist, const 
                              const int* __restrict__ edges, const int* __restrict__ vmesh,
                              const long long* __restrict__ eoffm, int* __restrict__ coll,
                              const int* __restrict__ fan_off, const int4* __restrict__ sface, float* P,
-                             const float* __restrict__ P0, int* T, const int* __restrict__ T0, int* occ) {
+                             const float* __restrict__ P0, int* T, const int* __restrict__ T0, int* occ,
+                             const float* __restrict__ Q, int placement) {
   const int e = wlist[i];
   const int u = edges[2 * e], v = edges[2 * e + 1];
   const V3 pu = ldp(P, u), pv = ldp(P, v);
-  const V3 m = vdivs(vadd(pu, pv), 2.0f);
+  const V3 m = edge_position(Q, u, v, pu, pv, placement);
   const int ua = fan_off[u], ub = fan_off[u + 1], va = fan_off[v], vb = fan_off[v + 1];
   if (g == 0) {
     int k = vmesh[u];
@@ -744,14 +760,15 @@ __global__ void __launch_bounds__(BLOCK) k_collapse_b(const int* __restrict__ wl
                                                       const int* __restrict__ fan_off,
                                                       const int4* __restrict__ sface, float* P,
                                                       const float* __restrict__ P0, int* T,
-                                                      const int* __restrict__ T0, int* occ) {
+                                                      const int* __restrict__ T0, int* occ,
+                                                      const float* __restrict__ Q, int placement) {
   // grid-stride over winners; every lane of a group has the same i, so the
   // loop exits, returns and shuffles are group-wide
   const int g = threadIdx.x & (CL_G - 1);
   const int W = *W_ptr;
   const int stride = gridDim.x * (BLOCK / CL_G);
   for (int i = (blockIdx.x * blockDim.x + threadIdx.x) / CL_G; i < W; i += stride)
-    collapse_one(i, g, wlist, winpos, edges, vmesh, eoffm, coll, fan_off, sface, P, P0, T, T0, occ);
+    collapse_one(i, g, wlist, winpos, edges, vmesh, eoffm, coll, fan_off, sface, P, P0, T, T0, occ, Q, placement);
 }
 
 // per-mesh edge offsets (faces emit their edges in order)

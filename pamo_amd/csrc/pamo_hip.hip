@@ -82,6 +82,7 @@ struct pamo_ctx {
   int guard_mode = 0;             // test hook: 0 by mesh size, 1 tree only, 2 exhaustive only
   int walk_split = -1;            // test hook: lanes per query of the LBVH walk, log2 (-1: by size)
   int strict = 0;                 // strict guard (engine; not in the reference)
+  int placement = 0;              // 0 edge midpoint (the reference's), 1 optimal vertex (engine; not in the reference)
 
   int* hc = nullptr;   // pinned host mirror of the counters
 
@@ -852,6 +853,12 @@ int pamo_debug_guard(pamo_ctx* ctx, int mode) {
   return 0;
 }
 
+int pamo_set_placement(pamo_ctx* ctx, int mode) {
+  if (!ctx || mode < 0 || mode > 1) return -1;
+  ctx->placement = mode;
+  return 0;
+}
+
 int pamo_debug_walk_split(pamo_ctx* ctx, int kshift) {
   if (!ctx || kshift < -1 || kshift > 3) return -1;
   ctx->walk_split = kshift;
@@ -1128,6 +1135,7 @@ int pamo_run_stage2_noalloc(pamo_ctx* ctx, void* stream, const float* d_verts, c
   c.guard_mode = ctx->guard_mode;
   c.pair_cap_limit = ctx->pair_cap_limit;
   c.walk_split = ctx->walk_split;
+  c.placement = ctx->placement;
   c.arena = nd.region;
   c.arena_cap = PAMO_NOALLOC_SCRATCH_BYTES;
   c.hc = c.hc_host;

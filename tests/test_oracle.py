@@ -680,3 +680,117 @@ def test_unguarded_quotient_in_double():
         alt = ce.edge_cost(P, F, 0, 1, 1.0, 1.0, unguarded=True, d10=False)
     assert want != alt
     assert got == want
+
+
+# ------------------------------------------- optimal placement (opt-in)
+def _quadric(planes):
+    """10 unique entries of sum p p^T over planes p = (n, d) (n unit)."""
+    Q = np.zeros((4, 4))
+    for p in planes:
+        p = np.asarray(p, np.float64)
+        Q += np.outer(p, p)
+    return np.array([Q[0, 0], Q[0, 1], Q[0, 2], Q[0, 3], Q[1, 1], Q[1, 2], Q[1, 3], Q[2, 2], Q[2, 3], Q[3, 3]],
+                    np.float32)
+
+
+def test_placement_closed_forms():
+    """Each branch of the placement on a quadric with a known minimiser.
+    Three orthogonal planes through p0: A = I, so the FP32 solve returns p0
+    exactly.  The same with the third plane weighted 2^-7 (entries exact in
+    float; det / tr^3 = 7.6e-6, below the FP32 bound): the FP64 solve,
+    again p0.  A flat patch
+    (every plane z = 0): A is singular, the rule takes the midpoint (all
+    three errors are 0).  A crease (planes x = 0 and y = 0): singular again,
+    the rule takes the endpoint on the crease.  An optimum farther from the
+    midpoint than the edge is long: rejected, the rule."""
+    p0 = np.array([0.25, -0.5, 0.75], np.float32)
+    a, b = p0 + np.float32(0.125), p0 - np.float32(0.0625)
+    q = _quadric([(1, 0, 0, -0.25), (0, 1, 0, 0.5), (0, 0, 1, -0.75)])
+    x, how = oracle.place(q, a, b)
+    assert how == 1 and x.tobytes() == p0.tobytes()
+    q = _quadric([(1, 0, 0, -0.25), (0, 1, 0, 0.5), (0, 0, 2.0 ** -7, -0.75 * 2.0 ** -7)])
+    x, how = oracle.place(q, a, b)
+    assert how == 2 and x.tobytes() == p0.tobytes()
+    q = _quadric([(0, 0, 1, 0)] * 6)
+    a2, b2 = np.array([0, 0, 0], np.float32), np.array([1, 0.5, 0], np.float32)
+    x, how = oracle.place(q, a2, b2)
+    assert how == 3 and x.tobytes() == ((a2 + b2) / np.float32(2)).tobytes()
+    q = _quadric([(1, 0, 0, 0), (0, 1, 0, 0)])
+    on, off = np.array([0, 0, 0.3], np.float32), np.array([0.2, 0.1, 0.3], np.float32)
+    x, how = oracle.place(q, off, on)
+    assert how == 5 and x.tobytes() == on.tobytes()
+    x, how = oracle.place(q, on, off)
+    assert how == 4 and x.tobytes() == on.tobytes()
+    far = np.array([10, 10, 10], np.float32)
+    q = _quadric([(1, 0, 0, -10), (0, 1, 0, -10), (0, 0, 1, -10)])
+    x, how = oracle.place(q, np.zeros(3, np.float32), np.array([0.1, 0, 0], np.float32))
+    assert how in (3, 4, 5) and np.linalg.norm(x - far) > 1
+
+
+def _surface_error(v0, f0, v1, f1):
+    """Mean and max distance from the input vertices to the output surface
+    (brute force over the output faces; small meshes)."""
+    tri = v1[f1].astype(np.float64)
+    p = v0.astype(np.float64)
+    best = np.full(len(p), np.inf)
+    for t in tri:
+        a, b, c = t
+        ab, ac, ap = b - a, c - a, p - a
+        d1, d2 = ap @ ab, ap @ ac
+        # closest point by barycentric projection clamped to the triangle (Ericson 2005, 5.1.5)
+        n = np.cross(ab, ac)
+        nn = n @ n
+        if nn == 0:
+            continue
+        w = (p - a) - np.outer((p - a) @ n / nn, n)
+        M = np.array([[ab @ ab, ab @ ac], [ab @ ac, ac @ ac]])
+        st = np.linalg.solve(M, np.stack([w @ ab, w @ ac]))
+        s, t2 = st
+        inside = (s >= 0) & (t2 >= 0) & (s + t2 <= 1)
+        dist = np.where(inside, np.abs((p - a) @ n) / np.sqrt(nn), np.inf)
+        for u, v in ((a, b), (b, c), (c, a)):
+            e = v - u
+            h = np.clip(((p - u) @ e) / (e @ e), 0, 1)
+            dist = np.minimum(dist, np.linalg.norm(p - (u + h[:, None] * e), axis=1))
+        best = np.minimum(best, dist)
+    return best.mean(), best.max()
+
+
+def test_optimal_placement_runs_stay_valid_and_lower_error():
+    """Whole runs with the optimal placement: outputs stay closed, manifold
+    and free of intersections, and sit closer to the input surface than the
+    midpoint runs at the same target."""
+    for v, f in (synth.displaced_icosphere(4, 5), synth.sdf_union(10, 40, 7)):
+        verts, scale, _ = oracle.preprocess(v, f)
+        target = int(0.02 * f.shape[0])
+        res = {}
+        for pl in (0, 1):
+            ov, of, st = oracle.run(verts, f, scale, target, placement=pl)
+            r = pv.verify(ov, of)
+            assert r["manifold"] and r["watertight"] and r["intersecting_pairs"] == 0, (pl, r)
+            res[pl] = (_surface_error(verts, f, ov, of), of.shape[0])
+        (m0, _), n0 = res[0]
+        (m1, _), n1 = res[1]
+        print("midpoint", res[0], "optimal", res[1])
+        assert m1 < m0
+
+
+def test_fp64_shadow_follows_the_fp32_run():
+    """The whole-run FP64 shadow runs the same FP32 loop as oracle.run
+    (rounds, final size), scores every edge of every round, and its double
+    positions follow the FP32 collapses: on a mesh with dyadic coordinates
+    every midpoint is exact in float, so the position difference is exactly
+    0; on a generic mesh it stays at float rounding."""
+    v, f = synth.displaced_icosphere(4, 5)
+    verts, scale, _ = oracle.preprocess(v, f)
+    target = int(0.01 * f.shape[0])
+    _, of, st = oracle.run(verts, f, scale, target)
+    r = oracle.fp64_shadow(verts, f, scale, target)
+    assert r["rounds"] == st["rounds"] and r["final_faces"] == of.shape[0]
+    assert r["winners_fp32"] == st["total_collapsed"]
+    assert r["edges_scored"] > 0 and 0 < r["edges_valued"] <= r["edges_scored"]
+    assert 0 < r["position_err_max"] < 1e-6
+    P, F = ce.flat_patch(p=(-0.5, 0.125))
+    assert np.all(P * 1024 == np.round(P * 1024))   # dyadic coordinates
+    r = oracle.fp64_shadow(P, F, 1.0, 4, threshold=1e-3)
+    assert r["winners_fp32"] > 0 and r["position_err_max"] == 0.0
