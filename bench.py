@@ -209,6 +209,9 @@ def run_plain(args):
                                        target, 100 * ratio, 4, EXIT_NAMES.get(st["exit_reason"], "?"),
                                        int(out_f.shape[0]), 100.0 * out_f.shape[0] / F0),
                               parallelism="single GPU", strict_guard=args.strict, placement=args.placement,
+                              tuning=dict(ctx.tuning(), source=("PAMO_TUNING_FILE" if os.environ.get("PAMO_TUNING_FILE")
+                                                               else "pamo_amd/tuning.json" if _lib.tuning_cache()
+                                                               else "defaults")),
                               guard_checks=st["checks"], query_faces=st["query_faces"],
                               candidate_pairs=st["candidate_pairs"]),
                   latency_ms=dict(p50=round(1e3 * float(np.percentile(times, 50)), 3),

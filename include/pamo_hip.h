@@ -232,6 +232,20 @@ int pamo_debug_walk_split(pamo_ctx* ctx, int kshift);
  * intersection.  The round kernels (pamo_free_forward) are unaffected. */
 int pamo_set_strict(pamo_ctx* ctx, int strict);
 
+/* Launch shapes of the engine, by name (no counterpart in the reference,
+ * whose BLOCK_SIZE 512 and 10-bit Morton resolution are compiled in,
+ * simp_cuda/src/cusimp_free.cu:14, bvh/morton_code.cuh:20).  None changes a
+ * result.  Names: "ec_lanes" (lanes per edge of the cost kernel: 2, 4, 8),
+ * "morton_bits" (per axis, 6..10), "bf_max_faces" (meshes up to this many
+ * faces use the exhaustive search, larger ones the LBVH),
+ * "bf_split_max_faces", "walk8_max", "walk4_max", "emit_groups".  A context
+ * starts from the defaults, then from the flat JSON file named by the
+ * environment variable PAMO_TUNING_FILE, if set (python -m pamo_amd.tune
+ * writes one).  Returns 0, or -1 for an unknown name or a value out of
+ * range. */
+int pamo_set_tuning(pamo_ctx* ctx, const char* name, long long value);
+int pamo_get_tuning(pamo_ctx* ctx, const char* name, long long* value);
+
 /* Placement for the engine (pamo_run_stage2, pamo_batch_*,
  * pamo_run_stage2_noalloc); no counterpart in the reference, which places
  * and scores every collapse at the edge midpoint (simp_cuda/src/

@@ -75,6 +75,10 @@ def lib():
     L.pamo_debug_walk_split.argtypes = [P, I]
     L.pamo_debug_walk_split.restype = I
     L.pamo_set_strict.argtypes = [P, I]
+    L.pamo_set_tuning.argtypes = [P, ctypes.c_char_p, LL]
+    L.pamo_set_tuning.restype = I
+    L.pamo_get_tuning.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(LL)]
+    L.pamo_get_tuning.restype = I
     L.pamo_set_placement.argtypes = [P, I]
     L.pamo_set_placement.restype = I
     L.pamo_debug_guard.restype = I
@@ -86,10 +90,29 @@ def lib():
     return L
 
 
+# launch shapes of the engine (pamo_set_tuning); python -m pamo_amd.tune
+# writes the measured choice to TUNING_FILE, which every Context applies
+TUNING_NAMES = ("ec_lanes", "morton_bits", "bf_max_faces", "bf_split_max_faces", "walk8_max", "walk4_max",
+                "emit_groups")
+TUNING_FILE = os.path.join(_HERE, "tuning.json")
+
+
+def tuning_cache():
+    """The tuner's cache (name -> value), or {} when there is none.  The
+    environment variable PAMO_TUNING_FILE, read by the library itself, takes
+    precedence over this file."""
+    import json
+    if os.environ.get("PAMO_TUNING_FILE") or not os.path.exists(TUNING_FILE):
+        return {}
+    with open(TUNING_FILE) as fp:
+        d = json.load(fp)
+    return {k: int(v) for k, v in d.items() if k in TUNING_NAMES}
+
+
 class Context:
     """Owns one ``pamo_ctx`` (one simplifier's device state)."""
 
-    def __init__(self, device=0, shuffle=SHUFFLE_FEISTEL):
+    def __init__(self, device=0, shuffle=SHUFFLE_FEISTEL, tuning=True):
         self._L = lib()
         h = ctypes.c_void_p()
         rc = self._L.pamo_ctx_create(ctypes.byref(h), int(device), int(shuffle))
@@ -97,6 +120,22 @@ class Context:
             raise PamoError("pamo_ctx_create failed (%d): no usable GPU %d" % (rc, device))
         self.handle = h
         self.device = device
+        if tuning:
+            for k, v in tuning_cache().items():
+                self.set_tuning(k, v)
+
+    def set_tuning(self, name, value):
+        """One launch-shape parameter of the engine (results do not change)."""
+        self.check(self._L.pamo_set_tuning(self.handle, name.encode(), int(value)), "pamo_set_tuning(%s)" % name)
+
+    def tuning(self):
+        """Every launch-shape parameter in effect."""
+        out = {}
+        for k in TUNING_NAMES:
+            v = ctypes.c_longlong()
+            self.check(self._L.pamo_get_tuning(self.handle, k.encode(), ctypes.byref(v)), "pamo_get_tuning")
+            out[k] = v.value
+        return out
 
     def close(self):
         if getattr(self, "handle", None):

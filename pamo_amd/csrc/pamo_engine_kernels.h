@@ -486,20 +486,24 @@ __device__ __forceinline__ V3 edge_position(const float* __restrict__ Q, int u, 
 // +0 or more (a norm, or 1 - clamp(x, 0, 1)) or NaN.  The link count and the
 // flip test are an integer sum and an OR, exact in any order; a flip
 // anywhere makes the cost the maximum, as the reference's early return.
+// EC_G is the default group size (4 lanes: 11.7 ms per 2.0M-face run,
+// against 13.1 ms with 8, 14.6 ms with 2 and 20.3 ms with 16); the kernel is
+// instantiated for 2, 4 and 8 and the autotuner may pick another.
 constexpr int EC_G = 4;
-
+template <int G>
 __device__ __forceinline__ int ec_group_sum(int x) {
 #pragma unroll
-  for (int o = 1; o < EC_G; o <<= 1) x += __shfl_xor(x, o, EC_G);
+  for (int o = 1; o < G; o <<= 1) x += __shfl_xor(x, o, G);
   return x;
 }
 
 // one fan's slots for the edge: serial length and skinny sums, counts, flip
+template <int G>
 __device__ __forceinline__ void ec_fan(const float* __restrict__ P, const int4* __restrict__ sface,
                                        const float4* __restrict__ splane, const float4* __restrict__ slen, int a,
                                        int b, int g, int moved, int other, V3 m, float& el, float& Qa, int& ne,
                                        int& nt, int& flip) {
-  for (int base = a; base < b; base += EC_G) {
+  for (int base = a; base < b; base += G) {
     int s = base + g;
     float tx = 0.0f, ty = 0.0f, tz = 0.0f, tq = 0.0f;
     if (s < b) {
@@ -526,12 +530,12 @@ __device__ __forceinline__ void ec_fan(const float* __restrict__ P, const int4* 
         nt++;
       }
     }
-    const int n = b - base < EC_G ? b - base : EC_G;
+    const int n = b - base < G ? b - base : G;
     for (int j = 0; j < n; j++) {
-      el += __shfl(tx, j, EC_G);
-      el += __shfl(ty, j, EC_G);
-      el += __shfl(tz, j, EC_G);
-      Qa += __shfl(tq, j, EC_G);
+      el += __shfl(tx, j, G);
+      el += __shfl(ty, j, G);
+      el += __shfl(tz, j, G);
+      Qa += __shfl(tq, j, G);
     }
   }
 }
@@ -547,6 +551,7 @@ __device__ __forceinline__ void ec_finish(uint32_t* c, u64* vmin, int e, int u, 
   atomicMin(&vmin[v], key);
 }
 
+template <int G>
 __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__ P, const int4* __restrict__ vedge,
                                                        const int* __restrict__ E_ptr, const int* __restrict__ fan_off,
                                                        const int* __restrict__ fsucc, const int4* __restrict__ sface,
@@ -558,9 +563,9 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__
                                                        const long long* __restrict__ eoffm,
                                                        const MeshParam* __restrict__ mp, u64* vmin, int placement) {
   // every lane of a group has the same i, so each return below is taken by
-  // the whole group and the width-EC_G shuffles always see all its lanes
-  const int i = (blockIdx.x * blockDim.x + threadIdx.x) / EC_G;
-  const int g = threadIdx.x & (EC_G - 1);
+  // the whole group and the width-G shuffles always see all its lanes
+  const int i = (blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const int g = threadIdx.x & (G - 1);
   if (i >= *E_ptr) return;
   int4 ev = vedge[i];
   const int e = ev.x, u = ev.y, v = ev.z;
@@ -579,18 +584,18 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__
   }
   int ua = fan_off[u], ub = fan_off[u + 1], va = fan_off[v], vb = fan_off[v + 1];
   // link condition: equal successor pairs; more than 2 -> maximum, not 2 ->
-  // unchanged.  The group loads EC_G slots of each fan at once and compares
+  // unchanged.  The group loads G slots of each fan at once and compares
   // every pair through shuffles (successors are vertex indices, >= 0).
   int dup = 0;
-  for (int a0 = ua; a0 < ub; a0 += EC_G) {
+  for (int a0 = ua; a0 < ub; a0 += G) {
     const int iu = a0 + g < ub ? fsucc[a0 + g] : -1;
-    for (int b0 = va; b0 < vb; b0 += EC_G) {
+    for (int b0 = va; b0 < vb; b0 += G) {
       const int iv = b0 + g < vb ? fsucc[b0 + g] : -2;
-      const int n = vb - b0 < EC_G ? vb - b0 : EC_G;
-      for (int j = 0; j < n; j++) dup += __shfl(iv, j, EC_G) == iu;
+      const int n = vb - b0 < G ? vb - b0 : G;
+      for (int j = 0; j < n; j++) dup += __shfl(iv, j, G) == iu;
     }
   }
-  dup = ec_group_sum(dup);
+  dup = ec_group_sum<G>(dup);
   if (dup > 2) {
     if (g == 0) ec_finish(c, vmin, e, u, v, 0xFFFFFFFFu, true);
     return;
@@ -604,11 +609,11 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__
   V3 m = edge_position(Q, u, v, v0, v1, placement);
   float el = 0.0f, Qa = 0.0f;
   int ne = 0, nt = 0, flip = 0;
-  ec_fan(P, sface, splane, slen, ua, ub, g, u, v, m, el, Qa, ne, nt, flip);
-  ec_fan(P, sface, splane, slen, va, vb, g, v, u, m, el, Qa, ne, nt, flip);
-  ne = ec_group_sum(ne);
-  nt = ec_group_sum(nt);
-  flip = ec_group_sum(flip);
+  ec_fan<G>(P, sface, splane, slen, ua, ub, g, u, v, m, el, Qa, ne, nt, flip);
+  ec_fan<G>(P, sface, splane, slen, va, vb, g, v, u, m, el, Qa, ne, nt, flip);
+  ne = ec_group_sum<G>(ne);
+  nt = ec_group_sum<G>(nt);
+  flip = ec_group_sum<G>(flip);
   if (g != 0) return;
   if (flip) { ec_finish(c, vmin, e, u, v, 0xFFFFFFFFu, true); return; }
   el = el / (float)ne / scale * tau;
@@ -966,27 +971,31 @@ __global__ void k_scene_init_b(int* scene, int M) {
 }
 // 64-bit keys: mesh index above the 30-bit Morton code of the box centre in
 // the mesh's admission box
-// mesh id above the 30-bit Morton code of the box centre in the mesh's box
-__device__ __forceinline__ u64 mesh_morton_key(int k, const MeshParam* __restrict__ mp, float4 lo, float4 hi) {
+// mesh id above the (3 x bits)-bit Morton code of the box centre in the
+// mesh's box (bits <= 10 per axis; ties are broken by index in the Karras
+// build, so the bit width changes the tree's shape, never the candidates)
+__device__ __forceinline__ u64 mesh_morton_key(int k, const MeshParam* __restrict__ mp, float4 lo, float4 hi,
+                                               int bits) {
   const float* box = mp[k].box;
   float c[3] = {0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
+  const float cells = (float)(1 << bits);
   uint32_t q[3];
 #pragma unroll
   for (int a = 0; a < 3; a++) {
     float ext = box[3 + a] - box[a];
     float t = ext > 0.0f ? (c[a] - box[a]) / ext : 0.0f;
-    t = fminf(fmaxf(t * 1024.0f, 0.0f), 1023.0f);
+    t = fminf(fmaxf(t * cells, 0.0f), cells - 1.0f);
     q[a] = (uint32_t)t;
   }
-  return ((u64)(uint32_t)k << 30) | (u64)(expand10(q[0]) * 4 + expand10(q[1]) * 2 + expand10(q[2]));
+  return ((u64)(uint32_t)k << (3 * bits)) | (u64)(expand10(q[0]) * 4 + expand10(q[1]) * 2 + expand10(q[2]));
 }
 __global__ void k_morton_b(const float4* __restrict__ lbox_lo, const float4* __restrict__ lbox_hi,
                            const int* __restrict__ n_ptr, const int* __restrict__ leaf_face,
                            const int* __restrict__ fmesh, const MeshParam* __restrict__ mp, u64* __restrict__ keys,
-                           int* __restrict__ vals) {
+                           int* __restrict__ vals, int bits) {
   int l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= *n_ptr) return;
-  keys[l] = mesh_morton_key(fmesh[leaf_face[l]], mp, lbox_lo[l], lbox_hi[l]);
+  keys[l] = mesh_morton_key(fmesh[leaf_face[l]], mp, lbox_lo[l], lbox_hi[l], bits);
   vals[l] = l;
 }
 // the face's box (the expressions of k_leaf_boxes_b)
@@ -1006,13 +1015,13 @@ __device__ __forceinline__ void face_box(const float* __restrict__ P, I3 t, floa
 // candidate set; the order only decides how much the culling saves.
 __global__ void k_sorder_keys(const int* __restrict__ T0, const float* __restrict__ P0, int F,
                               const int* __restrict__ fmesh, const MeshParam* __restrict__ mp, u64* __restrict__ keys,
-                              int* __restrict__ vals) {
+                              int* __restrict__ vals, int bits) {
   int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= F) return;
   const I3 t = ldt(T0, f);   // every face of the snapshot is alive
   float4 lo, hi;
   face_box(P0, t, lo, hi);
-  keys[f] = mesh_morton_key(fmesh[f], mp, lo, hi);
+  keys[f] = mesh_morton_key(fmesh[f], mp, lo, hi, bits);
   vals[f] = f;
 }
 // keys of an admitted mesh's faces [f0, f0 + n): the Morton code of the box
@@ -1355,15 +1364,16 @@ __global__ void k_karras64(const u64* __restrict__ key, const int* __restrict__ 
   rfirst[i] = lo;
   rlast[i] = hi;
 }
-// sorted leaf ranges per mesh: seg_a[k] / seg_b[k] (first / last sorted leaf)
+// sorted leaf ranges per mesh: seg_a[k] / seg_b[k] (first / last sorted
+// leaf); the mesh id sits above the code_bits-bit Morton code of the key
 __global__ void k_seg_bounds(const u64* __restrict__ key, const int* __restrict__ n_ptr, int* __restrict__ seg_a,
-                             int* __restrict__ seg_b) {
+                             int* __restrict__ seg_b, int code_bits) {
   int l = blockIdx.x * blockDim.x + threadIdx.x;
   int n = *n_ptr;
   if (l >= n) return;
-  int k = (int)(key[l] >> 30);
-  if (l == 0 || (int)(key[l - 1] >> 30) != k) seg_a[k] = l;
-  if (l == n - 1 || (int)(key[l + 1] >> 30) != k) seg_b[k] = l;
+  int k = (int)(key[l] >> code_bits);
+  if (l == 0 || (int)(key[l - 1] >> code_bits) != k) seg_a[k] = l;
+  if (l == n - 1 || (int)(key[l + 1] >> code_bits) != k) seg_b[k] = l;
 }
 // subtree root of mesh k (the node whose range is exactly [a, b]: a radix
 // tree has one, and a Karras node is numbered by one end of its range) and

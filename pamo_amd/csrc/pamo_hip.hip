@@ -14,6 +14,7 @@
 #include <initializer_list>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <string>
 #include <vector>
 #include <mutex>
@@ -54,6 +55,65 @@ inline uint32_t collapse_threshold(float tau) {
 
 struct Engine;
 
+// Launch-shape parameters of the engine.  None changes a result: they pick
+// work distributions whose outputs are identical (lanes per edge of the cost
+// kernel, Morton bits of the LBVH and spatial-order keys, the face counts at
+// which the exhaustive search hands over to the LBVH and splits its pairs
+// over waves, the walk's lanes per query, the exhaustive search's persistent
+// workgroups).  The defaults are the hand-measured values; pamo_set_tuning
+// or a PAMO_TUNING_FILE (a flat JSON object of these names) overrides them.
+struct Tuning {
+  long long ec_lanes = 4;               // lanes per edge of the cost kernel (2, 4 or 8)
+  long long morton_bits = 10;           // Morton bits per axis (6..10)
+  long long bf_max_faces = 262144;      // exhaustive search for meshes up to this many faces, else the LBVH
+  long long bf_split_max_faces = 65536; // exhaustive pairs split over four waves up to this many alive faces
+  long long walk8_max = 65536;          // LBVH walk: 8 lanes per query up to this many faces ...
+  long long walk4_max = 262144;         // ... 4 lanes up to this many, else 1
+  long long emit_groups = 2048;         // persistent workgroups of the exhaustive search's pair kernel
+};
+struct TuningField { const char* name; long long Tuning::*field; long long lo, hi; };
+static const TuningField kTuningFields[] = {
+    {"ec_lanes", &Tuning::ec_lanes, 2, 8},
+    {"morton_bits", &Tuning::morton_bits, 6, 10},
+    {"bf_max_faces", &Tuning::bf_max_faces, 0, 1LL << 40},
+    {"bf_split_max_faces", &Tuning::bf_split_max_faces, 0, 1LL << 40},
+    {"walk8_max", &Tuning::walk8_max, 0, 1LL << 40},
+    {"walk4_max", &Tuning::walk4_max, 0, 1LL << 40},
+    {"emit_groups", &Tuning::emit_groups, 1, 65536},
+};
+static bool tuning_set(Tuning& t, const char* name, long long v) {
+  for (const TuningField& f : kTuningFields)
+    if (strcmp(f.name, name) == 0) {
+      if (v < f.lo || v > f.hi) return false;
+      if (f.field == &Tuning::ec_lanes && v != 2 && v != 4 && v != 8) return false;
+      t.*(f.field) = v;
+      return true;
+    }
+  return false;
+}
+// a flat JSON object {"name": integer, ...}; unknown names are ignored
+static void tuning_load_file(Tuning& t, const char* path) {
+  FILE* fp = fopen(path, "rb");
+  if (!fp) return;
+  std::string txt;
+  char buf[4096];
+  size_t n;
+  while ((n = fread(buf, 1, sizeof(buf), fp)) > 0) txt.append(buf, n);
+  fclose(fp);
+  size_t i = 0;
+  while ((i = txt.find('"', i)) != std::string::npos) {
+    size_t j = txt.find('"', i + 1);
+    if (j == std::string::npos) break;
+    std::string key = txt.substr(i + 1, j - i - 1);
+    size_t k = txt.find(':', j);
+    if (k == std::string::npos) break;
+    char* end = nullptr;
+    long long v = strtoll(txt.c_str() + k + 1, &end, 10);
+    if (end != txt.c_str() + k + 1) (void)tuning_set(t, key.c_str(), v);
+    i = end ? (size_t)(end - txt.c_str()) : k + 1;
+  }
+}
+
 struct pamo_ctx {
   Engine* eng = nullptr;   // batched engine state (pamo_engine.inc)
   int device = 0;
@@ -83,6 +143,7 @@ struct pamo_ctx {
   int walk_split = -1;            // test hook: lanes per query of the LBVH walk, log2 (-1: by size)
   int strict = 0;                 // strict guard (engine; not in the reference)
   int placement = 0;              // 0 edge midpoint (the reference's), 1 optimal vertex (engine; not in the reference)
+  Tuning tune;                    // launch shapes (results do not depend on them)
 
   int* hc = nullptr;   // pinned host mirror of the counters
 
@@ -853,6 +914,19 @@ int pamo_debug_guard(pamo_ctx* ctx, int mode) {
   return 0;
 }
 
+int pamo_set_tuning(pamo_ctx* ctx, const char* name, long long value) {
+  if (!ctx || !name) return -1;
+  if (!tuning_set(ctx->tune, name, value)) { ctx->fail("unknown tuning name or value out of range: %s", name); return -1; }
+  return 0;
+}
+
+int pamo_get_tuning(pamo_ctx* ctx, const char* name, long long* value) {
+  if (!ctx || !name || !value) return -1;
+  for (const TuningField& f : kTuningFields)
+    if (strcmp(f.name, name) == 0) { *value = ctx->tune.*(f.field); return 0; }
+  return -1;
+}
+
 int pamo_set_placement(pamo_ctx* ctx, int mode) {
   if (!ctx || mode < 0 || mode > 1) return -1;
   ctx->placement = mode;
@@ -897,6 +971,7 @@ int pamo_ctx_create(pamo_ctx** out, int device, int shuffle_mode) {
   pamo_ctx* c = new pamo_ctx();
   c->device = device;
   c->shuffle_mode = shuffle_mode;
+  if (const char* tf = getenv("PAMO_TUNING_FILE")) tuning_load_file(c->tune, tf);
   if (hipHostMalloc((void**)&c->hc, C_COUNT * sizeof(int), hipHostMallocDefault) != hipSuccess) {
     delete c;
     return -3;
@@ -1136,6 +1211,7 @@ int pamo_run_stage2_noalloc(pamo_ctx* ctx, void* stream, const float* d_verts, c
   c.pair_cap_limit = ctx->pair_cap_limit;
   c.walk_split = ctx->walk_split;
   c.placement = ctx->placement;
+  c.tune = ctx->tune;
   c.arena = nd.region;
   c.arena_cap = PAMO_NOALLOC_SCRATCH_BYTES;
   c.hc = c.hc_host;

@@ -1,0 +1,141 @@
+"""Autotuner of the engine's launch shapes: ``python -m pamo_amd.tune``.
+
+Coordinate descent over the parameters of pamo_set_tuning (lanes per edge
+of the cost kernel, Morton bits per axis, the exhaustive-search / LBVH face
+limit, the pair split limit, the walk's lanes per query, the exhaustive
+search's persistent workgroups) on two workloads: the plain 2.0M-face SDF
+union and a 64-mesh sample of the many-mesh mix.  Every candidate's outputs
+must equal the default configuration's bit for bit (the parameters only
+distribute work); the objective is the sum of both workloads' times relative
+to the defaults.  The choice is written as a flat JSON object to
+pamo_amd/tuning.json (or --out), which every Context applies; C callers can
+point the environment variable PAMO_TUNING_FILE at the same file.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+from . import _lib, batch, preprocess_numpy, simplify_stage2, synth
+
+CANDIDATES = {
+    "ec_lanes": [2, 4, 8],
+    "morton_bits": [8, 9, 10],
+    "bf_max_faces": [65536, 131072, 262144, 524288],
+    "bf_split_max_faces": [0, 32768, 65536, 131072],
+    "walk8_max": [16384, 65536, 131072],
+    "walk4_max": [131072, 262144, 524288],
+    "emit_groups": [1024, 2048, 4096],
+}
+
+
+def _log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class Bench:
+    def __init__(self, grid=300, batch_meshes=64, reps=3):
+        dev = torch.device("cuda:0")
+        v, f = synth.sdf_union(200, grid, seed=2, k=0.02)
+        verts, self.scale, _ = preprocess_numpy(v, f)
+        self.target = int(0.001 * f.shape[0])
+        self.dv, self.df = torch.from_numpy(verts).to(dev), torch.from_numpy(f).to(dev)
+        specs = synth.batch_specs(batch_meshes, seed_base=6_000_000)
+        self.meshes = [(v, f, r) for v, f, r in synth.batch_meshes(specs)]
+        self.prep = [batch.prepare(v, f, dev) for v, f, r in self.meshes]
+        self.reps = reps
+
+    def run(self, tuning):
+        """Median seconds of the plain run and of the batch, and their outputs."""
+        ctx = _lib.Context(0, tuning=False)
+        for k, v in tuning.items():
+            ctx.set_tuning(k, v)
+        outs = []
+        tp, tb = [], []
+        for r in range(self.reps + 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            gv, gf, _ = simplify_stage2(self.dv, self.df, self.scale, self.target, ctx=ctx)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            eng = batch.BatchEngine(ctx=ctx)
+            hs = {}
+            for i, ((dv, dt, sc, m), (v, f, rr)) in enumerate(zip(self.prep, self.meshes)):
+                hs[eng.submit(dv, dt, sc, int(rr * len(f)), job=i)] = i
+            res = {}
+            while len(res) < len(self.meshes):
+                _, active = eng.step(1)
+                for h, _ in eng.poll():
+                    res[hs[h]] = eng.take(h)[:2]
+                if active == 0 and len(res) < len(self.meshes):
+                    raise RuntimeError("engine idle")
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            if r:   # the first repetition warms the buffers
+                tp.append(t1 - t0)
+                tb.append(t2 - t1)
+            if r == self.reps:
+                outs = [gv.cpu().numpy().tobytes(), gf.cpu().numpy().tobytes()]
+                for i in range(len(self.meshes)):
+                    outs += [res[i][0].cpu().numpy().tobytes(), res[i][1].cpu().numpy().tobytes()]
+        del ctx
+        return float(np.median(tp)), float(np.median(tb)), outs
+
+
+def tune(out_path, grid=300, batch_meshes=64, reps=3, rounds=2):
+    b = Bench(grid, batch_meshes, reps)
+    best = dict(_lib.Context(0, tuning=False).tuning())
+    b.run(best)   # the first runs of a process are slower (clocks, caches): not a reference
+    tp0, tb0, ref = b.run(best)
+    _log("defaults %s: plain %.1f ms, batch %.1f ms" % (best, 1e3 * tp0, 1e3 * tb0))
+    score = lambda tp, tb: tp / tp0 + tb / tb0
+    cur = score(tp0, tb0)
+    trials = []
+    for it in range(rounds):
+        changed = False
+        for name, vals in CANDIDATES.items():
+            for v in vals:
+                if v == best[name]:
+                    continue
+                cand = dict(best, **{name: v})
+                tp, tb, outs = b.run(cand)
+                if outs != ref:
+                    raise RuntimeError("tuning %s=%d changed a result" % (name, v))
+                s = score(tp, tb)
+                trials.append(dict(cand, plain_ms=round(1e3 * tp, 2), batch_ms=round(1e3 * tb, 2)))
+                _log("%s=%d: plain %.1f ms, batch %.1f ms, score %.4f (best %.4f)" % (name, v, 1e3 * tp, 1e3 * tb,
+                                                                                    s, cur))
+                if s < cur * 0.995:   # keep a change only for a measurable gain
+                    best, cur, changed = cand, s, True
+        if not changed:
+            break
+    tp, tb, _ = b.run(best)
+    doc = dict(best)
+    doc["_measured"] = dict(plain_ms_default=round(1e3 * tp0, 2), batch_ms_default=round(1e3 * tb0, 2),
+                            plain_ms=round(1e3 * tp, 2), batch_ms=round(1e3 * tb, 2),
+                            workloads="plain: SDF union of 200 spheres, seed 2, %d^3 lattice, ratio 0.001; "
+                                      "batch: %d meshes of the many-mesh mix, seeds 6000000+i" % (grid, batch_meshes),
+                            device=torch.cuda.get_device_name(0), trials=len(trials))
+    with open(out_path, "w") as fp:
+        json.dump(doc, fp, indent=1)
+    _log("wrote %s: %s" % (out_path, best))
+    return doc
+
+
+def main():
+    p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    p.add_argument("--out", default=_lib.TUNING_FILE)
+    p.add_argument("--grid", type=int, default=300)
+    p.add_argument("--batch-meshes", type=int, default=64)
+    p.add_argument("--reps", type=int, default=3)
+    p.add_argument("--rounds", type=int, default=2)
+    a = p.parse_args()
+    tune(a.out, a.grid, a.batch_meshes, a.reps, a.rounds)
+
+
+if __name__ == "__main__":
+    main()
