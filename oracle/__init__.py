@@ -16,7 +16,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_ORC = os.path.join(_HERE, "libpamo_oracle.so")
+# PAMO_ORACLE_LIB names another build of this oracle (the mutation tests load
+# deliberately broken copies through it); by default the one built here
+_ORC = os.environ.get("PAMO_ORACLE_LIB") or os.path.join(_HERE, "libpamo_oracle.so")
 
 SHUFFLE_IDENTITY = 0
 SHUFFLE_FEISTEL = 1
@@ -29,6 +31,8 @@ def build():
 
 def _load(path):
     if not os.path.exists(path):
+        if os.environ.get("PAMO_ORACLE_LIB"):
+            raise FileNotFoundError(path)
         build()
     return ctypes.CDLL(path)
 
