@@ -17,6 +17,7 @@
 #include "pamo_bvh.h"
 #include "pamo_kernels.h"
 #include "placement.h"
+#include "flip_filter.h"
 
 namespace pamo {
 namespace eng {
@@ -497,6 +498,43 @@ __device__ __forceinline__ int ec_group_sum(int x) {
   return x;
 }
 
+// The moved face's term (cusimp_free.cu:497-575), or -1 when it turns over
+// (a term lies in [0, 1]).  Two steps are cheaper than the reference's and
+// give the same bits:
+//  * the edge that does not touch the moved corner keeps its length, read
+//    from the slot record (the same expression over the same positions);
+//  * the flip test is decided from the unnormalised normal where
+//    flip_filter.h proves the answer, else by the reference's
+//    normalise-then-dot.
+__device__ __forceinline__ float ec_moved_term(const float* __restrict__ P, int4 t, float4 L, float4 op, int moved,
+                                               V3 m) {
+  const bool mx = t.x == moved, my = t.y == moved, mz = t.z == moved;
+  const V3 n0 = mx ? m : ldp(P, t.x);
+  const V3 n1 = my ? m : ldp(P, t.y);
+  const V3 n2 = mz ? m : ldp(P, t.z);
+  const V3 nn = vcross(vsub(n1, n0), vsub(n2, n0));
+  int fl = flipf::flip_decide(op.x, op.y, op.z, nn.x, nn.y, nn.z);
+  if (fl < 0) fl = vdot(V3{op.x, op.y, op.z}, vdivs(nn, vnorm(nn))) < 0;
+  if (fl) return -1.0f;
+  const V3 d01 = vsub(n0, n1), d12 = vsub(n1, n2), d20 = vsub(n2, n0);
+  float e01, e12, e20;
+  if ((int)mx + (int)my + (int)mz == 1) {
+    // the two edges at the moved corner; the third from the slot record
+    const float lx = vnorm(mz ? d12 : d01);
+    const float ly = vnorm(my ? d12 : d20);
+    e01 = mz ? L.x : lx;
+    e12 = mx ? L.y : (my ? ly : lx);
+    e20 = my ? L.z : ly;
+  } else {
+    e01 = vnorm(d01); e12 = vnorm(d12); e20 = vnorm(d20);
+  }
+  const float sh = (e01 + e12 + e20) / 2;
+  const float area = sqrtf(sh * (sh - e01) * (sh - e12) * (sh - e20));
+  const float k43 = 4.0f * 1.73205077648162841796875f;   // 4.0f * sqrtf(3.0f)
+  const float term = k43 * area / (e01 * e01) + e12 * e12 + e20 * e20 + 0.0000001f;
+  return 1.0f - clamp_ref(term, 0.0f, 1.0f);
+}
+
 // one fan's slots for the edge: serial length and skinny sums, counts, flip
 template <int G>
 __device__ __forceinline__ void ec_fan(const float* __restrict__ P, const int4* __restrict__ sface,
@@ -513,20 +551,8 @@ __device__ __forceinline__ void ec_fan(const float* __restrict__ P, const int4* 
       if (t.y > t.z) { ty = L.y; ne++; }
       if (t.z > t.x) { tz = L.z; ne++; }
       if (!(t.x == other || t.y == other || t.z == other)) {
-        V3 n0 = t.x == moved ? m : ldp(P, t.x);
-        V3 n1 = t.y == moved ? m : ldp(P, t.y);
-        V3 n2 = t.z == moved ? m : ldp(P, t.z);
-        float4 op = splane[s];
-        V3 on = {op.x, op.y, op.z};
-        V3 nn = vcross(vsub(n1, n0), vsub(n2, n0));
-        nn = vdivs(nn, vnorm(nn));
-        if (vdot(on, nn) < 0) flip = 1;
-        float e01 = vnorm(vsub(n0, n1)), e12 = vnorm(vsub(n1, n2)), e20 = vnorm(vsub(n2, n0));
-        float sh = (e01 + e12 + e20) / 2;
-        float area = sqrtf(sh * (sh - e01) * (sh - e12) * (sh - e20));
-        const float k43 = 4.0f * 1.73205077648162841796875f;   // 4.0f * sqrtf(3.0f)
-        float term = k43 * area / (e01 * e01) + e12 * e12 + e20 * e20 + 0.0000001f;
-        tq = 1.0f - clamp_ref(term, 0.0f, 1.0f);
+        tq = ec_moved_term(P, t, L, splane[s], moved, m);
+        if (tq < 0.0f) { flip = 1; tq = 0.0f; }
         nt++;
       }
     }
