@@ -32,7 +32,29 @@ struct MeshParam {
   float box[6];           // vertex bounds at admission (lo xyz, hi xyz): every
                           // later position is a midpoint of earlier ones or a
                           // restored original, so the box bounds the mesh
+  long long target, iters;   // termination (simp_cuda/pamo/__init__.py:142-158)
+  int tolerance, pad;
 };
+
+// A mesh's run state between rounds, kept on the device by the round's
+// status kernel: its sizes after the last round and the termination
+// counters of PaMO.run (simp_cuda/pamo/__init__.py:142-158).
+struct MeshState {
+  int F, V, n_undo;   // faces, vertices and undo-list length after the last round
+  int count, stuck;   // rounds without a change of F; the is_stuck flag
+  int rounds;         // rounds run
+  int exit;           // 0 running, 1 target, 2 <= 10 faces, 3 tolerance, 4 iteration cap
+  int done;
+};
+// Words of the engine's live-size buffer: the next round's totals (the
+// round reads them on the device, so a round can be enqueued before the host
+// knows them), the attention flag and the count of meshes the last round
+// finished.  A non-zero attention flag skips every launch of a round queued
+// behind it (gated()): the last round marked collapses for undo, or
+// finished a mesh, and the host acts before anything else runs.
+// The epoch word is the base of the scans' device epochs in a replayed
+// graph (k_scan_op); each round's status moves it on.
+constexpr int LIVE_V = 0, LIVE_F = 1, LIVE_UIN = 2, LIVE_ATTN = 3, LIVE_DONE = 4, LIVE_EPOCH = 5, LIVE_WORDS = 8;
 
 // Undo passes per round: the reference repeats the check at most 5 times
 // (cusimp_free.cu:1014-1040); the strict guard repeats until nothing is
@@ -61,7 +83,9 @@ __device__ __forceinline__ int find_seg(const long long* __restrict__ off, int M
 // every incoming undo entry marks its mesh's table (cusimp_free.cu:910-916);
 // bounds-checked (the reference is not)
 __global__ void k_undo_table(const int* __restrict__ undo_in, const long long* __restrict__ uioff, int M,
-                             const MeshParam* __restrict__ mp, uint8_t* __restrict__ table, int* err) {
+                             const MeshParam* __restrict__ mp, uint8_t* __restrict__ table, int* err,
+                             const int* __restrict__ gate) {
+  if (gated(gate)) return;
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= uioff[M]) return;
   int k = find_seg(uioff, M, i);
@@ -73,7 +97,9 @@ __global__ void k_undo_table(const int* __restrict__ undo_in, const long long* _
 // of their undo output (cusimp_free.cu:899-904)
 __global__ void k_undo_remap_in(const int* __restrict__ undo_in, const long long* __restrict__ uioff, int M,
                                 const MeshParam* __restrict__ mp, const int* __restrict__ pm,
-                                const long long* __restrict__ uobase, int* __restrict__ undo_out) {
+                                const long long* __restrict__ uobase, int* __restrict__ undo_out,
+                                const int* __restrict__ gate) {
+  if (gated(gate)) return;
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= uioff[M]) return;
   int k = find_seg(uioff, M, i);
@@ -100,14 +126,16 @@ __global__ void k_init_mesh_state(int M, const int* __restrict__ nust, int* __re
 // the table is read at local vertex indices, which stay below the mesh's
 // current vertex count for the rest of the run (it never grows), so entries
 // past it are never read again.
-__global__ void k_round_init(long long V, long long F, int M, int* __restrict__ counters, int n_counters,
+__global__ void k_round_init(const int* __restrict__ live, int M, int* __restrict__ counters, int n_counters,
                              int c_v, int c_f, unsigned* __restrict__ cstats, int n_cstats,
                              const int* __restrict__ nust, int* __restrict__ first, int* __restrict__ nvu,
                              int* __restrict__ lcnt, int* __restrict__ cnt, int* __restrict__ ucnt,
                              int* __restrict__ occ, u64* __restrict__ vmin, const int* __restrict__ vmesh,
                              const MeshParam* __restrict__ mp, const long long* __restrict__ uioff,
                              const long long* __restrict__ voff, uint8_t* __restrict__ table,
-                             int* __restrict__ npass) {
+                             int* __restrict__ npass, const int* __restrict__ gate) {
+  if (gated(gate)) return;
+  const long long V = live[LIVE_V], F = live[LIVE_F];
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < V) {
     const int k = vmesh[i];
@@ -132,10 +160,11 @@ __global__ void k_round_init(long long V, long long F, int M, int* __restrict__ 
 // The count's atomic returns the face's slot in each corner's fan (in an
 // arbitrary order: k_fan_slots sorts every fan), kept in frank so the fill
 // is plain stores.
-__global__ void k_fan_count_snap(const int* __restrict__ T, int F, int* cnt, int* __restrict__ T0,
-                                 int* __restrict__ frank) {
+__global__ void k_fan_count_snap(const int* __restrict__ T, const int* __restrict__ F_ptr, int* cnt,
+                                 int* __restrict__ T0, int* __restrict__ frank, const int* __restrict__ gate) {
+  if (gated(gate)) return;
   int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= F) return;
+  if (f >= *F_ptr) return;
   I3 t = ldt(T, f);
   T0[3LL * f] = t.i;
   T0[3LL * f + 1] = t.j;
@@ -147,10 +176,12 @@ __global__ void k_fan_count_snap(const int* __restrict__ T, int F, int* cnt, int
   frank[3LL * f + 2] = rk;
 }
 // the fans from the counts' slots
-__global__ void k_fan_place(const int* __restrict__ T, int F, const int* __restrict__ fan_off,
-                            const int* __restrict__ frank, int* __restrict__ fan) {
+__global__ void k_fan_place(const int* __restrict__ T, const int* __restrict__ F_ptr,
+                            const int* __restrict__ fan_off, const int* __restrict__ frank, int* __restrict__ fan,
+                            const int* __restrict__ gate) {
+  if (gated(gate)) return;
   int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= F) return;
+  if (f >= *F_ptr) return;
   I3 t = ldt(T, f);
   if (dead3(t)) return;
   const I3 r = ldt(frank, f);
@@ -264,10 +295,14 @@ constexpr int FS_MAX = 32;
 constexpr int FS_GROUPS = BLOCK / FS_G;
 
 __global__ void __launch_bounds__(BLOCK) k_fan_slots(const float* __restrict__ P, const int* __restrict__ T,
-                                                     const int* __restrict__ fan_off, int* __restrict__ fan, int V,
+                                                     const int* __restrict__ fan_off, int* __restrict__ fan,
+                                                     const int* __restrict__ V_ptr,
                                                      int* __restrict__ fsucc, int4* __restrict__ sface,
                                                      float4* __restrict__ splane, float4* __restrict__ slen,
-                                                     float* __restrict__ Q, float* __restrict__ P0) {
+                                                     float* __restrict__ Q, float* __restrict__ P0,
+                                                     const int* __restrict__ gate) {
+  if (gated(gate)) return;   // uniform over the workgroup
+  const int V = *V_ptr;
   __shared__ int s_raw[FS_GROUPS][FS_MAX];
   __shared__ int s_sorted[FS_GROUPS][FS_MAX];
   __shared__ float4 s_plane[FS_GROUPS][FS_MAX];
@@ -342,7 +377,7 @@ __global__ void __launch_bounds__(BLOCK) k_fan_slots(const float* __restrict__ P
 // edge in the vertex-ordered list, k_edge_vorder).
 struct EdgeScanOp {
   const int* T;
-  int F;
+  const int* F_ptr;
   int* eoff;
   int* edges;
   int* ucnt;
@@ -352,7 +387,8 @@ struct EdgeScanOp {
   // f = F (the entry past the last face) reads face max(F - 1, 0), which
   // the buffer always holds, and takes it as dead
   __device__ Rec load(long long f) const {
-    const I3 t = ldt(T, (int)max(min(f, (long long)F - 1), 0LL));
+    const long long F = *F_ptr;
+    const I3 t = ldt(T, (int)max(min(f, F - 1), 0LL));
     Rec r;
     r.t = f < F ? t : I3{-1, -1, -1};
     return r;
@@ -370,7 +406,7 @@ struct EdgeScanOp {
   }
   __device__ void emit(long long f, const Rec& r, int o) const {
     eoff[f] = o;
-    if (f >= F) { win[o] = 0; return; }
+    if (f >= *F_ptr) { win[o] = 0; return; }
     const I3 t = r.t;
     if (t.i > t.j) { edges[2 * o] = t.j; edges[2 * o + 1] = t.i; erank[o] = r.r0; win[o++] = 0; }
     if (t.j > t.k) { edges[2 * o] = t.k; edges[2 * o + 1] = t.j; erank[o] = r.r1; win[o++] = 0; }
@@ -453,7 +489,8 @@ struct FaceCompactOp {
 __global__ void k_edge_vorder(const int* __restrict__ edges, const int* __restrict__ E_ptr,
                               const int* __restrict__ uoff, const int* __restrict__ erank, int4* __restrict__ vedge,
                               int M, const long long* __restrict__ foff, const int* __restrict__ eoff_face,
-                              long long* __restrict__ eoffm) {
+                              long long* __restrict__ eoffm, const int* __restrict__ gate) {
+  if (gated(gate)) return;
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e <= M) eoffm[e] = eoff_face[foff[e]];
   if (e >= *E_ptr) return;
@@ -587,7 +624,9 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__
                                                        const int* __restrict__ vmesh,
                                                        const long long* __restrict__ voff,
                                                        const long long* __restrict__ eoffm,
-                                                       const MeshParam* __restrict__ mp, u64* vmin, int placement) {
+                                                       const MeshParam* __restrict__ mp, u64* vmin, int placement,
+                                                       const int* __restrict__ gate) {
+  if (gated(gate)) return;
   // every lane of a group has the same i, so each return below is taken by
   // the whole group and the width-G shuffles always see all its lanes
   const int i = (blockIdx.x * blockDim.x + threadIdx.x) / G;
@@ -689,11 +728,14 @@ __device__ __forceinline__ int fan_group_or(int x) {
 // round-start sface) has a smaller vertex minimum.  win[] was cleared by the
 // edge emit.  Winners' fans are disjoint, so applying them later, in any
 // order, gives the reference's result.
-__global__ void __launch_bounds__(BLOCK) k_winners_v(int V, const u64* __restrict__ vmin,
+__global__ void __launch_bounds__(BLOCK) k_winners_v(const int* __restrict__ V_ptr, const u64* __restrict__ vmin,
                                                      const int* __restrict__ edges, const int* __restrict__ vmesh,
                                                      const MeshParam* __restrict__ mp,
                                                      const int* __restrict__ fan_off,
-                                                     const int4* __restrict__ sface, int* __restrict__ win) {
+                                                     const int4* __restrict__ sface, int* __restrict__ win,
+                                                     const int* __restrict__ gate) {
+  if (gated(gate)) return;
+  const int V = *V_ptr;
   // a group of FAN_G lanes per vertex; every early return depends on u
   // alone, so it is taken by the whole group
   const int u = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) / FAN_G);
@@ -792,7 +834,9 @@ __global__ void __launch_bounds__(BLOCK) k_collapse_b(const int* __restrict__ wl
                                                       const int4* __restrict__ sface, float* P,
                                                       const float* __restrict__ P0, int* T,
                                                       const int* __restrict__ T0, int* occ,
-                                                      const float* __restrict__ Q, int placement) {
+                                                      const float* __restrict__ Q, int placement,
+                                                      const int* __restrict__ gate) {
+  if (gated(gate)) return;
   // grid-stride over winners; every lane of a group has the same i, so the
   // loop exits, returns and shuffles are group-wide
   const int g = threadIdx.x & (CL_G - 1);
@@ -811,13 +855,17 @@ __global__ void __launch_bounds__(BLOCK) k_collapse_b(const int* __restrict__ wl
 // The face workgroups also do the first check's initialisation
 // (k_check_init's work: nothing between here and the check reads those
 // arrays); nbf covers max(Fl, M) threads.
-__global__ void k_invalid_line_b(int* T, int F, int nbf, const float* __restrict__ P, const int* __restrict__ edges,
-                                 const int* __restrict__ E_ptr, const int* __restrict__ vmesh, int* lcnt, long long Fl,
+__global__ void k_invalid_line_b(int* T, const int* __restrict__ F_ptr, int nbf, const float* __restrict__ P,
+                                 const int* __restrict__ edges, const int* __restrict__ E_ptr,
+                                 const int* __restrict__ vmesh, int* lcnt, long long Fl,
                                  int M, uint8_t* __restrict__ qflag, uint8_t* __restrict__ S,
                                  int* __restrict__ seg_a, int* __restrict__ seg_b, int* __restrict__ kparent,
                                  int* __restrict__ nmark, int* __restrict__ novf,
-                                 unsigned long long* __restrict__ npairs, int* __restrict__ oflag) {
+                                 unsigned long long* __restrict__ npairs, int* __restrict__ oflag,
+                                 const int* __restrict__ gate) {
+  if (gated(gate)) return;
   if ((int)blockIdx.x < nbf) {
+    const int F = *F_ptr;
     int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f < Fl) {
       qflag[f] = 0;
@@ -892,7 +940,8 @@ __global__ void __launch_bounds__(BLOCK) k_line_cut_tail(int M, const int* __res
                                                          int* __restrict__ cut, long long* __restrict__ cutoff,
                                                          long long* __restrict__ uobase,
                                                          const int* __restrict__ nustk, int* __restrict__ part,
-                                                         int strict) {
+                                                         int strict, const int* __restrict__ gate) {
+  if (gated(gate)) return;
   for (int k = threadIdx.x; k < M; k += BLOCK) {
     const int hk = winpos[eoffm[k + 1]] - winpos[eoffm[k]];   // the mesh's winners
     h[k] = hk;
@@ -925,7 +974,8 @@ __global__ void __launch_bounds__(BLOCK) k_line_cut_tail(int M, const int* __res
 __global__ void k_qflag_b(const long long* __restrict__ cutoff, int M, const long long* __restrict__ eoffm,
                           const int* __restrict__ coll, const int* __restrict__ edges,
                           const int* __restrict__ fan_off, const int* __restrict__ fan,
-                          const int* __restrict__ part, uint8_t* __restrict__ qflag) {
+                          const int* __restrict__ part, uint8_t* __restrict__ qflag, const int* __restrict__ gate) {
+  if (gated(gate)) return;
   // a group of FAN_G lanes per collapse takes the two fans' slots in turn
   const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / FAN_G;
   const int g = threadIdx.x & (FAN_G - 1);
@@ -1078,14 +1128,17 @@ __global__ void k_sorder_keys_new(const int* __restrict__ T, const float* __rest
 // and the union of the boxes of each group of 16 positions and of each tile
 // of BLOCK positions (one workgroup), aligned to the position grid and
 // unbounded if a box has a NaN coordinate
-__global__ void __launch_bounds__(BLOCK) k_sorder_boxes(const int* __restrict__ sorder, int F,
+__global__ void __launch_bounds__(BLOCK) k_sorder_boxes(const int* __restrict__ sorder, const int* __restrict__ F_ptr,
                                                         const int* __restrict__ fmesh, const int* __restrict__ part,
                                                         const float* __restrict__ P,
                                                         const int* __restrict__ T, int* __restrict__ sid,
                                                         float4* __restrict__ slo, float4* __restrict__ shi,
                                                         float4* __restrict__ tlo, float4* __restrict__ thi,
-                                                        float4* __restrict__ glo, float4* __restrict__ ghi) {
+                                                        float4* __restrict__ glo, float4* __restrict__ ghi,
+                                                        const int* __restrict__ gate) {
   __shared__ float4 wlo[BLOCK / 64], whi[BLOCK / 64];
+  if (gated(gate)) return;   // uniform over the workgroup
+  const int F = *F_ptr;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.0f), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
   if (p < F) {
@@ -1131,7 +1184,7 @@ __global__ void __launch_bounds__(BLOCK) k_sorder_boxes(const int* __restrict__ 
 // (a query: alive, in a participating mesh, and flagged)
 struct SorderQueryOp {
   const int* sorder;
-  int F;   // >= 1
+  const int* F_ptr;   // live faces (the order's length)
   const int* T;
   const int* fmesh;
   const int* part;
@@ -1140,7 +1193,8 @@ struct SorderQueryOp {
   int* qlist;
   struct Rec { int f, q; };
   __device__ Rec load(long long p) const {
-    const int f = sorder[min(p, (long long)F - 1)];
+    const long long F = *F_ptr;
+    const int f = sorder[max(min(p, F - 1), 0LL)];
     const I3 t = ldt(T, f);
     Rec r;
     r.f = f;
@@ -1157,12 +1211,13 @@ struct SorderQueryOp {
 // the order through the round's face compaction: kept faces, renamed
 struct SorderCompactOp {
   const int* sorder;
-  int F;   // >= 1
+  const int* F_ptr;   // live faces (the order's length)
   const int* kpos;
   int* out;
   struct Rec { int keep, to; };
   __device__ Rec load(long long p) const {
-    const int f = sorder[min(p, (long long)F - 1)];
+    const long long F = *F_ptr;
+    const int f = sorder[max(min(p, F - 1), 0LL)];
     const int a = kpos[f], b = kpos[f + 1];
     Rec r;
     r.keep = p < F ? b - a : 0;
@@ -1726,7 +1781,9 @@ __device__ __forceinline__ float wave_uniform(float x) {
 }
 __global__ void __launch_bounds__(BLOCK) k_bf_blocks(int M, const long long* __restrict__ foff,
                                                      const int* __restrict__ qpos, int* __restrict__ nb,
-                                                     long long* __restrict__ boff, int* __restrict__ npair) {
+                                                     long long* __restrict__ boff, int* __restrict__ npair,
+                                                     const int* __restrict__ gate) {
+  if (gated(gate)) return;
   for (int k = threadIdx.x; k < M; k += BLOCK) {
     int nq = qpos[foff[k + 1]] - qpos[foff[k]];
     nb[k] = (nq + BF_NT - 1) / BF_NT;
@@ -1742,7 +1799,8 @@ __global__ void __launch_bounds__(BF_NT) k_bf_waves(
     const long long* __restrict__ boff, int M, const float* __restrict__ P, const int* __restrict__ T,
     const float4* __restrict__ tlo, const float4* __restrict__ thi, float4* __restrict__ qlo,
     float4* __restrict__ qhi, float4* __restrict__ wlo, float4* __restrict__ whi, int4* __restrict__ wrec,
-    int2* __restrict__ plist, int* __restrict__ npair) {
+    int2* __restrict__ plist, int* __restrict__ npair, const int* __restrict__ gate) {
+  if (gated(gate)) return;
   const int t = threadIdx.x;   // == lane
   const long long bx = blockIdx.x;
   if (bx >= boff[M]) return;   // workgroup-uniform exits
@@ -1807,7 +1865,7 @@ __global__ void __launch_bounds__(BF_NT * WAVES) k_bf_emit(
     const float4* __restrict__ slo, const float4* __restrict__ shi, const float4* __restrict__ gulo,
     const float4* __restrict__ guhi, int2* __restrict__ pairs, long long pair_cap,
     unsigned long long* __restrict__ npairs, int* __restrict__ ovf, int* __restrict__ novf,
-    int* __restrict__ oflag) {
+    int* __restrict__ oflag, const int* __restrict__ gate) {
   constexpr int BF_CAP = BfCap<WAVES>::value;   // candidate slots per lane and wave before a flush
   __shared__ int cls[WAVES][BF_CAP * BF_NT];
   __shared__ float4 blo[BF_TILE], bhi[BF_TILE];
@@ -1817,7 +1875,7 @@ __global__ void __launch_bounds__(BF_NT * WAVES) k_bf_emit(
   const int t = tid & (BF_NT - 1);   // lane == query of the wave
   const int wv = tid / BF_NT;        // this wave's share of the tile's groups
   int* cl = cls[wv];
-  const int np = *npair;
+  const int np = gated(gate) ? 0 : *npair;   // uniform over the grid
   for (int it = blockIdx.x; it < np; it += gridDim.x) {   // workgroup-uniform loop
     // the pair, then the wave's record and the tile together (the tile's
     // loads wait only for the pair; the mesh range masks them afterwards)
@@ -2007,7 +2065,9 @@ template <int STRICT>
 __global__ void __launch_bounds__(BLOCK) k_pairs_bf(int nbp, const int2* __restrict__ pairs,
                                                     const unsigned long long* __restrict__ npairs, long long pair_cap,
                                                     BfFull a, const float* __restrict__ P, const int* __restrict__ T,
-                                                    uint8_t* __restrict__ S, unsigned long long* __restrict__ stats) {
+                                                    uint8_t* __restrict__ S, unsigned long long* __restrict__ stats,
+                                                    const int* __restrict__ gate) {
+  if (gated(gate)) return;
   if ((int)blockIdx.x < nbp) pair_test_blocks<STRICT>(blockIdx.x, nbp, pairs, npairs, pair_cap, P, T, S, stats);
   else bf_full_query<STRICT>((long long)(blockIdx.x - nbp) * blockDim.x + threadIdx.x, a, P, T, S);
 }
@@ -2016,7 +2076,9 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_bf(int nbp, const int2* __restr
 __global__ void k_mark_b(const long long* __restrict__ cutoff, int M, const long long* __restrict__ eoffm,
                          const int* __restrict__ coll, const int* __restrict__ edges,
                          const int* __restrict__ fan_off, const int* __restrict__ fan, const int* __restrict__ part,
-                         const uint8_t* __restrict__ S, int* __restrict__ mflag, int* __restrict__ total) {
+                         const uint8_t* __restrict__ S, int* __restrict__ mflag, int* __restrict__ total,
+                         const int* __restrict__ gate) {
+  if (gated(gate)) return;
   // a group of FAN_G lanes per collapse: hit if a face of either fan is flagged
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long i = gid / FAN_G;
@@ -2046,7 +2108,8 @@ __global__ void k_marked_b(const long long* __restrict__ cutoff, int M, const lo
                            const int* __restrict__ coll, const int* __restrict__ edges, const int* __restrict__ mflag,
                            const int* __restrict__ mpos, int* __restrict__ marked, int* __restrict__ tmp, int nb0,
                            int* __restrict__ hm, int* __restrict__ part, int* __restrict__ nvu,
-                           int* __restrict__ n_active) {
+                           int* __restrict__ n_active, int* __restrict__ attn, const int* __restrict__ gate) {
+  if (gated(gate)) return;
   if ((int)blockIdx.x >= nb0) {
     int k = (blockIdx.x - nb0) * blockDim.x + threadIdx.x;
     if (k >= M) return;
@@ -2054,7 +2117,10 @@ __global__ void k_marked_b(const long long* __restrict__ cutoff, int M, const lo
     hm[k] = c;
     nvu[k] += 2 * c;
     part[k] = c > 0 ? 1 : 0;
-    if (c > 0) atomicAdd(n_active, 1);
+    if (c > 0) {
+      atomicAdd(n_active, 1);
+      *attn = 1;   // the host runs the undo passes before the round's end
+    }
     return;
   }
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2135,7 +2201,7 @@ __global__ void k_local_map(const int* __restrict__ g, const long long* __restri
                             int* __restrict__ pm, int nbv, const long long* __restrict__ uobase,
                             const int* __restrict__ first, const int* __restrict__ nvu, int* __restrict__ undo_out,
                             const int* __restrict__ gate) {
-  if (gate && *gate) return;   // the round's end waits for the undo passes
+  if (gated(gate)) return;   // the round's end waits for the undo passes
   if ((int)blockIdx.x >= nbv) {
     const int k = blockIdx.x - nbv;
     if (k >= M) return;
@@ -2157,7 +2223,7 @@ __global__ void __launch_bounds__(BLOCK) k_new_offsets(int M, const long long* _
                                                        const int* __restrict__ fpos, long long* __restrict__ nvoff,
                                                        long long* __restrict__ nfoff, const int* __restrict__ nvu,
                                                        long long* __restrict__ uioff_new, const int* __restrict__ gate) {
-  if (gate && *gate) return;   // the round's end waits for the undo passes
+  if (gated(gate)) return;   // the round's end waits for the undo passes
   for (int k = threadIdx.x; k <= M; k += BLOCK) {
     nvoff[k] = g[voff[k]];
     nfoff[k] = fpos[foff[k]];
@@ -2167,11 +2233,70 @@ __global__ void __launch_bounds__(BLOCK) k_new_offsets(int M, const long long* _
 // next round's packed undo input from the per-mesh undo outputs
 __global__ void k_pack_undo(int M, const long long* __restrict__ uioff_new, const long long* __restrict__ uobase,
                             const int* __restrict__ undo_out, int* __restrict__ undo_in, const int* __restrict__ gate) {
-  if (gate && *gate) return;   // the round's end waits for the undo passes
+  if (gated(gate)) return;   // the round's end waits for the undo passes
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= uioff_new[M]) return;
   int k = find_seg(uioff_new, M, i);
   undo_in[i] = undo_out[uobase[k] + (i - uioff_new[k])];
+}
+
+// The round's end on the device (simp_cuda/pamo/__init__.py:142-158, per
+// mesh): the new sizes and the termination state, the stuck flag and the
+// stuck undo count the next round reads, the next round's offsets and
+// totals, and the attention flag, set when a mesh finished (the host
+// retires it before the next round).  Unless forced it does nothing while
+// the attention flag is set: the round's first check marked collapses, and
+// the host runs it again, forced, after the undo passes.  One block.
+__global__ void __launch_bounds__(BLOCK) k_round_status(int M, const long long* __restrict__ nvoff,
+                                                        const long long* __restrict__ nfoff,
+                                                        const long long* __restrict__ uioff2, MeshParam* mp,
+                                                        MeshState* __restrict__ ms, int* __restrict__ nustk,
+                                                        long long* __restrict__ voff, long long* __restrict__ foff,
+                                                        long long* __restrict__ uioff, int* live, int force) {
+  __shared__ int ndone;
+  if (!force && live[LIVE_ATTN]) return;   // uniform over the block
+  if (threadIdx.x == 0) ndone = 0;
+  __syncthreads();
+  for (int k = threadIdx.x; k < M; k += BLOCK) {
+    MeshState s = ms[k];
+    const long long target = mp[k].target;
+    const int nprev = s.F;
+    const int ncur = (int)(nfoff[k + 1] - nfoff[k]);
+    s.rounds++;
+    s.F = ncur;
+    s.V = (int)(nvoff[k + 1] - nvoff[k]);
+    s.n_undo = (int)(uioff2[k + 1] - uioff2[k]);
+    bool done = false;
+    if (target >= 0 && (ncur <= target || ncur <= 10)) {
+      s.exit = ncur <= target ? 1 : 2;
+      done = true;
+    } else {
+      if (ncur == nprev) s.count++;
+      else { s.count = 0; s.stuck = 0; }
+      if (s.count >= 2) s.stuck = 1;
+      if (s.count == mp[k].tolerance) { s.exit = 3; done = true; }
+    }
+    if (!done && s.rounds >= mp[k].iters) { s.exit = 4; done = true; }
+    s.done = done ? 1 : 0;
+    ms[k] = s;
+    mp[k].stuck = s.stuck;
+    nustk[k] = s.stuck ? s.n_undo : 0;
+    if (done) atomicAdd(&ndone, 1);
+  }
+  for (int k = threadIdx.x; k <= M; k += BLOCK) {
+    voff[k] = nvoff[k];
+    foff[k] = nfoff[k];
+    uioff[k] = uioff2[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    live[LIVE_V] = (int)nvoff[M];
+    live[LIVE_F] = (int)nfoff[M];
+    live[LIVE_UIN] = (int)uioff2[M];
+    live[LIVE_DONE] = ndone;
+    live[LIVE_ATTN] = ndone > 0 ? 1 : 0;
+    live[LIVE_EPOCH] = (int)(((unsigned)live[LIVE_EPOCH] + SCAN_EPOCH_STEP) & SCAN_EPOCH_MASK);
+  }
 }
 
 // ---------------------------------------------------------- retirement

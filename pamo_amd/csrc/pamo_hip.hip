@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cstdarg>
 #include <initializer_list>
 #include <cstdio>
@@ -70,6 +71,9 @@ struct Tuning {
   long long walk8_max = 65536;          // LBVH walk: 8 lanes per query up to this many faces ...
   long long walk4_max = 262144;         // ... 4 lanes up to this many, else 1
   long long emit_groups = 2048;         // persistent workgroups of the exhaustive search's pair kernel
+  long long pipeline = 1;               // enqueue the next round before reading this one's result (0: never)
+  long long graph = 1;                  // record rounds into hipGraphs and replay them (one mesh; 0: never) ...
+  long long graph_max_faces = 32768;    // ... up to this many faces
 };
 struct TuningField { const char* name; long long Tuning::*field; long long lo, hi; };
 static const TuningField kTuningFields[] = {
@@ -80,6 +84,9 @@ static const TuningField kTuningFields[] = {
     {"walk8_max", &Tuning::walk8_max, 0, 1LL << 40},
     {"walk4_max", &Tuning::walk4_max, 0, 1LL << 40},
     {"emit_groups", &Tuning::emit_groups, 1, 65536},
+    {"pipeline", &Tuning::pipeline, 0, 1},
+    {"graph", &Tuning::graph, 0, 1},
+    {"graph_max_faces", &Tuning::graph_max_faces, 0, 1LL << 40},
 };
 static bool tuning_set(Tuning& t, const char* name, long long v) {
   for (const TuningField& f : kTuningFields)
@@ -133,6 +140,16 @@ struct pamo_ctx {
   Buf kleft, kright, kparent, rfirst, rlast, nodeA, nodeB, S, S_first, mflag, mpos, marked;
   Buf scan_desc, counters, cstats, shuf_flags, shuf_pos;
   unsigned scan_epoch = 0;   // epoch 0 never names a call, so zeroed words read as unpublished
+  // Graph capture of a round (the engine): launches are recorded, not run.
+  // Scans then use their own descriptors with device epochs (the call index
+  // cap_epoch plus the base at cap_epoch_base, which the round's status
+  // kernel advances); a buffer that would have to grow fails the capture
+  // (the round then runs eagerly and grows it).
+  bool capturing = false;
+  unsigned cap_epoch = 0;
+  const unsigned* cap_epoch_base = nullptr;
+  Buf scan_desc_g;
+  long long alloc_gen = 0;   // counts (re)allocations: a recorded graph names the buffers of its generation
   // run loop
   Buf RV[2], RF[2], keep, kpos, undo_in;
   int run_cur = 0, run_V = 0, run_F = 0;
@@ -178,6 +195,20 @@ struct pamo_ctx {
   void prof_end(int h) {
     if (h < 0) return;
     (void)hipEventRecord(prof_open[h].b, st);
+  }
+  int prof_count() const { return (int)prof_open.size(); }
+  // the last n records: dropped (their launches skipped themselves), or
+  // given the byte counts of the sizes their round actually started from
+  void prof_drop_last(int n) {
+    for (; n > 0 && !prof_open.empty(); n--) {
+      ev_pool.push_back(prof_open.back().a);
+      ev_pool.push_back(prof_open.back().b);
+      prof_open.pop_back();
+    }
+  }
+  void prof_rebytes_last(int n, double (*bytes)(int, double, double), double F, double V) {
+    for (int i = std::max(0, (int)prof_open.size() - n); i < (int)prof_open.size(); i++)
+      prof_open[i].bytes = bytes(prof_open[i].k, F, V);
   }
   void prof_resolve() {
     for (auto& r : prof_open) {
@@ -230,6 +261,8 @@ struct pamo_ctx {
   // grow without preserving contents
   bool ensure(Buf& b, size_t bytes) {
     if (bytes <= b.bytes && b.p) return true;
+    if (capturing) return fail("buffer growth during graph capture");
+    alloc_gen++;
     size_t nb = std::max<size_t>(bytes + bytes / 8, 256);
     if (arena) return carve(b, nb);
     if (b.p) (void)hipFree(b.p);
@@ -242,6 +275,8 @@ struct pamo_ctx {
   // grow preserving contents (stream-ordered copy)
   bool ensure_keep(Buf& b, size_t bytes) {
     if (bytes <= b.bytes && b.p) return true;
+    if (capturing) return fail("buffer growth during graph capture");
+    alloc_gen++;
     Buf nbuf;
     size_t nb = std::max<size_t>(bytes + bytes / 2, 256);
     if (arena) {
@@ -296,12 +331,13 @@ struct pamo_ctx {
   // the same scan over a capacity n whose live length is on the device
   // (*d_n + add); entries past the live length are neither read nor written
   bool scan_dev(const int* in, int* out, long long n, const int* d_n, int add, int* total = nullptr,
-                int* list = nullptr) {
+                int* list = nullptr, const int* gate = nullptr) {
     if (n <= 0) return true;
-    return scan_launch(in, out, n, d_n, add, total, list);
+    return scan_launch(in, out, n, d_n, add, total, list, gate);
   }
-  bool scan_launch(const int* in, int* out, long long n, const int* d_n, int add, int* total, int* list = nullptr) {
-    return scan_op(ScanArray{in, out, list}, n, d_n, add, total);
+  bool scan_launch(const int* in, int* out, long long n, const int* d_n, int add, int* total, int* list = nullptr,
+                   const int* gate = nullptr) {
+    return scan_op(ScanArray{in, out, list}, n, d_n, add, total, SCAN_SMALL_MAX, gate);
   }
   // one-pass scan whose input and output are the op's count and emit
   // (inputs up to small_max take the small tile shape; a non-zero *gate on
@@ -313,6 +349,21 @@ struct pamo_ctx {
     const bool small = n <= small_max;
     const long long tile = small ? SCANS_TILE : SCAN1_TILE;
     long long nb = (n + tile - 1) / tile;
+    if (capturing) {
+      // device epochs over the graph's own descriptors (sized and cleared
+      // before the capture)
+      if (scan_desc_g.bytes < (size_t)nb * 8 || cap_epoch + 1 >= SCAN_EPOCH_STEP)
+        return fail("graph capture: scan descriptors");
+      const unsigned ep = ++cap_epoch;
+      unsigned long long* dg = ptr<unsigned long long>(scan_desc_g);
+      if (small)
+        k_scan_op<Op, SCANS_NT, SCANS_ITEMS><<<(unsigned)nb, SCANS_NT, 0, st>>>(
+            op, n, d_n, add, dg, ep, cap_epoch_base, total, cnt_dev(C_ERR), gate);
+      else
+        k_scan_op<Op><<<(unsigned)nb, BLOCK1K, 0, st>>>(op, n, d_n, add, dg, ep, cap_epoch_base, total,
+                                                        cnt_dev(C_ERR), gate);
+      return check_launch("scan");
+    }
     // descriptors start at zero (unpublished); cleared by a fill kernel
     if (scan_desc.bytes < (size_t)nb * 8) {
       if (!ensure(scan_desc, (size_t)nb * 8)) return false;
@@ -324,10 +375,10 @@ struct pamo_ctx {
     }
     if (small)
       k_scan_op<Op, SCANS_NT, SCANS_ITEMS><<<(unsigned)nb, SCANS_NT, 0, st>>>(
-          op, n, d_n, add, ptr<unsigned long long>(scan_desc), scan_epoch, total, cnt_dev(C_ERR), gate);
+          op, n, d_n, add, ptr<unsigned long long>(scan_desc), scan_epoch, nullptr, total, cnt_dev(C_ERR), gate);
     else
       k_scan_op<Op><<<(unsigned)nb, BLOCK1K, 0, st>>>(op, n, d_n, add, ptr<unsigned long long>(scan_desc), scan_epoch,
-                                                      total, cnt_dev(C_ERR), gate);
+                                                      nullptr, total, cnt_dev(C_ERR), gate);
     return check_launch("scan");
   }
   bool read_counters() {
@@ -377,7 +428,7 @@ pamo_ctx::~pamo_ctx() {
                 &ecnt, &eoff, &edges, &Q, &tri_min, &win, &winpos, &collapsed, &qflag, &qf, &qpos, &af, &apos,
                 &qlist, &pairs, &ovf, &seg_lo, &seg_hi, &leaf_face, &lbox_lo, &lbox_hi, &keys0, &keys1, &vals0, &vals1, &hist, &kleft, &kright,
                 &kparent, &rfirst, &rlast, &nodeA, &nodeB, &S, &S_first, &mflag, &mpos, &marked,
-                &scan_desc, &counters, &cstats, &shuf_flags, &shuf_pos, &RV[0], &RV[1], &RF[0], &RF[1], &keep,
+                &scan_desc, &scan_desc_g, &counters, &cstats, &shuf_flags, &shuf_pos, &RV[0], &RV[1], &RF[0], &RF[1], &keep,
                 &kpos, &undo_in};
   for (Buf* b : all)
     if (b->p) (void)hipFree(b->p);
@@ -703,7 +754,7 @@ static void eng_free(pamo_ctx* c) {
                 &g.root, &g.root_end, &g.scene, &g.g, &g.keep, &g.kpos, &g.newk, &g.vkeep, &g.vpos, &g.npass,
                 &g.fsucc, &g.sface, &g.splane, &g.slen, &g.erank, &g.vedge, &g.uoff, &g.vmin, &g.wlist, &g.frank,
                 &g.ucnt, &g.oflag, &g.bfnb, &g.bfoff, &g.pk, &g.sorder, &g.sorder2, &g.nsort, &g.tbox, &g.bfq,
-                &g.bfw, &g.bfl, &g.bfn};
+                &g.bfw, &g.bfl, &g.bfn, &g.ms, &g.live, &g.pkd[0], &g.pkd[1]};
   if (!c->arena) {
     for (Buf* b : all)
       if (b->p) (void)hipFree(b->p);
@@ -712,6 +763,12 @@ static void eng_free(pamo_ctx* c) {
       if (s.outF.p) (void)hipFree(s.outF.p);
     }
     if (g.hb) (void)hipHostFree(g.hb);
+    for (int i = 0; i < 2; i++) {
+      if (g.hd[i]) (void)hipHostFree(g.hd[i]);
+      if (g.dev_done[i]) (void)hipEventDestroy(g.dev_done[i]);
+    }
+    for (auto& e : g.graphs) (void)hipGraphExecDestroy(e.exec);
+    if (g.cap_st) (void)hipStreamDestroy(g.cap_st);
   }
   delete c->eng;
   c->eng = nullptr;
@@ -751,11 +808,7 @@ int pamo_batch_step(pamo_ctx* ctx, void* stream, int max_rounds, int* n_active) 
   c.err.clear();
   if (!eng_ready(c)) return -1;
   int finished = 0;
-  for (int r = 0; r < max_rounds && c.eng->M > 0; r++) {
-    int before = c.eng->M;
-    if (!eng_round(c)) return -1;
-    finished += before - c.eng->M;
-  }
+  if (!eng_rounds(c, max_rounds, -1, &finished)) return -1;
   if (n_active) *n_active = c.eng->M;
   return finished;
 }
@@ -1036,11 +1089,9 @@ int pamo_run_stage2(pamo_ctx* ctx, void* stream, const float* d_verts, const int
                             iters, -1);
   if (h < 0) return h;
   const long long s0[4] = {c.eng->st_checks, c.eng->st_queries, c.eng->st_cand, c.eng->st_hits};
-  while (!c.eng->slots[h].done) {
-    int na = 0;
-    if (pamo_batch_step(ctx, stream, 1, &na) < 0) return -1;
-    if (na == 0 && !c.eng->slots[h].done) { c.fail("engine stopped before the job finished"); return -1; }
-  }
+  c.st = (hipStream_t)stream;
+  if (!eng_rounds(c, INT_MAX, h, nullptr)) return -1;
+  if (!c.eng->slots[h].done) { c.fail("engine stopped before the job finished"); return -1; }
   const EngSlot& s = c.eng->slots[h];
   c.run_faces = s.round_faces;
   c.run_handle = h;

@@ -48,6 +48,12 @@ __device__ __forceinline__ I3 ldt(const int* __restrict__ T, int f) {
 }
 __device__ __forceinline__ bool dead3(I3 t) { return t.i == -1 && t.j == -1 && t.k == -1; }
 
+// Work enqueued ahead of a decision the device makes (a round queued behind
+// one whose result the host has not read) skips itself, writing nothing,
+// while *gate is non-zero; a null gate never skips.  The gate is written by
+// an earlier launch on the same stream, so it is uniform over the grid.
+__device__ __forceinline__ bool gated(const int* gate) { return gate && *gate; }
+
 // ------------------------------------------------------------ generic
 __global__ void k_fill_i32(int* p, int v, long long n) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -152,17 +158,25 @@ __device__ __forceinline__ unsigned long long scan_word(unsigned epoch, unsigned
 // so no item's loads wait behind another item's stores (which the compiler
 // must assume may alias them); the gathers are issued before the
 // look-back and overlap it.
-// A non-zero *gate skips the scan (live length 0; only the total, 0, is
-// written), so work can be enqueued behind a decision the device makes.
+// A non-zero *gate skips the scan and writes nothing (gated()), so work can
+// be enqueued behind a decision the device makes.
+// A scan recorded into a graph that is replayed takes its epoch from the
+// device: the call's index within the graph plus *epoch_base, which moves on
+// by SCAN_EPOCH_STEP every replay (multiples of the step keep the sum
+// non-zero within the 22 bits).
+constexpr unsigned SCAN_EPOCH_MASK = (1u << 22) - 1, SCAN_EPOCH_STEP = 64;
 template <class Op, int NT = BLOCK1K, int ITEMS = SCAN1_ITEMS>
 __global__ void __launch_bounds__(NT) k_scan_op(Op op, long long n, const int* __restrict__ d_n, int add,
-                                                unsigned long long* desc, unsigned epoch, int* __restrict__ total,
+                                                unsigned long long* desc, unsigned epoch,
+                                                const unsigned* __restrict__ epoch_base, int* __restrict__ total,
                                                 int* err, const int* __restrict__ gate) {
   __shared__ unsigned lds[NT / 64];
   constexpr int TILE = NT * ITEMS;
   __shared__ int tile[TILE];
   __shared__ long long excl_s;
-  const long long live = (gate && *gate) ? 0 : (d_n ? min(n, (long long)*d_n + add) : n);
+  if (gated(gate)) return;
+  if (epoch_base) epoch = (epoch + *epoch_base) & SCAN_EPOCH_MASK;
+  const long long live = d_n ? min(n, (long long)*d_n + add) : n;
   const long long t = blockIdx.x;
   const long long base = t * TILE;
   if (live <= 0) {

@@ -30,6 +30,7 @@ CANDIDATES = {
     "walk8_max": [16384, 65536, 131072],
     "walk4_max": [131072, 262144, 524288],
     "emit_groups": [1024, 2048, 4096],
+    "graph_max_faces": [0, 16384, 32768, 65536],
 }
 
 

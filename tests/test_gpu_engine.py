@@ -306,14 +306,23 @@ def test_optimal_placement_matches_oracle(cuda):
     dict(ec_lanes=2, morton_bits=6, emit_groups=1),
     dict(ec_lanes=8, bf_max_faces=0, walk8_max=0, walk4_max=0, morton_bits=7),
     dict(bf_split_max_faces=0, bf_max_faces=1 << 30, emit_groups=65536),
+    dict(pipeline=0, graph=0),
+    dict(pipeline=1, graph=0),
+    dict(pipeline=1, graph=1, graph_max_faces=1 << 30, bf_max_faces=1 << 30),
+    dict(pipeline=0, graph=1, graph_max_faces=1 << 30, bf_max_faces=0),
 ])
 def test_tuning_does_not_change_results(cuda, setting):
     """Launch shapes from the autotuner's range, including its extremes
     (2 and 8 lanes per edge, 6-bit Morton codes, the LBVH at every size or
-    the exhaustive search at every size, one persistent workgroup): every
-    result equals the oracle's, alone and batched.  The batched LBVH with
-    7-bit codes places each mesh's id at bit 21 of the key; the per-mesh
-    leaf ranges must be read from there."""
+    the exhaustive search at every size, one persistent workgroup), and the
+    round schedules (rounds queued behind an unread round or not, recorded
+    into graphs or launched one by one, with the exhaustive search or the
+    LBVH inside the recording): every result equals the oracle's, alone and
+    batched, the batch stepping many rounds per call so that queued rounds
+    meet marks and finished meshes.  Each mesh runs twice on the same
+    context, the second time from the recordings of the first.  The batched
+    LBVH with 7-bit codes places each mesh's id at bit 21 of the key; the
+    per-mesh leaf ranges must be read from there."""
     from pamo_amd import _lib
     meshes = [_meshes()[0], _two_spheres(), synth.sdf_union(20, 64, 8)]
     ctx = _lib.Context(0, tuning=False)
@@ -324,12 +333,16 @@ def test_tuning_does_not_change_results(cuda, setting):
         verts, scale, _ = oracle.preprocess(v, f)
         target = int(0.01 * f.shape[0])
         ov, of, st = oracle.run(verts, f, scale, target)
-        gv, gf, gst = simplify_stage2(torch.from_numpy(verts).to(cuda), torch.from_numpy(f).to(cuda), scale, target,
-                                      ctx=ctx)
-        print("tuning", setting, f.shape, gst["rounds"], st["rounds"], flush=True)
-        assert gst["rounds"] == st["rounds"]
-        assert _same(gv.cpu().numpy(), ov) and _same(gf.cpu().numpy(), of)
-    res = batch.simplify_batch(meshes, 0.01, engine=batch.BatchEngine(ctx=ctx))
+        checks = []
+        for rep in range(2):
+            gv, gf, gst = simplify_stage2(torch.from_numpy(verts).to(cuda), torch.from_numpy(f).to(cuda), scale,
+                                          target, ctx=ctx)
+            print("tuning", setting, f.shape, rep, gst["rounds"], st["rounds"], gst["checks"], flush=True)
+            assert gst["rounds"] == st["rounds"]
+            assert _same(gv.cpu().numpy(), ov) and _same(gf.cpu().numpy(), of)
+            checks.append(gst["checks"])
+        assert checks[0] == checks[1] >= st["rounds"]   # a check per round plus one per undo pass
+    res = batch.simplify_batch(meshes, 0.01, engine=batch.BatchEngine(ctx=ctx), rounds_per_step=64)
     for (v, f), (bv, bf, info) in zip(meshes, res):
         ov, of, _ = _oracle_result(v, f, 0.01)
         assert _same(bv, ov) and _same(bf, of)
