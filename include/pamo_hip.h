@@ -238,7 +238,10 @@ int pamo_set_strict(pamo_ctx* ctx, int strict);
  * result.  Names: "ec_lanes" (lanes per edge of the cost kernel: 2, 4, 8),
  * "morton_bits" (per axis, 6..10), "bf_max_faces" (meshes up to this many
  * faces use the exhaustive search, larger ones the LBVH),
- * "bf_split_max_faces", "walk8_max", "walk4_max", "emit_groups".  A context
+ * "bf_split_max_faces", "walk8_max", "walk4_max", "emit_groups", "pipeline"
+ * (1: the next round is queued before this one's result is read; 0: never),
+ * "graph" and "graph_max_faces" (rounds of one mesh of at most that many
+ * faces are recorded into hipGraphs and replayed; 0: never).  A context
  * starts from the defaults, then from the flat JSON file named by the
  * environment variable PAMO_TUNING_FILE, if set (python -m pamo_amd.tune
  * writes one).  Returns 0, or -1 for an unknown name or a value out of
@@ -257,6 +260,27 @@ int pamo_get_tuning(pamo_ctx* ctx, const char* name, long long* value);
  * cost is scored at that position and the collapse moves u there.  The
  * round kernels (pamo_free_forward) are unaffected. */
 int pamo_set_placement(pamo_ctx* ctx, int mode);
+
+/* Stage 1 (remeshing), the two calls of simp_cuda/pamo/__init__.py:75-93.
+ * pamo_mesh_to_sdf replaces torchcumesh2sdf.get_sdf(tris, R, band)
+ * (__init__.py:82): d_tris holds n_tris triangles as [n_tris, 3, 3] floats
+ * in [0, 1]^3; d_volume receives the R^3 signed distance lattice (sample
+ * (i, j, k) at ((i + 0.5) / R, ...), index (k R + j) R + i): exact distance
+ * to the nearest triangle within band, +-band beyond it, positive outside
+ * (samples connected to the lattice boundary through samples at least
+ * 0.75 / R from the surface), negative inside.
+ * pamo_volume_to_mesh replaces pdmc.DMC(volume) (__init__.py:48,85): the
+ * zero level set of an R^3 volume (positive: outside) as a closed,
+ * consistently oriented (outward) 2-manifold, vertices in the lattice's
+ * [0, 1]^3 coordinates; the counts are returned and the mesh is kept until
+ * pamo_volume_mesh_fetch copies it to caller buffers (n_verts x 3 floats,
+ * n_faces x 3 ints).  R is at most PAMO_REMESH_MAX_R.  Returns 0, -4 for
+ * invalid arguments, -1 for a device error. */
+#define PAMO_REMESH_MAX_R 512
+int pamo_mesh_to_sdf(pamo_ctx* ctx, void* stream, const float* d_tris, int n_tris, int R, float band,
+                     float* d_volume);
+int pamo_volume_to_mesh(pamo_ctx* ctx, void* stream, const float* d_volume, int R, int* n_verts, int* n_faces);
+int pamo_volume_mesh_fetch(pamo_ctx* ctx, void* stream, float* d_verts, int* d_faces);
 
 /* Parity hook: every ordered candidate pair (query, current) of one
  * self-intersection check over all alive faces of a mesh, with the

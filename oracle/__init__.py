@@ -73,6 +73,9 @@ def _lib():
                                        P, P, P, ctypes.c_int]
         _orc.orc_cost_twin.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, P, P, P, P,
                                        P, P, ctypes.c_int]
+        _orc.orc_mesh_to_sdf.argtypes = [P, ctypes.c_longlong, ctypes.c_int, ctypes.c_float, P]
+        _orc.orc_volume_to_mesh.argtypes = [P, ctypes.c_int, P]
+        _orc.orc_mesh_fetch.argtypes = [P, P]
     return _orc
 
 
@@ -361,3 +364,30 @@ def preprocess(points, faces):
     ext = verts.max(axis=0) - verts.min(axis=0)
     scale = max(max(ext[0], ext[1]), ext[2])
     return verts.astype(np.float32), np.float32(scale), tris_mean
+
+
+def mesh_to_sdf(tris, R, band):
+    """Stage 1's signed distance lattice (R^3 float32, index [k, j, i]) of
+    triangles given as [n, 3, 3] float32 in [0, 1]^3 (pamo_mesh_to_sdf's
+    definition)."""
+    tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    vol = np.empty((R, R, R), np.float32)
+    rc = _lib().orc_mesh_to_sdf(_ptr(tris), tris.shape[0], int(R), float(band), _ptr(vol))
+    if rc != 0:
+        raise RuntimeError("orc_mesh_to_sdf failed (%d)" % rc)
+    return vol
+
+
+def volume_to_mesh(vol):
+    """Stage 1's extraction of an R^3 volume's zero level set (positive
+    outside): (verts [V, 3] float32 in [0, 1]^3, faces [F, 3] int32)."""
+    vol = np.ascontiguousarray(vol, np.float32)
+    R = vol.shape[0]
+    cnt = np.zeros(2, np.int64)
+    rc = _lib().orc_volume_to_mesh(_ptr(vol), int(R), _ptr(cnt))
+    if rc != 0:
+        raise RuntimeError("orc_volume_to_mesh failed (%d)" % rc)
+    v = np.empty((int(cnt[0]), 3), np.float32)
+    f = np.empty((int(cnt[1]), 3), np.int32)
+    _lib().orc_mesh_fetch(_ptr(v), _ptr(f))
+    return v, f

@@ -95,6 +95,7 @@
 #include <vector>
 
 #include "../pamo_amd/csrc/exact_pred.h"
+#include "../pamo_amd/csrc/remesh.h"
 #include "../pamo_amd/csrc/placement.h"
 #ifdef _OPENMP
 #include <omp.h>
@@ -1229,6 +1230,174 @@ static int run_shadow(Ctx& c, const float* verts, const int* faces, int V, int F
 }  // namespace orc
 
 // ================================================================ C API
+// ------------------------------------------------------------ stage 1
+// The remeshing pair of simp_cuda/pamo/__init__.py:75-93 as the library
+// defines it (the reference calls torchcumesh2sdf and pdmc, which it does
+// not ship): the signed distance lattice and its zero level set, serially,
+// with the arithmetic of ../pamo_amd/csrc/remesh.h.  The flood is a
+// breadth-first search (the same connected set as the device's sweeps).
+namespace stage1 {
+namespace rm = pamo::rm;
+using rm::F3;
+static const rm::McTab kTab = rm::mc_tables();
+
+static F3 corner(const float* tris, long long t, int c) {
+  const float* q = tris + 9 * t + 3 * c;
+  return F3{q[0], q[1], q[2]};
+}
+static void axis_range(float a, float b, float band, int R, int& lo, int& hi) {
+  lo = (int)std::ceil((a - band) * (float)R - 0.5f);
+  hi = (int)std::floor((b + band) * (float)R - 0.5f);
+  lo = lo < 0 ? 0 : lo;
+  hi = hi > R - 1 ? R - 1 : hi;
+}
+static F3 spos(int i, int j, int k, int R) { return F3{rm::sample_coord(i, R), rm::sample_coord(j, R), rm::sample_coord(k, R)}; }
+
+static void sdf(const float* tris, long long F, int R, float band, float* vol) {
+  const long long n = (long long)R * R * R;
+  std::vector<unsigned long long> key(n, rm::NO_KEY);
+  for (long long t = 0; t < F; t++) {
+    const F3 a = corner(tris, t, 0), b = corner(tris, t, 1), c = corner(tris, t, 2);
+    int i0, i1, j0, j1, k0, k1;
+    axis_range(std::fmin(a.x, std::fmin(b.x, c.x)), std::fmax(a.x, std::fmax(b.x, c.x)), band, R, i0, i1);
+    axis_range(std::fmin(a.y, std::fmin(b.y, c.y)), std::fmax(a.y, std::fmax(b.y, c.y)), band, R, j0, j1);
+    axis_range(std::fmin(a.z, std::fmin(b.z, c.z)), std::fmax(a.z, std::fmax(b.z, c.z)), band, R, k0, k1);
+    for (int k = k0; k <= k1; k++)
+      for (int j = j0; j <= j1; j++)
+        for (int i = i0; i <= i1; i++) {
+          const F3 p = spos(i, j, k, R);
+          const F3 d = rm::f3sub(p, rm::closest_on_triangle(p, a, b, c));
+          const float dist = std::sqrt(rm::f3dot(d, d));
+          if (dist <= band) {
+            unsigned long long& q = key[((long long)k * R + j) * R + i];
+            q = std::min(q, rm::dist_key(dist, (int)t));
+          }
+        }
+  }
+  const float tau_p = 0.75f / (float)R;
+  std::vector<uint8_t> st(n, 0);
+  std::vector<long long> queue;
+  for (long long s = 0; s < n; s++) {
+    const int i = (int)(s % R), j = (int)((s / R) % R), k = (int)(s / ((long long)R * R));
+    const bool pass = key[s] == rm::NO_KEY || rm::key_dist(key[s]) >= tau_p;
+    const bool edge = i == 0 || j == 0 || k == 0 || i == R - 1 || j == R - 1 || k == R - 1;
+    st[s] = (uint8_t)((pass ? rm::ST_PASS : 0) | (pass && edge ? rm::ST_OUT : 0));
+    if (pass && edge) queue.push_back(s);
+  }
+  for (size_t h = 0; h < queue.size(); h++) {
+    const long long s = queue[h];
+    const int i = (int)(s % R), j = (int)((s / R) % R), k = (int)(s / ((long long)R * R));
+    const int dn[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    for (auto& d : dn) {
+      const int x = i + d[0], y = j + d[1], z = k + d[2];
+      if (x < 0 || y < 0 || z < 0 || x >= R || y >= R || z >= R) continue;
+      const long long q = ((long long)z * R + y) * R + x;
+      if ((st[q] & rm::ST_PASS) && !(st[q] & rm::ST_OUT)) {
+        st[q] |= rm::ST_OUT;
+        queue.push_back(q);
+      }
+    }
+  }
+  std::vector<int> votes(std::max<long long>(F, 1), 0);
+  for (long long s = 0; s < n; s++) {
+    if (key[s] == rm::NO_KEY || !(st[s] & rm::ST_PASS)) continue;
+    const long long t = (long long)(uint32_t)key[s];
+    const int i = (int)(s % R), j = (int)((s / R) % R), k = (int)(s / ((long long)R * R));
+    const int side = rm::tri_side(spos(i, j, k, R), corner(tris, t, 0), corner(tris, t, 1), corner(tris, t, 2));
+    if (side == 0) continue;
+    votes[t] += ((st[s] & rm::ST_OUT) != 0) == (side > 0) ? 1 : -1;
+  }
+  for (long long s = 0; s < n; s++) {
+    bool out;
+    float u;
+    if (key[s] == rm::NO_KEY) {
+      out = (st[s] & rm::ST_OUT) != 0;
+      u = band;
+    } else {
+      u = rm::key_dist(key[s]);
+      if (st[s] & rm::ST_PASS) out = (st[s] & rm::ST_OUT) != 0;
+      else {
+        const long long t = (long long)(uint32_t)key[s];
+        const int i = (int)(s % R), j = (int)((s / R) % R), k = (int)(s / ((long long)R * R));
+        const int side = rm::tri_side(spos(i, j, k, R), corner(tris, t, 0), corner(tris, t, 1), corner(tris, t, 2));
+        out = side * (votes[t] >= 0 ? 1 : -1) > 0;
+      }
+    }
+    vol[s] = out ? u : -u;
+  }
+}
+
+static std::vector<float> g_V;
+static std::vector<int> g_F;
+
+// the zero level set: vertices on crossing lattice edges in edge order, then
+// the cubes' centre vertices in cube order; triangles in cube order
+static int extract(const float* vol, int R) {
+  const long long n = (long long)R * R * R, ne = 3 * n;
+  std::vector<int> ids(ne, -1);
+  g_V.clear();
+  g_F.clear();
+  int nv = 0;
+  for (long long e = 0; e < ne; e++) {
+    const long long s = e / 3;
+    const int axis = (int)(e % 3);
+    const int i = (int)(s % R), j = (int)((s / R) % R), k = (int)(s / ((long long)R * R));
+    const int x = axis == 0 ? i : (axis == 1 ? j : k);
+    if (x >= R - 1) continue;
+    const long long o = axis == 0 ? 1 : (axis == 1 ? R : (long long)R * R);
+    if ((vol[s] > 0.0f) == (vol[s + o] > 0.0f)) continue;
+    ids[e] = nv++;
+    const float t = rm::crossing_t(vol[s], vol[s + o]);
+    float c[3] = {(float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f};
+    c[axis] = c[axis] + t;
+    g_V.push_back(c[0] / (float)R);
+    g_V.push_back(c[1] / (float)R);
+    g_V.push_back(c[2] / (float)R);
+  }
+  const int Rc = R - 1;
+  std::vector<float> cent;
+  for (long long cell = 0; cell < (long long)Rc * Rc * Rc; cell++) {
+    const int i = (int)(cell % Rc), j = (int)((cell / Rc) % Rc), k = (int)(cell / ((long long)Rc * Rc));
+    float val[8];
+    bool any_p = false, any_n = false;
+    for (int c = 0; c < 8; c++) {
+      val[c] = vol[((long long)(k + ((c >> 2) & 1)) * R + (j + ((c >> 1) & 1))) * R + (i + (c & 1))];
+      if (val[c] > 0.0f) any_p = true;
+      else any_n = true;
+    }
+    if (!(any_p && any_n)) continue;
+    int next[12], nc = 0;
+    rm::CubeOut out;
+    if (!rm::cube_segments(kTab, val, next)) return -1;
+    const int nt = rm::cube_tris(kTab, next, &out, &nc);
+    if (nt < 0) return -1;
+    const int c0 = (int)(cent.size() / 3);
+    for (int x = 0; x < nc; x++) {
+      float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+      int m = 0;
+      for (; m < 12 && out.cent[x][m] >= 0; m++) {
+        const int id = ids[rm::cube_edge_id(kTab, out.cent[x][m], i, j, k, R)];
+        sx = sx + g_V[3 * (size_t)id];
+        sy = sy + g_V[3 * (size_t)id + 1];
+        sz = sz + g_V[3 * (size_t)id + 2];
+      }
+      cent.push_back(sx / (float)m);
+      cent.push_back(sy / (float)m);
+      cent.push_back(sz / (float)m);
+    }
+    for (int q = 0; q < nt; q++)
+      for (int c = 0; c < 3; c++) {
+        const int l = out.tris[q][c];
+        g_F.push_back(l >= 12 ? -1 - (c0 + (l - 12)) : ids[rm::cube_edge_id(kTab, l, i, j, k, R)]);
+      }
+  }
+  for (int& x : g_F)
+    if (x < 0) x = nv + (-1 - x);
+  g_V.insert(g_V.end(), cent.begin(), cent.end());
+  return 0;
+}
+}  // namespace stage1
+
 extern "C" {
 
 void* orc_create(void) { return new orc::Ctx(); }
@@ -1469,6 +1638,26 @@ int orc_cost_twin(const float* pts, const int* tris, int nV, int nF, float scale
     mid[e] = st32[e] == 0 ? r : 0.0;
   }
   return c.E;
+}
+
+// stage 1: the signed distance lattice of n_tris triangles ([n, 3, 3] in
+// [0, 1]^3) into vol (R^3 floats); the zero level set of vol as a mesh,
+// counts returned, arrays copied by orc_mesh_fetch
+int orc_mesh_to_sdf(const float* tris, long long n_tris, int R, float band, float* vol) {
+  if (R < 2 || n_tris < 0) return -1;
+  stage1::sdf(tris, n_tris, R, band, vol);
+  return 0;
+}
+int orc_volume_to_mesh(const float* vol, int R, long long* counts) {
+  if (R < 2) return -1;
+  if (stage1::extract(vol, R) != 0) return -2;
+  counts[0] = (long long)(stage1::g_V.size() / 3);
+  counts[1] = (long long)(stage1::g_F.size() / 3);
+  return 0;
+}
+void orc_mesh_fetch(float* verts, int* faces) {
+  std::memcpy(verts, stage1::g_V.data(), stage1::g_V.size() * 4);
+  std::memcpy(faces, stage1::g_F.data(), stage1::g_F.size() * 4);
 }
 
 }  // extern "C"

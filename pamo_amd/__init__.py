@@ -4,8 +4,8 @@ Drop-in for the reference's ``pamo`` package on its stage-2 path
 (simp_cuda/pamo/__init__.py): ``PaMO(mesh, use_stage1, use_stage3).run(
 points, triangles, ratio, tolerance, threshold, iter, min_verts)`` and the
 extension module ``pamo_amd._C`` (``CUDSP_Free``).  The rounds run in
-libpamo_hip.so (hand-written gfx950 HIP kernels); stages 1 and 3 are not
-part of this package.
+libpamo_hip.so (hand-written gfx950 HIP kernels), as does stage 1's
+remeshing (``pamo_amd.remesh``); stage 3 is not part of this package.
 """
 import time
 
@@ -108,7 +108,8 @@ def simplify_stage2(verts, faces, scale, target, threshold=1e-3, tolerance=4, it
 
 
 class PaMO(torch.nn.Module):
-    """simp_cuda/pamo/__init__.py:18-178 restricted to stage 2.
+    """simp_cuda/pamo/__init__.py:18-178: stage 1 (remeshing, optional) and
+    stage 2; stage 3 is not part of this package.
 
     ``engine="device"`` (default) keeps the whole round loop on the GPU;
     ``engine="rounds"`` runs the reference's Python loop over
@@ -118,8 +119,6 @@ class PaMO(torch.nn.Module):
     def __init__(self, input_mesh, use_stage1=True, use_stage3=True, verbose=True, engine="device",
                  shuffle=_lib.SHUFFLE_FEISTEL, strict=False, placement="midpoint"):
         super().__init__()
-        if use_stage1:
-            raise NotImplementedError("stage 1 (SDF remeshing) is not part of pamo_amd; pass use_stage1=False")
         if use_stage3:
             raise NotImplementedError("stage 3 (safe projection) is not part of pamo_amd; pass use_stage3=False")
         self.use_stage1 = use_stage1
@@ -138,9 +137,15 @@ class PaMO(torch.nn.Module):
         if placement != "midpoint" and engine != "device":
             raise ValueError("placement=%r needs engine='device' (the round kernels follow the reference)" % placement)
         self.pamo._ctx.set_placement(placement)
+        # stage 1's lattice (simp_cuda/pamo/__init__.py:49-52): the band and
+        # margin stay those of R = 256 when run() lowers R for small targets
+        self.R = 256
+        self.band = 3 / self.R
+        self.margin = self.band * 2 + 1
         self.target_faces = None
         self.last_stats = None
         self.last_stage2_seconds = None
+        self.last_stage1_seconds = None
 
     def run(self, points, triangles, ratio, tolerance=4, threshold=1e-3, iter=1000000, min_verts=10000000000):
         self.target_faces = max(int(ratio * len(triangles)), min_verts)
@@ -151,6 +156,20 @@ class PaMO(torch.nn.Module):
         hv, _, scale, tris_mean = prepare_host(points.cpu().numpy(), triangles.cpu().numpy())
         verts = torch.from_numpy(hv).to(points.device)
         faces = triangles
+        if self.use_stage1:
+            from . import remesh
+            self.R = remesh.resolution(self.target_faces)
+            start1 = time.time()
+            tris, tris_min, tris_max, tris_mean = remesh.normalize(points.cpu().numpy(), triangles.cpu().numpy(),
+                                                                   self.band, self.margin)
+            verts, faces = remesh.remesh(torch.from_numpy(tris).to(points.device), tris_min, tris_max, self.R,
+                                         self.band, self.margin, ctx=self.pamo._ctx)
+            torch.cuda.synchronize()
+            self.last_stage1_seconds = time.time() - start1
+            if self.verbose:
+                print(f"Time for Remeshing: {self.last_stage1_seconds} sec")
+            # stage 2's scale over the remeshed vertices (__init__.py:126)
+            scale = _scale(verts)
         start = time.time()
         if self.engine == "device":
             verts, faces, stats = simplify_stage2(verts, faces, scale, self.target_faces, threshold, tolerance,

@@ -24,6 +24,7 @@
 #include "pamo_bvh.h"
 #include "pamo_engine_kernels.h"
 #include "pamo_kernels.h"
+#include "remesh_kernels.h"
 
 using namespace pamo;
 
@@ -152,6 +153,10 @@ struct pamo_ctx {
   long long alloc_gen = 0;   // counts (re)allocations: a recorded graph names the buffers of its generation
   // run loop
   Buf RV[2], RF[2], keep, kpos, undo_in;
+  // stage 1: distance keys, sample states, triangle votes; extraction ids,
+  // per-cube offsets and the extracted mesh
+  Buf rm_key, rm_st, rm_votes, rm_ids, rm_toff, rm_coff, rm_V, rm_F;
+  int rm_nV = 0, rm_nF = 0;
   int run_cur = 0, run_V = 0, run_F = 0;
   std::vector<int> run_faces;   // faces after each round of the last run
   int run_handle = -1;          // engine slot of the last pamo_run_stage2
@@ -429,7 +434,7 @@ pamo_ctx::~pamo_ctx() {
                 &qlist, &pairs, &ovf, &seg_lo, &seg_hi, &leaf_face, &lbox_lo, &lbox_hi, &keys0, &keys1, &vals0, &vals1, &hist, &kleft, &kright,
                 &kparent, &rfirst, &rlast, &nodeA, &nodeB, &S, &S_first, &mflag, &mpos, &marked,
                 &scan_desc, &scan_desc_g, &counters, &cstats, &shuf_flags, &shuf_pos, &RV[0], &RV[1], &RF[0], &RF[1], &keep,
-                &kpos, &undo_in};
+                &kpos, &undo_in, &rm_key, &rm_st, &rm_votes, &rm_ids, &rm_toff, &rm_coff, &rm_V, &rm_F};
   for (Buf* b : all)
     if (b->p) (void)hipFree(b->p);
   for (auto& r : prof_open) { ev_pool.push_back(r.a); ev_pool.push_back(r.b); }
@@ -742,6 +747,7 @@ int pamo_ctx::forward(const float* d_pts, const int* d_tris, const int* d_undo, 
 }
 
 #include "pamo_engine.inc"
+#include "pamo_remesh.inc"
 
 static void eng_free(pamo_ctx* c) {
   if (!c->eng) return;
