@@ -155,7 +155,7 @@ struct pamo_ctx {
   Buf RV[2], RF[2], keep, kpos, undo_in;
   // stage 1: distance keys, sample states, triangle votes; extraction ids,
   // per-cube offsets and the extracted mesh
-  Buf rm_key, rm_st, rm_votes, rm_ids, rm_toff, rm_coff, rm_V, rm_F;
+  Buf rm_key, rm_st, rm_votes, rm_ids, rm_toff, rm_coff, rm_code, rm_tab, rm_V, rm_F;
   int rm_nV = 0, rm_nF = 0;
   int run_cur = 0, run_V = 0, run_F = 0;
   std::vector<int> run_faces;   // faces after each round of the last run
@@ -434,7 +434,7 @@ pamo_ctx::~pamo_ctx() {
                 &qlist, &pairs, &ovf, &seg_lo, &seg_hi, &leaf_face, &lbox_lo, &lbox_hi, &keys0, &keys1, &vals0, &vals1, &hist, &kleft, &kright,
                 &kparent, &rfirst, &rlast, &nodeA, &nodeB, &S, &S_first, &mflag, &mpos, &marked,
                 &scan_desc, &scan_desc_g, &counters, &cstats, &shuf_flags, &shuf_pos, &RV[0], &RV[1], &RF[0], &RF[1], &keep,
-                &kpos, &undo_in, &rm_key, &rm_st, &rm_votes, &rm_ids, &rm_toff, &rm_coff, &rm_V, &rm_F};
+                &kpos, &undo_in, &rm_key, &rm_st, &rm_votes, &rm_ids, &rm_toff, &rm_coff, &rm_code, &rm_tab, &rm_V, &rm_F};
   for (Buf* b : all)
     if (b->p) (void)hipFree(b->p);
   for (auto& r : prof_open) { ev_pool.push_back(r.a); ev_pool.push_back(r.b); }

@@ -188,11 +188,27 @@ RM_FN bool seg_add(const McTab& T, int f, int eA, int eB, const long long nu[3],
   next[a] = b;
   return true;
 }
-RM_FN bool cube_segments(const McTab& T, const float val[8], int next[12]) {
-  for (int e = 0; e < 12; e++) next[e] = -1;
+// A cube's code: bit c (0..7) corner c positive; bit 8 + f face f joins its
+// 0-2 diagonal (set only on faces with four crossings).  The segments, and
+// so the triangles, depend on the code alone.
+RM_FN int cube_code(const McTab& T, const float val[8]) {
+  int code = 0;
+  for (int c = 0; c < 8; c++)
+    if (val[c] > 0.0f) code |= 1 << c;
   for (int f = 0; f < 6; f++) {
     bool pos[4];
     for (int q = 0; q < 4; q++) pos[q] = val[T.fc[f][q]] > 0.0f;
+    if (pos[0] == pos[1] || pos[1] == pos[2] || pos[2] == pos[3]) continue;   // not four crossings
+    const float p02 = val[T.fc[f][0]] * val[T.fc[f][2]], p13 = val[T.fc[f][1]] * val[T.fc[f][3]];
+    if (p02 > p13 || (p02 == p13 && pos[0])) code |= 1 << (8 + f);
+  }
+  return code;
+}
+RM_FN bool cube_segments_code(const McTab& T, int code, int next[12]) {
+  for (int e = 0; e < 12; e++) next[e] = -1;
+  for (int f = 0; f < 6; f++) {
+    bool pos[4];
+    for (int q = 0; q < 4; q++) pos[q] = (code >> T.fc[f][q]) & 1;
     int ncross = 0;
     for (int q = 0; q < 4; q++) ncross += pos[q] != pos[(q + 1) % 4];
     if (ncross == 0) continue;
@@ -217,8 +233,7 @@ RM_FN bool cube_segments(const McTab& T, const float val[8], int next[12]) {
       for (int k = 0; k < 3; k++) nu[k] = sp[k] * (12 / np) - sn[k] * (12 / nn);
       if (!seg_add(T, f, es[0], es[1], nu, next)) return false;
     } else {
-      const float p02 = val[T.fc[f][0]] * val[T.fc[f][2]], p13 = val[T.fc[f][1]] * val[T.fc[f][3]];
-      const bool conn02 = p02 > p13 || (p02 == p13 && pos[0]);
+      const bool conn02 = (code >> (8 + f)) & 1;
       const int cuts[2] = {conn02 ? 1 : 0, conn02 ? 3 : 2};
       for (int ci = 0; ci < 2; ci++) {
         const int q = cuts[ci];
@@ -234,6 +249,10 @@ RM_FN bool cube_segments(const McTab& T, const float val[8], int next[12]) {
   }
   return true;
 }
+RM_FN bool cube_segments(const McTab& T, const float val[8], int next[12]) {
+  return cube_segments_code(T, cube_code(T, val), next);
+}
+constexpr int CUBE_CODES = 1 << 14;
 // The cube's triangles.  Every loop of next[] (taken from its lowest edge)
 // is fanned from the first of its edges whose fan has no triangle lying flat
 // in a cube face (all three corners' edges on one face: on a saddle face the

@@ -40,9 +40,11 @@ __global__ void __launch_bounds__(BLOCK) k_rm_udf(const float* __restrict__ tris
     axis_range(fminf(a.y, fminf(b.y, c.y)), fmaxf(a.y, fmaxf(b.y, c.y)), band, R, j0, j1);
     axis_range(fminf(a.z, fminf(b.z, c.z)), fmaxf(a.z, fmaxf(b.z, c.z)), band, R, k0, k1);
     if (i1 < i0 || j1 < j0 || k1 < k0) continue;
-    const long long ni = i1 - i0 + 1, nj = j1 - j0 + 1, n = ni * nj * (k1 - k0 + 1);
-    for (long long q = threadIdx.x; q < n; q += blockDim.x) {
-      const int i = i0 + (int)(q % ni), j = j0 + (int)((q / ni) % nj), k = k0 + (int)(q / (ni * nj));
+    // (a box holds at most R^3 <= 2^27 samples: 32-bit index arithmetic)
+    const unsigned ni = i1 - i0 + 1, nj = j1 - j0 + 1, nij = ni * nj, n = nij * (unsigned)(k1 - k0 + 1);
+    for (unsigned q = threadIdx.x; q < n; q += blockDim.x) {
+      const unsigned qk = q / nij, qr = q - qk * nij, qj = qr / ni;
+      const int i = i0 + (int)(qr - qj * ni), j = j0 + (int)qj, k = k0 + (int)qk;
       const F3 p = sample_pos(i, j, k, R);
       const F3 d = rm::f3sub(p, rm::closest_on_triangle(p, a, b, c));
       const float dist = sqrtf(rm::f3dot(d, d));
@@ -175,44 +177,46 @@ __global__ void k_rm_vpos(const float* __restrict__ vol, int R, const int* __res
   verts[3 * (long long)id + 1] = c[1] / (float)R;
   verts[3 * (long long)id + 2] = c[2] / (float)R;
 }
-// one cube's corner values and its polygonisation; false for an empty cube
-__device__ __forceinline__ int cube_at(const float* __restrict__ vol, int R, long long cell, int& i, int& j, int& k,
-                                       int next[12], rm::CubeOut* out, int* ncent) {
+// Per cube code (remesh.h cube_code), and per code the cube's triangle
+// and centre counts (nt | nc << 8; 0xFFFF for a code whose segments do not
+// close), so the count scans read two bytes per cube.
+constexpr uint16_t RM_BAD_CODE = 0xFFFF;
+__global__ void k_rm_code_table(uint16_t* __restrict__ tab) {
+  const int code = blockIdx.x * blockDim.x + threadIdx.x;
+  if (code >= rm::CUBE_CODES) return;
+  int next[12], nc = 0;
+  if (!rm::cube_segments_code(kMc, code, next)) { tab[code] = RM_BAD_CODE; return; }
+  const int nt = rm::cube_tris(kMc, next, nullptr, &nc);
+  tab[code] = nt < 0 ? RM_BAD_CODE : (uint16_t)(nt | (nc << 8));
+}
+__global__ void k_rm_codes(const float* __restrict__ vol, int R, uint16_t* __restrict__ codes) {
+  const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int Rc = R - 1;
-  i = (int)(cell % Rc);
-  j = (int)((cell / Rc) % Rc);
-  k = (int)(cell / ((long long)Rc * Rc));
+  if (cell >= (long long)Rc * Rc * Rc) return;
+  const int i = (int)(cell % Rc), j = (int)((cell / Rc) % Rc), k = (int)(cell / ((long long)Rc * Rc));
   float val[8];
-  bool any_p = false, any_n = false;
-  for (int c = 0; c < 8; c++) {
+  for (int c = 0; c < 8; c++)
     val[c] = vol[((long long)(k + ((c >> 2) & 1)) * R + (j + ((c >> 1) & 1))) * R + (i + (c & 1))];
-    if (val[c] > 0.0f) any_p = true;
-    else any_n = true;
-  }
-  *ncent = 0;
-  if (!(any_p && any_n)) return 0;
-  if (!rm::cube_segments(kMc, val, next)) return -1;
-  return rm::cube_tris(kMc, next, out, ncent);
+  codes[cell] = (uint16_t)rm::cube_code(kMc, val);
 }
 // per cube: its triangle count (which 0) or centre count (which 1),
-// scanned to the cube's first triangle or centre; a malformed cube counts
-// in err
+// scanned to the cube's first triangle or centre; a cube whose segments do
+// not close counts in err
 struct RmCellOp {
-  const float* vol;
-  int R;
+  const uint16_t* codes;
+  const uint16_t* tab;
   int which;
   int* off;
   int* err;
   struct Rec { int n; };
   __device__ Rec load(long long cell) const {
-    int i, j, k, next[12], nc = 0;
-    const int nt = cube_at(vol, R, cell, i, j, k, next, nullptr, &nc);
+    const uint16_t t = tab[codes[cell]];
     Rec r;
-    if (nt < 0) {
+    if (t == RM_BAD_CODE) {
       if (which == 0) atomicAdd(err, 1);
       r.n = 0;
     } else {
-      r.n = which ? nc : nt;
+      r.n = which ? (t >> 8) : (t & 0xFF);
     }
     return r;
   }
@@ -221,15 +225,20 @@ struct RmCellOp {
   __device__ void emit(long long cell, const Rec&, int pos) const { off[cell] = pos; }
 };
 // the triangles (vertex ids) and centre vertices of every cube
-__global__ void k_rm_tris(const float* __restrict__ vol, int R, const int* __restrict__ ids,
-                          const int* __restrict__ toff, const int* __restrict__ coff, int nedge_verts,
-                          float* __restrict__ verts, int* __restrict__ faces) {
+__global__ void k_rm_tris(const uint16_t* __restrict__ codes, const uint16_t* __restrict__ tab, int R,
+                          const int* __restrict__ ids, const int* __restrict__ toff, const int* __restrict__ coff,
+                          int nedge_verts, float* __restrict__ verts, int* __restrict__ faces) {
   const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (cell >= (long long)(R - 1) * (R - 1) * (R - 1)) return;
-  int i, j, k, next[12], nc = 0;
+  const int Rc = R - 1;
+  if (cell >= (long long)Rc * Rc * Rc) return;
+  const int code = codes[cell];
+  const uint16_t t = tab[code];
+  if (t == RM_BAD_CODE || (t & 0xFF) == 0) return;
+  const int i = (int)(cell % Rc), j = (int)((cell / Rc) % Rc), k = (int)(cell / ((long long)Rc * Rc));
+  int next[12], nc = 0;
   rm::CubeOut out;
-  const int nt = cube_at(vol, R, cell, i, j, k, next, &out, &nc);
-  if (nt <= 0) return;
+  rm::cube_segments_code(kMc, code, next);
+  const int nt = rm::cube_tris(kMc, next, &out, &nc);
   const int c0 = nedge_verts + coff[cell];
   for (int x = 0; x < nc; x++) {   // centre: the mean of its loop's crossings
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
