@@ -249,6 +249,7 @@ def run_plain(args):
     if args.workload == "sdf2m" and not args.no_legs and not args.strict and args.placement == "midpoint":
         result["placement_optimal"] = _placement_leg(args, ctx, dv, df, scale, target, verts, out_v, out_f, sec,
                                                      st["rounds"])
+        result["stage1"] = _stage1_leg(args, v, f, ctx)
     if args.workload == "sdf2m" and not args.no_legs and not args.strict and args.placement == "midpoint":
         del dv, df, ov, of
         ctx.close()
@@ -327,6 +328,61 @@ def _placement_leg(args, ctx, dv, df, scale, target, verts, out_v, out_f, sec_mi
         from pamo_amd.verify import verify
         r = verify(o_v, o_f)
         out["verify"] = {k: r[k] for k in ("manifold", "watertight", "intersecting_pairs", "ok")}
+    return out
+
+
+def _stage1_leg(args, v, f, ctx):
+    """Stage 1 (remeshing, pamo_amd.remesh) of the same input at the
+    reference's lattice (R = 256, band 3/256): the signed distance and the
+    extraction, timed apart, the shell checked closed and oriented; beside
+    it the oracle's restatement on the host at R = 128 against the GPU at
+    R = 128, outputs compared."""
+    import numpy as np
+    import torch
+    from pamo_amd import remesh
+    band = 3.0 / 256
+    margin = 2 * band + 1
+    tris, tmin, tmax, mean = remesh.normalize(v, f, band, margin)
+    dt = torch.from_numpy(tris).to("cuda:0")
+    vm = remesh.VolumeToMesh(ctx)
+
+    def run(R, reps):
+        ts, te = [], []
+        for _ in range(reps + 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            vol = remesh.get_sdf(dt, R, band, ctx=ctx)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            gv, gf = vm(vol - 0.9 / R)
+            torch.cuda.synchronize()
+            ts.append(t1 - t0)
+            te.append(time.perf_counter() - t1)
+        return float(np.mean(ts[1:])), float(np.mean(te[1:])), gv, gf
+
+    s256, e256, gv, gf = run(256, max(args.steps, 1))
+    out = dict(R=256, band=band, input_faces=int(len(f)), sdf_ms=round(1e3 * s256, 3), extract_ms=round(1e3 * e256, 3),
+               value=round(len(f) / (s256 + e256) / 1e6, 3), unit="input Mfaces/s",
+               shell_vertices=int(gv.shape[0]), shell_faces=int(gf.shape[0]))
+    if not args.no_verify:
+        from pamo_amd.verify import verify
+        r = verify(gv.cpu().numpy(), gf.cpu().numpy())
+        out["verify"] = {k: r[k] for k in ("manifold", "watertight", "misoriented_edges", "intersecting_pairs",
+                                           "euler")}
+    if not args.no_cpu:
+        import oracle
+        s128, e128, g128v, g128f = run(128, 1)
+        t0 = time.perf_counter()
+        vol = oracle.mesh_to_sdf(tris, 128, band)
+        ov, of = oracle.volume_to_mesh(vol - np.float32(0.9 / 128))
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = dict(kind="port", cores=1, R=128, seconds=round(tc, 3),
+                                   gpu_ms=round(1e3 * (s128 + e128), 3),
+                                   value=round(len(f) / tc / 1e6, 4), unit="input Mfaces/s",
+                                   outputs_equal=bool(np.array_equal(of, g128f.cpu().numpy()) and
+                                                      np.array_equal(ov.view(np.uint32),
+                                                                     g128v.cpu().numpy().view(np.uint32))),
+                                   sample="the same mesh at R = 128 (the oracle is serial)")
     return out
 
 
