@@ -282,6 +282,51 @@ int pamo_mesh_to_sdf(pamo_ctx* ctx, void* stream, const float* d_tris, int n_tri
 int pamo_volume_to_mesh(pamo_ctx* ctx, void* stream, const float* d_volume, int R, int* n_verts, int* n_faces);
 int pamo_volume_mesh_fetch(pamo_ctx* ctx, void* stream, float* d_verts, int* d_faces);
 
+/* Stage 3 (safe projection): the Stage3System of simp_cuda/safe_project/src/
+ * pamo_safe_project/system.py:17-470 on the device, in float64, one system
+ * per context.  All arrays are device pointers.
+ * pamo_s3_configure replaces Stage3Config (config.py:6-73): cfg_d = {d_hat,
+ * coll_stiffness, contact_detection_radius, ee_classify_thres,
+ * ccd_slackness, ccd_thickness, elas_young_modulus, elas_poisson_ratio,
+ * hinge_stiffness, mesh2gt_dist_stiffness, min_angle_thres}, cfg_i =
+ * {n_newton_iters, n_cg_iters, n_line_search_iters, spd_max_iters,
+ * ccd_max_iters, max_blocks}; either may be NULL for the defaults.
+ * pamo_s3_register replaces register_mesh(gt_V, gt_F, V, F) (system.py:
+ * 210-320): the target mesh (n_gt_verts x 3 doubles, n_gt_faces x 3 ints)
+ * and the mesh to project (closed, consistently oriented: one hinge per
+ * edge, energy.py:626-628; -2 otherwise).  pamo_s3_step replaces n calls of
+ * step() (system.py:407-436).  pamo_s3_get / pamo_s3_set copy an
+ * n_verts x 3 vector: 0 positions (get_vertices, system.py:438-440),
+ * 1 targets, 2 forces (-dE/dx), 3 Hessian diagonal, 4 Newton step,
+ * 5 rest positions, 6 positions at the Newton iteration's start, 7 at the
+ * last contact detection.  pamo_s3_sizes: n_verts, n_faces, n_edges,
+ * n_hinges, n_contacts, do_nothing.  pamo_s3_scalars (host out[5]):
+ * energy, energy before the line search, CCD step, detection radius,
+ * halvings taken.  Returns 0, -4 for invalid arguments, -1 for an error. */
+int pamo_s3_configure(pamo_ctx* ctx, const double* cfg_d, const long long* cfg_i);
+int pamo_s3_register(pamo_ctx* ctx, void* stream, const double* d_gt_verts, int n_gt_verts, const int* d_gt_faces,
+                     int n_gt_faces, const double* d_verts, int n_verts, const int* d_faces, int n_faces);
+int pamo_s3_step(pamo_ctx* ctx, void* stream, int n_steps);
+int pamo_s3_sizes(pamo_ctx* ctx, long long* out);
+int pamo_s3_get(pamo_ctx* ctx, void* stream, int which, double* d_out);
+int pamo_s3_set(pamo_ctx* ctx, void* stream, int which, const double* d_in);
+int pamo_s3_scalars(pamo_ctx* ctx, void* stream, double* out);
+/* Stage-3 parity hooks (the parts of a step, system.py:407-436):
+ * pamo_s3_update_target (_update_vertex_target), pamo_s3_newton_part
+ * (0 contact detection, 1 energy, 2 derivatives, 3 CG solve, 4 step clamp,
+ * 5 CCD, 6 line search, 7 one whole Newton iteration), the energy's four
+ * parts at positions d_x (host parts[4]: target distance, membrane, hinge,
+ * contact), the contact list as rows (type, i0, i1, i2, i3, distance type)
+ * with distances (returns the count; writes when d_rows is not NULL), H dx,
+ * and the topology: edges (n_edges x 2), hinges (x0, x1, x2, x3 per edge),
+ * rest angle and length per hinge, Voronoi area per vertex. */
+int pamo_s3_update_target(pamo_ctx* ctx, void* stream);
+int pamo_s3_newton_part(pamo_ctx* ctx, void* stream, int part);
+int pamo_s3_energy_parts(pamo_ctx* ctx, void* stream, const double* d_x, double* parts);
+long long pamo_s3_contacts(pamo_ctx* ctx, void* stream, int* d_rows, double* d_dist);
+int pamo_s3_hess_dx(pamo_ctx* ctx, void* stream, const double* d_dx, double* d_out);
+int pamo_s3_topology(pamo_ctx* ctx, void* stream, int* d_edges, int* d_hinges, double* d_rest, double* d_vor);
+
 /* Parity hook: every ordered candidate pair (query, current) of one
  * self-intersection check over all alive faces of a mesh, with the
  * reference narrow-phase verdict (simp_cuda/src/bvh/self_intersect.cuh:151-392).

@@ -5,7 +5,8 @@ Drop-in for the reference's ``pamo`` package on its stage-2 path
 points, triangles, ratio, tolerance, threshold, iter, min_verts)`` and the
 extension module ``pamo_amd._C`` (``CUDSP_Free``).  The rounds run in
 libpamo_hip.so (hand-written gfx950 HIP kernels), as does stage 1's
-remeshing (``pamo_amd.remesh``); stage 3 is not part of this package.
+remeshing (``pamo_amd.remesh``) and stage 3's safe projection
+(``pamo_amd.safe_project``).
 """
 import time
 
@@ -108,8 +109,8 @@ def simplify_stage2(verts, faces, scale, target, threshold=1e-3, tolerance=4, it
 
 
 class PaMO(torch.nn.Module):
-    """simp_cuda/pamo/__init__.py:18-178: stage 1 (remeshing, optional) and
-    stage 2; stage 3 is not part of this package.
+    """simp_cuda/pamo/__init__.py:18-178: stage 1 (remeshing, optional),
+    stage 2 and stage 3 (safe projection toward the input mesh, optional).
 
     ``engine="device"`` (default) keeps the whole round loop on the GPU;
     ``engine="rounds"`` runs the reference's Python loop over
@@ -119,8 +120,6 @@ class PaMO(torch.nn.Module):
     def __init__(self, input_mesh, use_stage1=True, use_stage3=True, verbose=True, engine="device",
                  shuffle=_lib.SHUFFLE_FEISTEL, strict=False, placement="midpoint"):
         super().__init__()
-        if use_stage3:
-            raise NotImplementedError("stage 3 (safe projection) is not part of pamo_amd; pass use_stage3=False")
         self.use_stage1 = use_stage1
         self.use_stage3 = use_stage3
         self.verbose = verbose
@@ -128,8 +127,11 @@ class PaMO(torch.nn.Module):
         if verbose:
             print("Stage1 : ", self.use_stage1)
             print("Stage3 : ", self.use_stage3)
-        v, _ = _as_vf(input_mesh)
+        v, f = _as_vf(input_mesh)
         self.bbox = np.stack([v.min(0), v.max(0)])
+        # the target of stage 3 (the reference keeps a copy of the input mesh)
+        self.gt_vertices = np.array(v, np.float64)
+        self.gt_faces = np.array(f, np.int32)
         self.pamo = _C.CUDSP_Free(shuffle)
         if strict and engine != "device":
             raise ValueError("strict=True needs engine='device' (the round kernels follow the reference)")
@@ -146,6 +148,13 @@ class PaMO(torch.nn.Module):
         self.last_stats = None
         self.last_stage2_seconds = None
         self.last_stage1_seconds = None
+        self.last_stage3_seconds = None
+        self.last_stage2_output = None
+        if use_stage3:
+            # one stage-3 system, reused by every run (simp_cuda/pamo/__init__.py:31-33)
+            from . import safe_project
+            self.config = safe_project.Stage3Config()
+            self.system = safe_project.Stage3System(self.config, ctx=self.pamo._ctx)
 
     def run(self, points, triangles, ratio, tolerance=4, threshold=1e-3, iter=1000000, min_verts=10000000000):
         self.target_faces = max(int(ratio * len(triangles)), min_verts)
@@ -186,6 +195,14 @@ class PaMO(torch.nn.Module):
         faces = faces.cpu().numpy()
         if self.verbose:
             print(f"Time for Simplification: {end - start} sec")
+        self.last_stage2_output = (verts, faces)
+        if self.use_stage3:
+            # simp_cuda/pamo/__init__.py:163-176: 5 steps toward the input mesh
+            from . import safe_project
+            start3 = time.time()
+            verts, faces = safe_project.process(self.gt_vertices, self.gt_faces, verts, faces, 5, system=self.system,
+                                                config=self.config)
+            self.last_stage3_seconds = time.time() - start3
         return verts, faces
 
     def _run_rounds(self, verts, faces, scale, tolerance, threshold, iters):

@@ -25,6 +25,7 @@
 #include "pamo_engine_kernels.h"
 #include "pamo_kernels.h"
 #include "remesh_kernels.h"
+#include "stage3_kernels.h"
 
 using namespace pamo;
 
@@ -122,8 +123,10 @@ static void tuning_load_file(Tuning& t, const char* path) {
   }
 }
 
+struct S3State;
 struct pamo_ctx {
   Engine* eng = nullptr;   // batched engine state (pamo_engine.inc)
+  S3State* s3 = nullptr;   // stage-3 system (pamo_stage3.inc)
   int device = 0;
   int shuffle_mode = PAMO_SHUFFLE_FEISTEL;
   hipStream_t st = nullptr;
@@ -411,6 +414,7 @@ struct pamo_ctx {
     return fill_i32(edge_cost.p, 0, (long long)alloc_edge_cost + 1);
   }
 
+  bool build_bvh(int Fn);
   bool build_bvh_and_traverse(int Fn, bool debug_pairs, std::vector<int>* pairs_out);
   bool node_boxes(const int* n_alive, const int* sorted_leaf, long long Fl);
   bool check_and_mark(int* n_marked);
@@ -420,15 +424,18 @@ struct pamo_ctx {
 };
 
 static void eng_free(pamo_ctx* c);
+static void s3_free(pamo_ctx* c);
 
 pamo_ctx::~pamo_ctx() {
   (void)hipSetDevice(device);
   if (arena) {   // no-allocation mode: the region is not ours to free
     eng_free(this);
+    s3_free(this);
     return;
   }
   (void)hipDeviceSynchronize();
   eng_free(this);
+  s3_free(this);
   Buf* all[] = {&pts_map, &table, &edge_cost, &undo_list, &tmp_undo, &P, &P0, &occ, &T, &T0, &cnt, &fan_off, &fan,
                 &ecnt, &eoff, &edges, &Q, &tri_min, &win, &winpos, &collapsed, &qflag, &qf, &qpos, &af, &apos,
                 &qlist, &pairs, &ovf, &seg_lo, &seg_hi, &leaf_face, &lbox_lo, &lbox_hi, &keys0, &keys1, &vals0, &vals1, &hist, &kleft, &kright,
@@ -461,9 +468,9 @@ bool pamo_ctx::node_boxes(const int* n_alive, const int* sorted_leaf, long long 
   return check_launch("node boxes");
 }
 
-// BVH over the alive faces, then the restricted traversal (query list in
-// qlist/C_NQUERY) or, for the debug export, the full pair list.
-bool pamo_ctx::build_bvh_and_traverse(int Fn, bool debug_pairs, std::vector<int>* pairs_out) {
+// LBVH over the faces leaf_face[0..n) (n at C_NALIVE, at most Fn) of P/T:
+// nodes in nodeA/nodeB.
+bool pamo_ctx::build_bvh(int Fn) {
   const long long Fl = std::max(Fn, 1);
   if (!ensure(lbox_lo, Fl * 16) || !ensure(lbox_hi, Fl * 16) || !ensure(keys0, Fl * 4) || !ensure(keys1, Fl * 4) ||
       !ensure(vals0, Fl * 4) || !ensure(vals1, Fl * 4) || !ensure(kleft, Fl * 4) || !ensure(kright, Fl * 4) ||
@@ -502,7 +509,15 @@ bool pamo_ctx::build_bvh_and_traverse(int Fn, bool debug_pairs, std::vector<int>
                                       ptr<int>(rfirst), ptr<int>(rlast));
   if (!node_boxes(n_alive, vin, Fl)) return false;
   prof_end(pbv);
-  if (!check_launch("lbvh")) return false;
+  return check_launch("lbvh");
+}
+
+// BVH over the alive faces, then the restricted traversal (query list in
+// qlist/C_NQUERY) or, for the debug export, the full pair list.
+bool pamo_ctx::build_bvh_and_traverse(int Fn, bool debug_pairs, std::vector<int>* pairs_out) {
+  const long long Fl = std::max(Fn, 1);
+  if (!build_bvh(Fn)) return false;
+  int* n_alive = cnt_dev(C_NALIVE);
   if (!debug_pairs) {
     int pt = prof_begin(PAMO_K_TRAVERSE, 0.0);
     if (!fill_u8(S.p, 0, Fl)) return false;
@@ -748,6 +763,7 @@ int pamo_ctx::forward(const float* d_pts, const int* d_tris, const int* d_undo, 
 
 #include "pamo_engine.inc"
 #include "pamo_remesh.inc"
+#include "pamo_stage3.inc"
 
 static void eng_free(pamo_ctx* c) {
   if (!c->eng) return;

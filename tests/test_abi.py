@@ -47,15 +47,19 @@ def test_input_checks_match_reference_messages():
         _check_input(torch.zeros(3, 3), "points")
 
 
-def test_stage_flags_raise():
+def test_pamo_without_gpu_fails_loudly():
+    """Every stage runs on the device: without a GPU, PaMO refuses to build
+    instead of falling back to the host."""
     import numpy as np
+    import torch
     from pamo_amd import PaMO
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
     v = np.zeros((3, 3), np.float32)
     f = np.array([[0, 1, 2]], np.int32)
-    with pytest.raises(NotImplementedError):
-        PaMO((v, f))
-    with pytest.raises(NotImplementedError):
-        PaMO((v, f), use_stage1=False)
+    for kw in ({}, {"use_stage1": False}, {"use_stage1": False, "use_stage3": False}):
+        with pytest.raises(RuntimeError):
+            PaMO((v, f), verbose=False, **kw)
 
 
 def test_bench_refuses_multi_gpu():
