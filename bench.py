@@ -52,7 +52,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", default="sdf2m", choices=["sdf2m", "sdf7m", "ico20k", "batch512", "serve"])
+    p.add_argument("--workload", default="sdf2m", choices=["sdf2m", "sdf7m", "ico20k", "batch512", "serve", "stage3"])
     p.add_argument("--meshes", type=int, default=512, help="batch512: number of meshes")
     p.add_argument("--jobs", type=int, default=1000, help="serve: number of jobs")
     p.add_argument("--rate", default="200", help="serve: Poisson arrival rate(s) in jobs/s, comma-separated")
@@ -91,7 +91,7 @@ def parse():
 def make_mesh(args):
     from pamo_amd import synth
     t = time.time()
-    if args.workload == "sdf2m":
+    if args.workload in ("sdf2m", "stage3"):
         grid = args.grid or 300
         v, f = synth.sdf_union(200, grid, seed=2, k=0.02)
         cfg = dict(workload="sdf_union_2M", generator="smooth-min union of 200 spheres (k=0.02), seed 2",
@@ -121,6 +121,8 @@ def main():
         return run_batch(args)
     if args.workload == "serve":
         return run_serve(args)
+    if args.workload == "stage3":
+        return run_stage3(args)
     return run_plain(args)
 
 
@@ -250,6 +252,7 @@ def run_plain(args):
         result["placement_optimal"] = _placement_leg(args, ctx, dv, df, scale, target, verts, out_v, out_f, sec,
                                                      st["rounds"])
         result["stage1"] = _stage1_leg(args, v, f, ctx)
+        result["stage3"] = _stage3_leg(args, v, f, out_v + tris_mean, out_f, ctx)
     if args.workload == "sdf2m" and not args.no_legs and not args.strict and args.placement == "midpoint":
         del dv, df, ov, of
         ctx.close()
@@ -384,6 +387,87 @@ def _stage1_leg(args, v, f, ctx):
                                                                      g128v.cpu().numpy().view(np.uint32))),
                                    sample="the same mesh at R = 128 (the oracle is serial)")
     return out
+
+
+def _stage3_leg(args, v, f, s2v, s2f, ctx):
+    """Stage 3 (safe projection, pamo_amd.safe_project) as PaMO runs it after
+    stage 2 (simp_cuda/pamo/__init__.py:160-176): 5 steps moving the stage-2
+    output toward the input mesh, one system reused across calls.  Timed:
+    registration (topology, rest state, the target's LBVH) and the steps;
+    beside it the float64 oracle on the host (all cores) on the same call,
+    outputs compared."""
+    import numpy as np
+    import torch
+    from pamo_amd import safe_project as s3
+    system = s3.Stage3System(ctx=ctx)
+    s, t = s3.get_normalization_transform(v)
+    gv = np.asarray(v, np.float64) * s + t
+    mv = np.asarray(s2v, np.float64) * s + t
+    dgv, dgf = torch.from_numpy(gv).cuda(), torch.from_numpy(np.asarray(f, np.int32)).cuda()
+    dmv, dmf = torch.from_numpy(mv).cuda(), torch.from_numpy(np.asarray(s2f, np.int32)).cuda()
+    reg, stp = [], []
+    for _ in range(max(args.steps, 1) + max(args.warmup, 1)):
+        system.clear()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        system.register_mesh(dgv, dgf, dmv, dmf)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        system.step(5)
+        torch.cuda.synchronize()
+        reg.append(t1 - t0)
+        stp.append(time.perf_counter() - t1)
+    out = (system.get_vertices() - t) / s
+    sz = system.sizes()
+    w = max(args.warmup, 1)
+    reg, stp = reg[w:], stp[w:]
+    sec = float(np.mean(reg)) + float(np.mean(stp))
+    res = dict(steps=5, newton_per_step=10, target_faces=int(len(f)), mesh_vertices=int(sz["nV"]),
+               mesh_faces=int(sz["nF"]), contacts_last=int(sz["contacts"]), dtype="f64",
+               register_ms=round(1e3 * float(np.mean(reg)), 3), steps_ms=round(1e3 * float(np.mean(stp)), 3),
+               ms=round(1e3 * sec, 3), value=round(len(s2f) / sec / 1e3, 3), unit="stage-2 kfaces/s")
+    if not args.no_cpu:
+        import oracle
+        from oracle import stage3 as o3
+        ncores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+        oracle.set_threads(ncores)
+        log("stage 3: oracle process on %d threads" % ncores)
+        t0 = time.perf_counter()
+        ov, _ = o3.process(v, f, s2v, s2f, 5)
+        tc = time.perf_counter() - t0
+        ext = float(np.max(np.max(v, 0) - np.min(v, 0)))
+        res["cpu_baseline"] = dict(kind="port", cores=ncores, seconds=round(tc, 3),
+                                   value=round(len(s2f) / tc / 1e3, 4), unit="stage-2 kfaces/s",
+                                   max_err_over_extent=float(np.abs(out - ov).max() / ext),
+                                   sample="the same call: 5 steps on the stage-2 output (the oracle finds "
+                                          "nearest target points by scanning every target triangle)")
+    return res
+
+
+def run_stage3(args):
+    """Stage 3 alone: the 2.0M-face SDF union through stage 2 (untimed), then
+    the safe projection of its output toward the input, timed over --steps
+    calls after --warmup (the stage3 leg of the default line)."""
+    import torch
+    from pamo_amd import _lib, preprocess_numpy, simplify_stage2
+    v, f, cfg = make_mesh(args)
+    verts, scale, tris_mean = preprocess_numpy(v, f)
+    ctx = _lib.Context(0)
+    dv, df = torch.from_numpy(verts).cuda(), torch.from_numpy(f).cuda()
+    ov, of, st = simplify_stage2(dv, df, scale, int(cfg["ratio"] * len(f)), ctx=ctx)
+    s2v, s2f = ov.cpu().numpy() + tris_mean, of.cpu().numpy()
+    log("stage 2: %d -> %d faces" % (len(f), len(s2f)))
+    del dv, df, ov, of
+    leg = _stage3_leg(args, v, f, s2v, s2f, ctx)
+    result = dict(metric="stage-3 safe projection of the stage-2 output (5 steps, PaMO's call)",
+                  value=leg.pop("value"), unit=leg.pop("unit"), n_gpus=1, steps=args.steps, warmup=args.warmup,
+                  ms_per_step=leg["ms"], higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64",
+                  data="synthetic (seeded SDF mesh generated in-process)",
+                  config=dict(workload="stage3_sdf_union_2M", input_faces=int(len(f)), stage2_faces=int(len(s2f))),
+                  stage3=leg)
+    if "cpu_baseline" in leg:
+        result["cpu_baseline"] = leg.pop("cpu_baseline")
+    print(json.dumps(result), flush=True)
 
 
 def _plain_each(meshes, ctx, with_prep):

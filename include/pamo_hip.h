@@ -241,7 +241,12 @@ int pamo_set_strict(pamo_ctx* ctx, int strict);
  * "bf_split_max_faces", "walk8_max", "walk4_max", "emit_groups", "pipeline"
  * (1: the next round is queued before this one's result is read; 0: never),
  * "graph" and "graph_max_faces" (rounds of one mesh of at most that many
- * faces are recorded into hipGraphs and replayed; 0: never).  A context
+ * faces are recorded into hipGraphs and replayed; 0: never), and for
+ * stage 3 "s3_bf_max_faces" (meshes up to this many faces search contacts
+ * exhaustively, larger ones through LBVHs), "s3_cg_one_block" (meshes up
+ * to this many vertices run each CG step in one workgroup) and "s3_graph"
+ * (each Newton iteration after its contact detection is recorded into a
+ * hipGraph and replayed; 0: never).  A context
  * starts from the defaults, then from the flat JSON file named by the
  * environment variable PAMO_TUNING_FILE, if set (python -m pamo_amd.tune
  * writes one).  Returns 0, or -1 for an unknown name or a value out of

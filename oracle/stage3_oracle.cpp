@@ -733,7 +733,7 @@ struct System {
       for (int i0 = 0; i0 < nV; i0++) {
         V x0 = q[i0];
         auto endp = std::upper_bound(fxs.begin(), fxs.end(), std::make_pair(x0.x + pad, INT32_MAX));
-        std::vector<Contact> out;
+        std::vector<std::pair<int, Contact>> hit;   // (triangle, contact)
         for (auto it = fxs.begin(); it != endp; ++it) {
           int t = it->second;
           if (fxh[t] < x0.x - pad) continue;
@@ -741,11 +741,12 @@ struct System {
           if (i0 == i1 || i0 == i2 || i0 == i3) continue;
           int ty = pt_classify(x0, q[i1], q[i2], q[i3]);
           double d = pt_distance(x0, q[i1], q[i2], q[i3], ty);
-          if (d <= thr) out.push_back(Contact{BT_PT, {i0, i1, i2, i3}, ty, d, {}});
+          if (d <= thr) hit.push_back({t, Contact{BT_PT, {i0, i1, i2, i3}, ty, d, {}}});
         }
-        std::sort(out.begin(), out.end(), [&](const Contact& a, const Contact& b) {
-          return std::array<int, 3>{a.i[1], a.i[2], a.i[3]} < std::array<int, 3>{b.i[1], b.i[2], b.i[3]};
-        });
+        std::sort(hit.begin(), hit.end(),
+                  [](const std::pair<int, Contact>& a, const std::pair<int, Contact>& b) { return a.first < b.first; });
+        std::vector<Contact> out;
+        for (auto& h : hit) out.push_back(h.second);
         per[i0] = out;
       }
 #pragma omp parallel for schedule(dynamic, 32)

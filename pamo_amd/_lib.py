@@ -93,7 +93,8 @@ def lib():
 # launch shapes of the engine (pamo_set_tuning); python -m pamo_amd.tune
 # writes the measured choice to TUNING_FILE, which every Context applies
 TUNING_NAMES = ("ec_lanes", "morton_bits", "bf_max_faces", "bf_split_max_faces", "walk8_max", "walk4_max",
-                "emit_groups", "pipeline", "graph", "graph_max_faces")
+                "emit_groups", "pipeline", "graph", "graph_max_faces", "s3_bf_max_faces", "s3_cg_one_block",
+                "s3_graph")
 TUNING_FILE = os.path.join(_HERE, "tuning.json")
 
 

@@ -76,6 +76,9 @@ struct Tuning {
   long long pipeline = 1;               // enqueue the next round before reading this one's result (0: never)
   long long graph = 1;                  // record rounds into hipGraphs and replay them (one mesh; 0: never) ...
   long long graph_max_faces = 32768;    // ... up to this many faces
+  long long s3_bf_max_faces = 16384;    // stage 3: exhaustive contact search up to this many faces, else LBVHs
+  long long s3_cg_one_block = 32768;    // stage 3: one-workgroup CG steps up to this many vertices
+  long long s3_graph = 1;               // stage 3: record each Newton iteration's tail into a hipGraph (0: never)
 };
 struct TuningField { const char* name; long long Tuning::*field; long long lo, hi; };
 static const TuningField kTuningFields[] = {
@@ -89,6 +92,9 @@ static const TuningField kTuningFields[] = {
     {"pipeline", &Tuning::pipeline, 0, 1},
     {"graph", &Tuning::graph, 0, 1},
     {"graph_max_faces", &Tuning::graph_max_faces, 0, 1LL << 40},
+    {"s3_bf_max_faces", &Tuning::s3_bf_max_faces, 0, 1LL << 40},
+    {"s3_cg_one_block", &Tuning::s3_cg_one_block, 0, 1LL << 40},
+    {"s3_graph", &Tuning::s3_graph, 0, 1},
 };
 static bool tuning_set(Tuning& t, const char* name, long long v) {
   for (const TuningField& f : kTuningFields)

@@ -13,10 +13,11 @@
 //
 // The reference computes in float32 with float atomics; here every
 // quantity is float64 and every sum has a fixed order:
-//  * per-vertex sums (forces, diagonal, H dx, Voronoi areas) are gathers
-//    over a vertex -> element-slot list sorted by element: the vertex's own
-//    term first, then its faces, hinges and contacts, each in ascending
-//    element order;
+//  * per-vertex sums (forces, diagonal, H dx) are gathers over a vertex ->
+//    element-slot list sorted by element (its faces, hinges and contacts,
+//    each in ascending order): 32 lanes add every 32nd entry in order, a
+//    butterfly adds the lanes, then the vertex's own term is added; Voronoi
+//    areas are serial sums over the vertex's faces in ascending order;
 //  * global sums (energies, CG dot products) are taken by one workgroup:
 //    lane t sums elements t, t + 1024, ... in order, then a fixed tree.
 // Contacts are listed point-triangle pairs first, by (vertex, triangle
@@ -504,65 +505,12 @@ __device__ void hinge_diff(D3 x0, D3 x1, D3 x2, D3 x3, double rest, double elen,
         for (int c = 0; c < 3; c++) H[12 * (3 * i + a) + 3 * j + c] = b[i][j].m[3 * a + c];
 }
 
-// The hinge Hessian projected on the PSD cone (wp.spd_project_blocks of the
-// authors' Warp fork, utils_kernels.py:20-41, which is not in the
-// reference): the leading 9x9 (vertices x0, x1, x2) by it_max cyclic Jacobi
-// sweeps (Golub and Van Loan, 8.5.2), negative eigenvalues set to zero, then
-// row and column 3 rebuilt from zero block-row sums.  A and Q are the
-// caller's 81-entry scratch arrays.
-__device__ void spd_project(double* __restrict__ H, int it_max, double* __restrict__ A, double* __restrict__ Q) {
-  for (int i = 0; i < 9; i++)
-    for (int j = 0; j < 9; j++) {
-      A[9 * i + j] = H[12 * i + j];
-      Q[9 * i + j] = i == j ? 1.0 : 0.0;
-    }
-  for (int sweep = 0; sweep < it_max; sweep++)
-    for (int p = 0; p < 8; p++)
-      for (int q = p + 1; q < 9; q++) {
-        const double apq = A[9 * p + q];
-        if (apq == 0.0) continue;
-        const double tau = (A[9 * q + q] - A[9 * p + p]) / (2.0 * apq);
-        const double t = tau >= 0.0 ? 1.0 / (tau + sqrt(1.0 + tau * tau)) : -1.0 / (-tau + sqrt(1.0 + tau * tau));
-        const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
-        for (int k = 0; k < 9; k++) {   // A <- A J
-          const double akp = A[9 * k + p], akq = A[9 * k + q];
-          A[9 * k + p] = c * akp - s * akq;
-          A[9 * k + q] = s * akp + c * akq;
-        }
-        for (int k = 0; k < 9; k++) {   // A <- J^T A
-          const double apk = A[9 * p + k], aqk = A[9 * q + k];
-          A[9 * p + k] = c * apk - s * aqk;
-          A[9 * q + k] = s * apk + c * aqk;
-        }
-        for (int k = 0; k < 9; k++) {   // Q <- Q J
-          const double qkp = Q[9 * k + p], qkq = Q[9 * k + q];
-          Q[9 * k + p] = c * qkp - s * qkq;
-          Q[9 * k + q] = s * qkp + c * qkq;
-        }
-      }
-  double lam[9];
-  for (int i = 0; i < 9; i++) lam[i] = A[9 * i + i] > 0.0 ? A[9 * i + i] : 0.0;
-  for (int i = 0; i < 9; i++)
-    for (int j = 0; j < 9; j++) {
-      double s = 0.0;
-      for (int k = 0; k < 9; k++) s += Q[9 * i + k] * lam[k] * Q[9 * j + k];
-      H[12 * i + j] = s;
-    }
-  // block row 3: -(blocks 0..2 of each block column); block column 3 its transpose
-  for (int j = 0; j < 9; j++)
-    for (int a = 0; a < 3; a++) {
-      const double s = H[12 * (0 + a) + j] + H[12 * (3 + a) + j] + H[12 * (6 + a) + j];
-      H[12 * (9 + a) + j] = -1.0 * s;
-    }
-  for (int j = 0; j < 3; j++)
-    for (int a = 0; a < 3; a++)
-      for (int c = 0; c < 3; c++) H[12 * (3 * j + a) + 9 + c] = H[12 * (9 + c) + 3 * j + a];
-  for (int a = 0; a < 3; a++)
-    for (int c = 0; c < 3; c++) {
-      const double s = H[12 * a + 9 + c] + H[12 * (3 + a) + 9 + c] + H[12 * (6 + a) + 9 + c];
-      H[12 * (9 + a) + 9 + c] = -1.0 * s;
-    }
-}
+// The hinge Hessian is projected on the PSD cone by k3_spd below
+// (wp.spd_project_blocks of the authors' Warp fork, utils_kernels.py:20-41,
+// which is not in the reference): the leading 9x9 (vertices x0, x1, x2) by
+// it_max cyclic Jacobi sweeps (Golub and Van Loan, 8.5.2), negative
+// eigenvalues set to zero, then block row and column 3 rebuilt from zero
+// block-row sums.
 
 // -------------------------------------------------------------- ACCD
 // Additive CCD of one contact along the search direction v
@@ -643,20 +591,16 @@ __global__ void k3_corner_fill(const int* __restrict__ idx, long long n, int K, 
   const int v = idx[i];
   list[off[v] + atomicAdd(&cur[v], 1)] = (int)i;
 }
-// segments are short (a vertex's faces, hinges or contacts): insertion sort
-__global__ void k3_seg_sort(const int* __restrict__ off, int n, int* __restrict__ list) {
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= n) return;
-  const int a = off[v], b = off[v + 1];
-  for (int i = a + 1; i < b; i++) {
-    const int x = list[i];
-    int j = i - 1;
-    while (j >= a && list[j] > x) {
-      list[j + 1] = list[j];
-      j--;
-    }
-    list[j + 1] = x;
-  }
+// Each entry's place in its vertex's segment is its rank there (entries are
+// distinct slot numbers): tmp (filled in any order) -> list (ascending).
+__global__ void k3_rank_place(const int* __restrict__ idx, long long m, const int* __restrict__ off,
+                              const int* __restrict__ tmp, int* __restrict__ list) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int s = tmp[i], v = idx[s], a = off[v], b = off[v + 1];
+  int r = 0;
+  for (int j = a; j < b; j++) r += tmp[j] < s;
+  list[a + r] = s;
 }
 
 // ------------------------------------------------------- registration
@@ -837,10 +781,37 @@ __global__ void k3_nearest(int nV, const double* __restrict__ q, const double* _
   if (i >= nV) return;
   const D3 x = ld3(q, i);
   const double mg = (double)box_margin(sc, SC_GMAXABS);
+  auto lbound = [&](float4 A, float4 B) {
+    const double dx = fmax(fmax((double)A.x - mg - x.x, x.x - ((double)B.x + mg)), 0.0);
+    const double dy = fmax(fmax((double)A.y - mg - x.y, x.y - ((double)B.y + mg)), 0.0);
+    const double dz = fmax(fmax((double)A.z - mg - x.z, x.z - ((double)B.z + mg)), 0.0);
+    return dx * dx + dy * dy + dz * dz;
+  };
+  auto leaf = [&](int t) {
+    return closest_on_tri(x, ld3(gV, gF[3 * t]), ld3(gV, gF[3 * t + 1]), ld3(gV, gF[3 * t + 2]));
+  };
+  // a first bound: descend into the nearer child down to one leaf (the right
+  // child of a node is the escape of its left child)
+  int node = 0, t0 = -1;
+  for (int depth = 0; depth <= n_nodes; depth++) {
+    const int c = __float_as_int(nodeB[node].w);
+    if (c < 0) {
+      t0 = ~c;
+      break;
+    }
+    const float4 Al = nodeA[c], Bl = nodeB[c];
+    const int r = __float_as_int(Al.w);
+    node = lbound(Al, Bl) <= lbound(nodeA[r], nodeB[r]) ? c : r;
+  }
   double best = INFINITY;
   int bestf = -1;
   D3 bp = x;
-  int node = 0;
+  if (t0 >= 0) {
+    bp = leaf(t0);
+    best = len2(bp - x);
+    bestf = t0;
+  }
+  node = 0;
   long long steps = 0;
   while (node != -1) {
     if (++steps > 2LL * n_nodes + 8) {
@@ -848,19 +819,15 @@ __global__ void k3_nearest(int nV, const double* __restrict__ q, const double* _
       break;
     }
     const float4 A = nodeA[node], B = nodeB[node];
-    const double dx = fmax(fmax((double)A.x - mg - x.x, x.x - ((double)B.x + mg)), 0.0);
-    const double dy = fmax(fmax((double)A.y - mg - x.y, x.y - ((double)B.y + mg)), 0.0);
-    const double dz = fmax(fmax((double)A.z - mg - x.z, x.z - ((double)B.z + mg)), 0.0);
-    const double lb = dx * dx + dy * dy + dz * dz;
     const int esc = __float_as_int(A.w);
-    if (lb <= best) {
+    if (lbound(A, B) <= best) {
       const int c = __float_as_int(B.w);
       if (c >= 0) {
         node = c;
         continue;
       }
       const int t = ~c;
-      const D3 cp = closest_on_tri(x, ld3(gV, gF[3 * t]), ld3(gV, gF[3 * t + 1]), ld3(gV, gF[3 * t + 2]));
+      const D3 cp = leaf(t);
       const double d = len2(cp - x);
       if (d < best || (d == best && t < bestf)) {
         best = d;
@@ -874,16 +841,40 @@ __global__ void k3_nearest(int nV, const double* __restrict__ q, const double* _
 }
 
 // ----------------------------------------------------------- contacts
-// Point-triangle pairs of vertex i0 within thr (contact_detection.py:
+// one query's pairs [base, base + n) sorted by key ce1 (insertion sort)
+__device__ void sort_run(int base, int n, int4* __restrict__ cidx, int2* __restrict__ ctyp, double* __restrict__ cd,
+                         int* __restrict__ ce1) {
+  for (int a = base + 1; a < base + n; a++) {
+    const int4 k = cidx[a];
+    const int2 kt = ctyp[a];
+    const double kd = cd[a];
+    const int ke = ce1[a];
+    int b = a - 1;
+    while (b >= base && ce1[b] > ke) {
+      cidx[b + 1] = cidx[b];
+      ctyp[b + 1] = ctyp[b];
+      cd[b + 1] = cd[b];
+      ce1[b + 1] = ce1[b];
+      b--;
+    }
+    cidx[b + 1] = k;
+    ctyp[b + 1] = kt;
+    cd[b + 1] = kd;
+    ce1[b + 1] = ke;
+  }
+}
+
+// Contact search through LBVHs (meshes above the exhaustive search's
+// limit).  Point-triangle pairs of vertex i0 within thr (contact_detection.py:
 // 120-216): the LBVH of the faces is walked with i0's box grown by thr and
 // the rounding margin; faces with i0 as a corner are skipped.  Count pass
 // (off == nullptr) writes cnt[i0]; emit pass writes the pairs at off[i0],
-// then sorts them by the face's corners.
+// then sorts them by triangle (kept in ce1).
 __global__ void k3_pt_query(int nV, const double* __restrict__ q, const int* __restrict__ F,
                             const float4* __restrict__ nodeA, const float4* __restrict__ nodeB, int n_nodes,
                             double thr, const double* __restrict__ sc, int* __restrict__ cnt,
                             const int* __restrict__ off, int4* __restrict__ cidx, int2* __restrict__ ctyp,
-                            double* __restrict__ cd, int* __restrict__ err) {
+                            double* __restrict__ cd, int* __restrict__ ce1, int* __restrict__ err) {
   const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
   if (i0 >= nV) return;
   const D3 x0 = ld3(q, i0);
@@ -917,6 +908,7 @@ __global__ void k3_pt_query(int nV, const double* __restrict__ q, const int* __r
             cidx[base + n] = make_int4(i0, i1, i2, i3);
             ctyp[base + n] = make_int2(BT_PT, ty);
             cd[base + n] = d;
+            ce1[base + n] = t;
           }
           n++;
         }
@@ -928,25 +920,7 @@ __global__ void k3_pt_query(int nV, const double* __restrict__ q, const int* __r
     cnt[i0] = n;
     return;
   }
-  // by (i1, i2, i3)
-  for (int a = base + 1; a < base + n; a++) {
-    const int4 k = cidx[a];
-    const int2 kt = ctyp[a];
-    const double kd = cd[a];
-    int b = a - 1;
-    while (b >= base) {
-      const int4 o = cidx[b];
-      const bool gt = o.y != k.y ? o.y > k.y : (o.z != k.z ? o.z > k.z : o.w > k.w);
-      if (!gt) break;
-      cidx[b + 1] = o;
-      ctyp[b + 1] = ctyp[b];
-      cd[b + 1] = cd[b];
-      b--;
-    }
-    cidx[b + 1] = k;
-    ctyp[b + 1] = kt;
-    cd[b + 1] = kd;
-  }
+  sort_run(base, n, cidx, ctyp, cd, ce1);
 }
 
 // Edge-edge pairs of edge e0 with the later edges e1 > e0 that share no
@@ -1007,38 +981,153 @@ __global__ void k3_ee_query(int nE, int nV, const double* __restrict__ q, const 
     cnt[nV + e0] = n;
     return;
   }
-  for (int a = base + 1; a < base + n; a++) {
-    const int4 k = cidx[a];
-    const int2 kt = ctyp[a];
-    const double kd = cd[a];
-    const int ke = ce1[a];
-    int b = a - 1;
-    while (b >= base && ce1[b] > ke) {
-      cidx[b + 1] = cidx[b];
-      ctyp[b + 1] = ctyp[b];
-      cd[b + 1] = cd[b];
-      ce1[b + 1] = ce1[b];
-      b--;
-    }
-    cidx[b + 1] = k;
-    ctyp[b + 1] = kt;
-    cd[b + 1] = kd;
-    ce1[b + 1] = ke;
+  sort_run(base, n, cidx, ctyp, cd, ce1);
+}
+
+// Exhaustive contact search (meshes up to the limit), in three passes:
+//  1. box filter: every query against every candidate, the candidates split
+//     into P ranges (part = blockIdx.y) and staged through LDS in tiles of
+//     float boxes rounded outward from the float64 corners (no margin
+//     needed); counted, then emitted as (query, candidate) pairs at scanned
+//     offsets laid out query-major (q P + part), so each query's pairs are
+//     in ascending candidate order;
+//  2. narrow phase: one lane per pair (no lane waits on another's
+//     classification);
+//  3. compaction of the pairs within thr, keeping that order.
+constexpr int S3_TILE = 256;
+__global__ void k3_boxes(int nF, int nE, const double* __restrict__ q, const int* __restrict__ F,
+                         const int* __restrict__ E, float4* __restrict__ flo, float4* __restrict__ fhi,
+                         float4* __restrict__ elo, float4* __restrict__ ehi) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nF + nE) return;
+  D3 a, b, c;
+  if (i < nF) {
+    a = ld3(q, F[3 * i]); b = ld3(q, F[3 * i + 1]); c = ld3(q, F[3 * i + 2]);
+  } else {
+    a = ld3(q, E[2 * (i - nF)]); b = ld3(q, E[2 * (i - nF) + 1]); c = b;
   }
+  const float4 lo = make_float4(__double2float_rd(fmin(a.x, fmin(b.x, c.x))), __double2float_rd(fmin(a.y, fmin(b.y, c.y))),
+                                __double2float_rd(fmin(a.z, fmin(b.z, c.z))), 0.0f);
+  const float4 hi = make_float4(__double2float_ru(fmax(a.x, fmax(b.x, c.x))), __double2float_ru(fmax(a.y, fmax(b.y, c.y))),
+                                __double2float_ru(fmax(a.z, fmax(b.z, c.z))), 0.0f);
+  if (i < nF) {
+    flo[i] = lo;
+    fhi[i] = hi;
+  } else {
+    elo[i - nF] = lo;
+    ehi[i - nF] = hi;
+  }
+}
+__device__ __forceinline__ bool box_meets(float4 lo, float4 hi, const float qlo[3], const float qhi[3]) {
+  return !(qhi[0] < lo.x || hi.x < qlo[0]) && !(qhi[1] < lo.y || hi.y < qlo[1]) && !(qhi[2] < lo.z || hi.z < qlo[2]);
+}
+// Pass 1.  EE (kind 1): query e0 against edges e1 > e0 sharing no vertex;
+// PT (kind 0): query vertex i0 against faces without i0.
+template <int KIND, bool EMIT>
+__global__ void __launch_bounds__(S3_TILE) k3_box_pairs(int nq, int ncand, int chunk, int P, const double* __restrict__ q,
+                                                       const int* __restrict__ F, const int* __restrict__ E,
+                                                       const float4* __restrict__ clo, const float4* __restrict__ chi,
+                                                       double thr, int* __restrict__ cnt, const int* __restrict__ off,
+                                                       int2* __restrict__ pairs) {
+  __shared__ float4 slo[S3_TILE], shi[S3_TILE];
+  const int part = blockIdx.y;
+  const int qi = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = qi < nq;
+  int i0 = 0, i1 = 0;
+  D3 a = mk3(0.0, 0.0, 0.0), b = a;
+  if (live) {
+    if (KIND == 0) {
+      i0 = qi;
+      a = b = ld3(q, i0);
+    } else {
+      i0 = E[2 * qi];
+      i1 = E[2 * qi + 1];
+      a = ld3(q, i0);
+      b = ld3(q, i1);
+    }
+  }
+  const float qlo[3] = {__double2float_rd(fmin(a.x, b.x) - thr), __double2float_rd(fmin(a.y, b.y) - thr),
+                        __double2float_rd(fmin(a.z, b.z) - thr)};
+  const float qhi[3] = {__double2float_ru(fmax(a.x, b.x) + thr), __double2float_ru(fmax(a.y, b.y) + thr),
+                        __double2float_ru(fmax(a.z, b.z) + thr)};
+  const int t0 = part * chunk, t1 = min(ncand, t0 + chunk);
+  const int base = EMIT && live ? off[(long long)qi * P + part] : 0;
+  int n = 0;
+  for (int tb = t0; tb < t1; tb += S3_TILE) {
+    const int m = min(S3_TILE, t1 - tb);
+    __syncthreads();
+    for (int j = threadIdx.x; j < m; j += blockDim.x) {
+      slo[j] = clo[tb + j];
+      shi[j] = chi[tb + j];
+    }
+    __syncthreads();
+    if (!live) continue;
+    for (int j = KIND == 1 ? max(0, qi + 1 - tb) : 0; j < m; j++) {
+      if (!box_meets(slo[j], shi[j], qlo, qhi)) continue;
+      const int t = tb + j;
+      if (KIND == 0) {
+        if (F[3 * t] == i0 || F[3 * t + 1] == i0 || F[3 * t + 2] == i0) continue;
+      } else {
+        const int i2 = E[2 * t], i3 = E[2 * t + 1];
+        if (i0 == i2 || i0 == i3 || i1 == i2 || i1 == i3) continue;
+      }
+      if (EMIT) pairs[base + n] = make_int2(qi, t);
+      n++;
+    }
+  }
+  if (!EMIT && live) cnt[(long long)qi * P + part] = n;
+}
+// Pass 2: pairs [0, npt) are point-triangle, [npt, n) edge-edge
+__device__ __forceinline__ int4 pair_corners(long long k, long long npt, int2 pr, const int* __restrict__ F,
+                                             const int* __restrict__ E) {
+  if (k < npt) return make_int4(pr.x, F[3 * pr.y], F[3 * pr.y + 1], F[3 * pr.y + 2]);
+  return make_int4(E[2 * pr.x], E[2 * pr.x + 1], E[2 * pr.y], E[2 * pr.y + 1]);
+}
+__global__ void k3_narrow(long long n, long long npt, const int2* __restrict__ pairs, const double* __restrict__ q,
+                          const int* __restrict__ F, const int* __restrict__ E, double thr, double ee_thres,
+                          int* __restrict__ hit, int* __restrict__ ptyp, double* __restrict__ pd) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int4 c = pair_corners(k, npt, pairs[k], F, E);
+  const D3 x0 = ld3(q, c.x), x1 = ld3(q, c.y), x2 = ld3(q, c.z), x3 = ld3(q, c.w);
+  int ty;
+  double d;
+  if (k < npt) {
+    ty = pt_classify(x0, x1, x2, x3);
+    d = pt_distance(x0, x1, x2, x3, ty);
+  } else {
+    ty = ee_classify(x0, x1, x2, x3, ee_thres);
+    d = ee_distance(x0, x1, x2, x3, ty);
+  }
+  hit[k] = d <= thr ? 1 : 0;
+  ptyp[k] = ty;
+  pd[k] = d;
+}
+// Pass 3
+__global__ void k3_compact(long long n, long long npt, const int2* __restrict__ pairs, const int* __restrict__ F,
+                           const int* __restrict__ E, const int* __restrict__ hit, const int* __restrict__ pos,
+                           const int* __restrict__ ptyp, const double* __restrict__ pd, int4* __restrict__ cidx,
+                           int2* __restrict__ ctyp, double* __restrict__ cd) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n || !hit[k]) return;
+  const int o = pos[k];
+  cidx[o] = pair_corners(k, npt, pairs[k], F, E);
+  ctyp[o] = make_int2(k < npt ? BT_PT : BT_EE, ptyp[k]);
+  cd[o] = pd[k];
 }
 
 // ------------------------------------------------------------ energy
 // Per-element energies at x into en[] (vertices, faces, hinges, contacts,
 // system.py:343-358); contacts are re-classified at x and keep the new type
 // and distance (the derivatives use them).
-__global__ void k3_energy_elem(int nV, int nF, int nH, int nC, const double* __restrict__ x,
+__global__ void k3_energy_elem(int nV, int nF, int nH, const int* __restrict__ nCp, const double* __restrict__ x,
                                const double* __restrict__ target, const double* __restrict__ vor,
                                const int* __restrict__ F, const double* __restrict__ W,
                                const double4* __restrict__ invDm, const int4* __restrict__ hinge,
                                const double2* __restrict__ rest, const int4* __restrict__ cidx,
                                int2* __restrict__ ctyp, double* __restrict__ cd, Cfg c, double* __restrict__ en) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long n = (long long)nV + nF + nH + nC;
+  const long long n = (long long)nV + nF + nH + *nCp;
   if (i >= n) return;
   if (i < nV) {
     en[i] = 0.5 * len2(ld3(x, i) - ld3(target, i)) * (vor[i] * c.m2gt_k);
@@ -1090,9 +1179,11 @@ __device__ double block_sum(const double* __restrict__ a, long long n, double* l
 }
 // the four parts of en[] and their total: parts summed, then
 // ((m2gt + membrane) + hinge) + contact
-__global__ void __launch_bounds__(S3_RED) k3_energy_sum(const double* __restrict__ en, int nV, int nF, int nH, int nC,
-                                                        double* __restrict__ sc, int slot) {
+__global__ void __launch_bounds__(S3_RED) k3_energy_sum(const double* __restrict__ en, int nV, int nF, int nH,
+                                                        const int* __restrict__ nCp, double* __restrict__ sc,
+                                                        int slot) {
   __shared__ double lds[S3_RED];
+  const int nC = *nCp;
   const long long o1 = nV, o2 = o1 + nF, o3 = o2 + nH;
   const double e0 = block_sum(en, nV, lds);
   const double e1 = block_sum(en + o1, nF, lds);
@@ -1133,10 +1224,12 @@ __global__ void k3_diff_faces(int nF, const double* __restrict__ x, const int* _
                 c.la, -1.0, o);
   for (int k = 0; k < 3; k++) st3(fs, 3LL * f + k, o[k]);
 }
-__global__ void __launch_bounds__(64) k3_diff_hinges(int nH, const double* __restrict__ x, const int4* __restrict__ hinge,
-                                                     const double2* __restrict__ rest, Cfg c, double* __restrict__ hs,
-                                                     double* __restrict__ hd, double* __restrict__ hb,
-                                                     double* __restrict__ scratch) {
+constexpr int S3_HINGE_BLOCK = 64;
+__global__ void __launch_bounds__(S3_HINGE_BLOCK) k3_diff_hinges(int nH, const double* __restrict__ x,
+                                                                 const int4* __restrict__ hinge,
+                                                                 const double2* __restrict__ rest, Cfg c,
+                                                                 double* __restrict__ hs, double* __restrict__ hd,
+                                                                 double* __restrict__ hb) {
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
   if (h >= nH) return;
   const int4 k = hinge[h];
@@ -1147,14 +1240,103 @@ __global__ void __launch_bounds__(64) k3_diff_hinges(int nH, const double* __res
     st3(hs, 4LL * h + j, g[j]);
     st3(hd, 4LL * h + j, mk3(H[12 * (3 * j) + 3 * j], H[12 * (3 * j + 1) + 3 * j + 1], H[12 * (3 * j + 2) + 3 * j + 2]));
   }
-  spd_project(H, c.spd_iters, scratch + 162LL * h, scratch + 162LL * h + 81);
 }
-__global__ void k3_diff_contacts(int nC, const double* __restrict__ x, const int4* __restrict__ cidx,
+
+// The projection of every hinge's Hessian by 9 lanes (7 hinges per 64-lane
+// workgroup): lane k owns row k of the leading 9x9 in the row updates and
+// column k in the column updates of each Jacobi rotation; every lane of a
+// hinge computes the rotation from the same entries.  Each update touches
+// one row (or column) per lane, as in the serial loops, so the results are
+// those of spd_project.  The rebuilt block row and column 3 and the
+// reconstruction run on the LDS copy, then the 12x12 goes back.
+constexpr int S3_SPD_H = 7;
+__global__ void __launch_bounds__(64) k3_spd(int nH, int it_max, double* __restrict__ hb) {
+  __shared__ double A[S3_SPD_H][81], Q[S3_SPD_H][81], Hl[S3_SPD_H][144];
+  const int w = threadIdx.x / 9, k = threadIdx.x % 9;
+  const int h = blockIdx.x * S3_SPD_H + w;
+  const bool live = w < S3_SPD_H && h < nH;
+  double* A_ = A[live ? w : 0];
+  double* Q_ = Q[live ? w : 0];
+  double* H_ = Hl[live ? w : 0];
+  if (live) {
+    const double* H = hb + 144LL * h;
+    for (int j = 0; j < 16; j++) H_[16 * k + j] = H[16 * k + j];
+  }
+  __syncthreads();
+  if (live)
+    for (int j = 0; j < 9; j++) {
+      A_[9 * k + j] = H_[12 * k + j];
+      Q_[9 * k + j] = k == j ? 1.0 : 0.0;
+    }
+  __syncthreads();
+  for (int sweep = 0; sweep < it_max; sweep++)
+    for (int pp = 0; pp < 8; pp++)
+      for (int qq = pp + 1; qq < 9; qq++) {
+        double c = 1.0, sn = 0.0;
+        bool rot = false;
+        if (live) {
+          const double apq = A_[9 * pp + qq];
+          rot = apq != 0.0;
+          if (rot) {
+            const double tau = (A_[9 * qq + qq] - A_[9 * pp + pp]) / (2.0 * apq);
+            const double t = tau >= 0.0 ? 1.0 / (tau + sqrt(1.0 + tau * tau)) : -1.0 / (-tau + sqrt(1.0 + tau * tau));
+            c = 1.0 / sqrt(1.0 + t * t);
+            sn = t * c;
+          }
+        }
+        __syncthreads();
+        if (rot) {   // A <- A J and Q <- Q J: row k
+          const double akp = A_[9 * k + pp], akq = A_[9 * k + qq];
+          A_[9 * k + pp] = c * akp - sn * akq;
+          A_[9 * k + qq] = sn * akp + c * akq;
+          const double qkp = Q_[9 * k + pp], qkq = Q_[9 * k + qq];
+          Q_[9 * k + pp] = c * qkp - sn * qkq;
+          Q_[9 * k + qq] = sn * qkp + c * qkq;
+        }
+        __syncthreads();
+        if (rot) {   // A <- J^T A: column k
+          const double apk = A_[9 * pp + k], aqk = A_[9 * qq + k];
+          A_[9 * pp + k] = c * apk - sn * aqk;
+          A_[9 * qq + k] = sn * apk + c * aqk;
+        }
+        __syncthreads();
+      }
+  if (live) {
+    double lam[9];
+    for (int i = 0; i < 9; i++) lam[i] = A_[9 * i + i] > 0.0 ? A_[9 * i + i] : 0.0;
+    for (int j = 0; j < 9; j++) {
+      double sum = 0.0;
+      for (int m = 0; m < 9; m++) sum += Q_[9 * k + m] * lam[m] * Q_[9 * j + m];
+      H_[12 * k + j] = sum;
+    }
+  }
+  __syncthreads();
+  if (live)   // block row 3, column k
+    for (int a = 0; a < 3; a++) {
+      const double sum = H_[12 * (0 + a) + k] + H_[12 * (3 + a) + k] + H_[12 * (6 + a) + k];
+      H_[12 * (9 + a) + k] = -1.0 * sum;
+    }
+  __syncthreads();
+  if (live)   // block column 3, row k: the transpose of block row 3
+    for (int cc = 0; cc < 3; cc++) H_[12 * k + 9 + cc] = H_[12 * (9 + cc) + k];
+  __syncthreads();
+  if (live) {   // block (3, 3), entry k
+    const int a = k / 3, cc = k % 3;
+    const double sum = H_[12 * a + 9 + cc] + H_[12 * (3 + a) + 9 + cc] + H_[12 * (6 + a) + 9 + cc];
+    H_[12 * (9 + a) + 9 + cc] = -1.0 * sum;
+  }
+  __syncthreads();
+  if (live) {
+    double* H = hb + 144LL * h;
+    for (int j = 0; j < 16; j++) H[16 * k + j] = H_[16 * k + j];
+  }
+}
+__global__ void k3_diff_contacts(const int* __restrict__ nCp, const double* __restrict__ x, const int4* __restrict__ cidx,
                                  const int2* __restrict__ ctyp, const double* __restrict__ cd, Cfg c,
                                  double* __restrict__ cdd, double2* __restrict__ cg, double* __restrict__ cs,
                                  double* __restrict__ cq) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nC) return;
+  if (j >= *nCp) return;
   D3 dd[4];
   for (int k = 0; k < 4; k++) dd[k] = mk3(0.0, 0.0, 0.0);
   const double d = cd[j];
@@ -1175,6 +1357,53 @@ __global__ void k3_diff_contacts(int nC, const double* __restrict__ x, const int
     st3(cq, 4LL * j + k, on ? g2 * cmul(dd[k], dd[k]) : mk3(0.0, 0.0, 0.0));
   }
 }
+// Per-vertex gathers by groups of S3_VG lanes over the vertex's slot list:
+// its faces, then its hinges, then its contacts (three CSR segments read as
+// one list).  Lane g adds entries g, g + S3_VG, g + 2 S3_VG, ... in order
+// (its list loads, then its slot loads, S3_VB at a time in flight), then
+// the lanes' partial sums meet in a butterfly: a fixed tree for a given list
+// length, so the sum is the same on every run.  Faces add no diagonal term.
+constexpr int S3_VG = 32, S3_VB = 8;
+struct VSeg {
+  const int* list;
+  int a, n;
+  const double* s1;
+  const double* s2;
+};
+template <bool TWO>
+__device__ __forceinline__ void gather3(const VSeg sf, const VSeg sh, const VSeg sc_, int g, D3& sum1, D3& sum2) {
+  const int tot = sf.n + sh.n + sc_.n;
+  D3 acc1 = mk3(0.0, 0.0, 0.0), acc2 = acc1;
+  for (int b0 = 0; b0 < tot; b0 += S3_VG * S3_VB) {
+    int id[S3_VB];
+    const double* p1[S3_VB];
+    const double* p2[S3_VB];
+#pragma unroll
+    for (int u = 0; u < S3_VB; u++) {
+      const int j = b0 + u * S3_VG + g;
+      const bool f = j < sf.n, h = !f && j < sf.n + sh.n;
+      const VSeg& sg = f ? sf : (h ? sh : sc_);
+      const int r = j - (f ? 0 : (h ? sf.n : sf.n + sh.n));
+      id[u] = j < tot ? sg.list[sg.a + r] : -1;
+      p1[u] = sg.s1;
+      p2[u] = sg.s2;
+    }
+#pragma unroll
+    for (int u = 0; u < S3_VB; u++) {
+      if (id[u] < 0) continue;
+      acc1 = acc1 + ld3(p1[u], id[u]);
+      if (TWO && p2[u]) acc2 = acc2 + ld3(p2[u], id[u]);
+    }
+  }
+#pragma unroll
+  for (int o = S3_VG / 2; o > 0; o >>= 1) {
+    acc1 = acc1 + mk3(__shfl_xor(acc1.x, o, S3_VG), __shfl_xor(acc1.y, o, S3_VG), __shfl_xor(acc1.z, o, S3_VG));
+    if (TWO) acc2 = acc2 + mk3(__shfl_xor(acc2.x, o, S3_VG), __shfl_xor(acc2.y, o, S3_VG), __shfl_xor(acc2.z, o, S3_VG));
+  }
+  sum1 = acc1;
+  sum2 = acc2;
+}
+
 // Forces and the Hessian diagonal per vertex (system.py:360-365): the
 // target term, then the slots of its faces, hinges and contacts in order.
 __global__ void k3_diff_vert(int nV, const double* __restrict__ x, const double* __restrict__ target,
@@ -1184,27 +1413,26 @@ __global__ void k3_diff_vert(int nV, const double* __restrict__ x, const double*
                              const int* __restrict__ vc_off, const int* __restrict__ vc,
                              const double* __restrict__ cs, const double* __restrict__ cq,
                              double* __restrict__ grad, double* __restrict__ diag) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // every lane of a group has the same vertex, so the group returns whole
+  const int i = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) / S3_VG);
+  const int g = threadIdx.x & (S3_VG - 1);
   if (i >= nV) return;
   const double w = vor[i] * c.m2gt_k;
-  D3 g = (ld3(x, i) - ld3(target, i)) * w * -1.0;
-  D3 dg = mk3(w, w, w);
-  for (int s = vf_off[i]; s < vf_off[i + 1]; s++) g = g + ld3(fs, vf[s]);
-  for (int s = vh_off[i]; s < vh_off[i + 1]; s++) {
-    g = g + ld3(hs, vh[s]);
-    dg = dg + ld3(hd, vh[s]);
+  const int af = vf_off[i], ah = vh_off[i], ac = vc_off[i];
+  D3 gs, ds;
+  gather3<true>(VSeg{vf, af, vf_off[i + 1] - af, fs, nullptr}, VSeg{vh, ah, vh_off[i + 1] - ah, hs, hd},
+                VSeg{vc, ac, vc_off[i + 1] - ac, cs, cq}, g, gs, ds);
+  const D3 gr = (ld3(x, i) - ld3(target, i)) * w * -1.0 + gs, dg = mk3(w, w, w) + ds;
+  if (g == 0) {
+    st3(grad, i, gr);
+    st3(diag, i, dg);
   }
-  for (int s = vc_off[i]; s < vc_off[i + 1]; s++) {
-    g = g + ld3(cs, vc[s]);
-    dg = dg + ld3(cq, vc[s]);
-  }
-  st3(grad, i, g);
-  st3(diag, i, dg);
 }
 
 // --------------------------------------------------------------- H dx
 // element parts of H dx (system.py:367-371) into the force slots
-__global__ void k3_hdx_elem(int nF, int nH, int nC, const double* __restrict__ x, const double* __restrict__ dx,
+__global__ void k3_hdx_elem(int nF, int nH, const int* __restrict__ nCp, const double* __restrict__ x,
+                            const double* __restrict__ dx,
                             const int* __restrict__ F, const double* __restrict__ W,
                             const double4* __restrict__ invDm, const int4* __restrict__ hinge,
                             const double* __restrict__ hb, const int4* __restrict__ cidx,
@@ -1220,23 +1448,21 @@ __global__ void k3_hdx_elem(int nF, int nH, int nC, const double* __restrict__ x
     elastic_hdx(ld3(x, a), ld3(x, b), ld3(x, e), ld3(dx, a), ld3(dx, b), ld3(dx, e), W[f], m2(m.x, m.y, m.z, m.w),
                 c.mu, c.la, o);
     for (int k = 0; k < 3; k++) st3(fs, 3LL * f + k, o[k]);
-  } else if (i < (long long)nF + nH) {
-    const int h = (int)(i - nF);
+  } else if (i < (long long)nF + 12LL * nH) {
+    // one lane per row of a hinge's 12x12 (12 consecutive lanes per hinge)
+    const long long rr = i - nF;
+    const int h = (int)(rr / 12), row = (int)(rr % 12);
     const int4 k = hinge[h];
-    const D3 d[4] = {ld3(dx, k.x), ld3(dx, k.y), ld3(dx, k.z), ld3(dx, k.w)};
-    const double* H = hb + 144LL * h;
-    for (int r = 0; r < 4; r++) {
-      double o[3];
-      for (int a = 0; a < 3; a++) {
-        const double* row = H + 12 * (3 * r + a);
-        double s = 0.0;
-        for (int j = 0; j < 4; j++) s = s + (row[3 * j] * d[j].x + row[3 * j + 1] * d[j].y + row[3 * j + 2] * d[j].z);
-        o[a] = s;
-      }
-      st3(hs, 4LL * h + r, mk3(o[0], o[1], o[2]));
+    const double* Hr = hb + 144LL * h + 12 * row;
+    const int id[4] = {k.x, k.y, k.z, k.w};
+    double s = 0.0;
+    for (int j = 0; j < 4; j++) {
+      const D3 d = ld3(dx, id[j]);
+      s = s + (Hr[3 * j] * d.x + Hr[3 * j + 1] * d.y + Hr[3 * j + 2] * d.z);
     }
-  } else if (i < (long long)nF + nH + nC) {
-    const int j = (int)(i - nF - nH);
+    hs[12LL * h + row] = s;
+  } else if (i < (long long)nF + 12LL * nH + *nCp) {
+    const int j = (int)(i - nF - 12LL * nH);
     const int4 k = cidx[j];
     if (!(cd[j] < c.d_hat)) {
       for (int r = 0; r < 4; r++) st3(cs, 4LL * j + r, mk3(0.0, 0.0, 0.0));
@@ -1259,15 +1485,19 @@ __global__ void k3_hdx_vert(int nV, const double* __restrict__ dx, const double*
                             const int* __restrict__ vh_off, const int* __restrict__ vh, const double* __restrict__ hs,
                             const int* __restrict__ vc_off, const int* __restrict__ vc, const double* __restrict__ cs,
                             double* __restrict__ out, double* __restrict__ prod) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) / S3_VG);
+  const int g = threadIdx.x & (S3_VG - 1);
   if (i >= nV) return;
   const D3 d = ld3(dx, i);
-  D3 o = d * (vor[i] * c.m2gt_k);
-  for (int s = vf_off[i]; s < vf_off[i + 1]; s++) o = o + ld3(fs, vf[s]);
-  for (int s = vh_off[i]; s < vh_off[i + 1]; s++) o = o + ld3(hs, vh[s]);
-  for (int s = vc_off[i]; s < vc_off[i + 1]; s++) o = o + ld3(cs, vc[s]);
-  st3(out, i, o);
-  if (prod) prod[i] = dot(d, o);
+  const int af = vf_off[i], ah = vh_off[i], ac = vc_off[i];
+  D3 os, unused;
+  gather3<false>(VSeg{vf, af, vf_off[i + 1] - af, fs, nullptr}, VSeg{vh, ah, vh_off[i + 1] - ah, hs, nullptr},
+                 VSeg{vc, ac, vc_off[i + 1] - ac, cs, nullptr}, g, os, unused);
+  const D3 o = d * (vor[i] * c.m2gt_k) + os;
+  if (g == 0) {
+    st3(out, i, o);
+    if (prod) prod[i] = dot(d, o);
+  }
 }
 
 // ----------------------------------------------------------------- CG
@@ -1312,6 +1542,41 @@ __global__ void k3_cg_next(int nV, int it, const double* __restrict__ sc, const 
   st3(v, i, ld3(z, i) + s * ld3(v, i));
 }
 
+// One CG iteration after H v (small meshes): v.Av from prod, alpha, the
+// updates of p, r and z, z.r, then v, in one workgroup; the same sums
+// (block_sum) and updates as k3_sum, k3_cg_update, k3_sum and k3_cg_next,
+// so the results are identical.
+__global__ void __launch_bounds__(S3_RED) k3_cg_step(int nV, int it, double* __restrict__ sc,
+                                                     const double* __restrict__ Av, const double* __restrict__ diag,
+                                                     double* __restrict__ v, double* __restrict__ p,
+                                                     double* __restrict__ r, double* __restrict__ z,
+                                                     double* __restrict__ prod) {
+  __shared__ double lds[S3_RED];
+  const double vAv = block_sum(prod, nV, lds);
+  const double zr_last = sc[SC_ZR0 + (it & 1)];
+  for (int i = threadIdx.x; i < nV; i += S3_RED) {
+    D3 rr = ld3(r, i);
+    if (vAv > 1e-16) {
+      const double alpha = zr_last / vAv;
+      st3(p, i, ld3(p, i) + alpha * ld3(v, i));
+      rr = rr - alpha * ld3(Av, i);
+      st3(r, i, rr);
+    }
+    const D3 dg = ld3(diag, i);
+    const D3 zz = mk3(rr.x / dg.x, rr.y / dg.y, rr.z / dg.z);
+    st3(z, i, zz);
+    prod[i] = dot(zz, rr);
+  }
+  __syncthreads();
+  const double zr_new = block_sum(prod, nV, lds);
+  const double sfac = zr_new / zr_last;
+  for (int i = threadIdx.x; i < nV; i += S3_RED) st3(v, i, ld3(z, i) + sfac * ld3(v, i));
+  if (threadIdx.x == 0) {
+    sc[SC_VAV] = vAv;
+    sc[SC_ZR0 + ((it + 1) & 1)] = zr_new;
+  }
+}
+
 // ------------------------------------------------- clamp, CCD, search
 // the step limited to the detection radius around q_det
 // (solver_kernels.py:76-94)
@@ -1322,10 +1587,10 @@ __global__ void k3_clamp(int nV, const double* __restrict__ qn, const double* __
   const D3 dq = ld3(qn, i) + ld3(p, i) - ld3(qdet, i);
   if (len(dq) > radius) st3(p, i, nrm(dq) * radius + ld3(qdet, i) - ld3(qn, i));
 }
-__global__ void k3_ccd(int nC, const double* __restrict__ q, const double* __restrict__ p,
+__global__ void k3_ccd(const int* __restrict__ nCp, const double* __restrict__ q, const double* __restrict__ p,
                        const int4* __restrict__ cidx, const int2* __restrict__ ctyp, Cfg c, double* __restrict__ sc) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nC) return;
+  if (j >= *nCp) return;
   const int4 k = cidx[j];
   const int id[4] = {k.x, k.y, k.z, k.w};
   D3 x[4], v[4];
@@ -1351,5 +1616,6 @@ __global__ void k3_ls_pos(int nV, int n, double* __restrict__ sc, const double* 
   if (i == 0) sc[SC_LS] = (double)n;
 }
 __global__ void k3_set_u64(unsigned long long* a, unsigned long long v) { *a = v; }
+__global__ void k3_set_i32(int* a, int v) { *a = v; }
 
 }  // namespace s3

@@ -3,17 +3,19 @@ tables: per kernel name, dispatches, total and average duration; with
 --rounds, the engine's rounds grouped by face count.  Used to write the
 summaries under profiles/; reads only the database it is given.
 
-    python3 rocprof_summary.py DB [TOP] [--rounds]
+    python3 rocprof_summary.py DB [TOP] [--rounds] [--match=SUBSTRING]
     python3 rocprof_summary.py --pmc AGENT_CSV COUNTER_CSV [COUNTER_CSV ...]
 """
 import sqlite3
 import sys
 
 
-def summary(db, top=30):
+def summary(db, top=30, match=None):
     con = sqlite3.connect(db)
     rows = con.execute("select name, count(*), sum(end-start), avg(end-start) from kernels group by name "
                        "order by sum(end-start) desc").fetchall()
+    if match:
+        rows = [r for r in rows if match in r[0]]
     tot = sum(r[2] for r in rows)
     out = ["total kernel time %.2f ms over %d dispatches" % (tot / 1e6, sum(r[1] for r in rows)), "",
            "| kernel | dispatches | total ms | avg us | share |", "|---|---|---|---|---|"]
@@ -126,7 +128,8 @@ if __name__ == "__main__":
     if "--pmc" in sys.argv:   # --pmc AGENT_CSV COUNTER_CSV [COUNTER_CSV ...]
         print(pmc(args[1:], args[0]))
         sys.exit(0)
-    print(summary(args[0], int(args[1]) if len(args) > 1 else 30))
+    match = next((a.split("=", 1)[1] for a in sys.argv if a.startswith("--match=")), None)
+    print(summary(args[0], int(args[1]) if len(args) > 1 else 30, match))
     if "--rounds" in sys.argv:
         print()
         print(rounds(args[0]))

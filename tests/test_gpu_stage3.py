@@ -106,6 +106,56 @@ def test_s3_contacts_active_barrier_exact(cuda):
     print("contacts: %d pairs (%d within d_hat), bit-exact" % (len(ro), int((do < 1e-3).sum())))
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+def test_s3_contact_search_modes(cuda, mode):
+    """The LBVH walk (mode 1) and the exhaustive tiled search (mode 2) list
+    the oracle's pairs in its order, on the close spheres and on a denser
+    stage-2 output with a wide radius."""
+    for gv, gf, v, f, radius in (close_spheres() + (None,), _pair(5, 2, 0.1) + (0.03,)):
+        o, g = _systems(gv, gf, v, f, radius)
+        g.ctx.debug_guard(mode)
+        o.part("detect")
+        g.part("detect")
+        ro, do = o.contacts()
+        rg, dg = g.contacts()
+        assert np.array_equal(rg, ro) and _ulps(dg, do) == 0
+        print("mode %d: %d pairs bit-exact" % (mode, len(ro)))
+
+
+def test_s3_cg_paths_identical(cuda):
+    """The one-workgroup CG step and the multi-kernel step give the same
+    bits (the launch shape never changes a result)."""
+    gv, gf, v, f = close_spheres()
+    out = []
+    for one in (1 << 30, 0):
+        o, g = _systems(gv, gf, v, f)
+        g.ctx.set_tuning("s3_cg_one_block", one)
+        g.update_target()
+        for p in ("detect", "energy", "diff", "cg"):
+            g.part(p)
+        out.append(g.get("p").cpu().numpy())
+    assert np.array_equal(out[0], out[1])
+
+
+def test_s3_graph_replays_equal_eager(cuda):
+    """The recorded Newton iteration, replayed after each eager contact
+    detection, gives the eager run's positions bit for bit, iteration by
+    iteration."""
+    gv, gf, v, f = close_spheres()
+    runs = []
+    for graph in (1, 0):
+        o, g = _systems(gv, gf, v, f)
+        g.ctx.set_tuning("s3_graph", graph)
+        g.update_target()
+        qs = []
+        for _ in range(6):
+            g.part("newton")
+            qs.append(g.get("q").cpu().numpy())
+        runs.append(qs)
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
+
+
 def test_s3_radius_halving(cuda):
     """More pairs than max_blocks: the radius is halved until they fit
     (energy.py:937-947), on both sides."""
