@@ -732,18 +732,24 @@ __global__ void k3_faces_rest(int nF, const double* __restrict__ X, const int* _
 // --------------------------------------------------------- float trees
 // Positions rounded to float for the LBVH boxes, and the largest
 // |coordinate| (its float bits, for the boxes' rounding margin).
-__global__ void k3_to_float(long long n, const double* __restrict__ X, float* __restrict__ P,
-                            unsigned* __restrict__ maxabs) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// (a grid-stride loop over at most S3_TF_BLOCKS workgroups of 256 lanes,
+// one atomic per workgroup: the atomics all go to one word)
+constexpr int S3_TF_BLOCKS = 1024;
+__global__ void __launch_bounds__(256) k3_to_float(long long n, const double* __restrict__ X, float* __restrict__ P,
+                                                   unsigned* __restrict__ maxabs) {
+  __shared__ float red[4];
   float m = 0.0f;
-  if (i < n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const float v = (float)X[i];
     P[i] = v;
-    m = fabsf(v);
-    if (!(m <= 3.0e38f)) m = 3.0e38f;
+    float a = fabsf(v);
+    if (!(a <= 3.0e38f)) a = 3.0e38f;
+    m = fmaxf(m, a);
   }
   for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(maxabs, __float_as_uint(m));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicMax(maxabs, __float_as_uint(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]))));
 }
 __global__ void k3_iota(int n, int* __restrict__ a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
