@@ -1126,12 +1126,21 @@ __global__ void k3_compact(long long n, long long npt, const int2* __restrict__ 
 // Per-element energies at x into en[] (vertices, faces, hinges, contacts,
 // system.py:343-358); contacts are re-classified at x and keep the new type
 // and distance (the derivatives use them).
-__global__ void k3_energy_elem(int nV, int nF, int nH, const int* __restrict__ nCp, const double* __restrict__ x,
+// Line-search step n_ls > 0 after the energy dropped below the energy
+// before the search: the positions did not move (k3_ls_pos), so the energy
+// and the contacts' types and distances would come out the same, and the
+// energy kernels return at once.  n_ls < 0: any other evaluation.
+__device__ __forceinline__ bool ls_settled(const double* __restrict__ sc, int n_ls) {
+  return n_ls > 0 && sc[SC_E] < sc[SC_EPREV];
+}
+__global__ void k3_energy_elem(int nV, int nF, int nH, const int* __restrict__ nCp, int n_ls,
+                               const double* __restrict__ sc, const double* __restrict__ x,
                                const double* __restrict__ target, const double* __restrict__ vor,
                                const int* __restrict__ F, const double* __restrict__ W,
                                const double4* __restrict__ invDm, const int4* __restrict__ hinge,
                                const double2* __restrict__ rest, const int4* __restrict__ cidx,
                                int2* __restrict__ ctyp, double* __restrict__ cd, Cfg c, double* __restrict__ en) {
+  if (ls_settled(sc, n_ls)) return;
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long n = (long long)nV + nF + nH + *nCp;
   if (i >= n) return;
@@ -1168,27 +1177,33 @@ __global__ void k3_energy_elem(int nV, int nF, int nH, const int* __restrict__ n
 }
 
 // Deterministic sums by one workgroup of S3_RED lanes: lane t adds elements
-// t, t + S3_RED, ... in order, then a fixed pairwise tree.
+// t, t + S3_RED, ... in order, then a fixed tree: a butterfly within each
+// 64-lane wave, the waves' sums through LDS, a butterfly over them; every
+// lane gets the sum.
 constexpr int S3_RED = 1024;
 __device__ double block_sum(const double* __restrict__ a, long long n, double* lds) {
   double s = 0.0;
   for (long long i = threadIdx.x; i < n; i += S3_RED) s += a[i];
-  lds[threadIdx.x] = s;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) lds[wave] = s;
   __syncthreads();
-  for (int w = S3_RED / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) lds[threadIdx.x] = lds[threadIdx.x] + lds[threadIdx.x + w];
-    __syncthreads();
-  }
-  const double r = lds[0];
-  __syncthreads();
-  return r;
+  // every aligned group of S3_RED / 64 lanes reduces the same wave sums in
+  // the same order, so every lane returns the same bits
+  double t = lds[lane & (S3_RED / 64 - 1)];
+#pragma unroll
+  for (int o = S3_RED / 128; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+  __syncthreads();   // every lane has read lds before the next call writes it
+  return t;
 }
 // the four parts of en[] and their total: parts summed, then
 // ((m2gt + membrane) + hinge) + contact
 __global__ void __launch_bounds__(S3_RED) k3_energy_sum(const double* __restrict__ en, int nV, int nF, int nH,
-                                                        const int* __restrict__ nCp, double* __restrict__ sc,
-                                                        int slot) {
+                                                        const int* __restrict__ nCp, int n_ls,
+                                                        double* __restrict__ sc, int slot) {
   __shared__ double lds[S3_RED];
+  if (ls_settled(sc, n_ls)) return;
   const int nC = *nCp;
   const long long o1 = nV, o2 = o1 + nF, o3 = o2 + nH;
   const double e0 = block_sum(en, nV, lds);
