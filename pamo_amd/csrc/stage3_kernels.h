@@ -1068,17 +1068,24 @@ __global__ void __launch_bounds__(S3_TILE) k3_box_pairs(int nq, int ncand, int c
     }
     __syncthreads();
     if (!live) continue;
-    for (int j = KIND == 1 ? max(0, qi + 1 - tb) : 0; j < m; j++) {
-      if (!box_meets(slo[j], shi[j], qlo, qhi)) continue;
-      const int t = tb + j;
-      if (KIND == 0) {
-        if (F[3 * t] == i0 || F[3 * t + 1] == i0 || F[3 * t + 2] == i0) continue;
-      } else {
-        const int i2 = E[2 * t], i3 = E[2 * t + 1];
-        if (i0 == i2 || i0 == i3 || i1 == i2 || i1 == i3) continue;
+    // four box tests per step (independent LDS reads), then their hits in order
+    for (int j = KIND == 1 ? max(0, qi + 1 - tb) : 0; j < m; j += 4) {
+      bool h[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) h[u] = j + u < m && box_meets(slo[j + u], shi[j + u], qlo, qhi);
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        if (!h[u]) continue;
+        const int t = tb + j + u;
+        if (KIND == 0) {
+          if (F[3 * t] == i0 || F[3 * t + 1] == i0 || F[3 * t + 2] == i0) continue;
+        } else {
+          const int i2 = E[2 * t], i3 = E[2 * t + 1];
+          if (i0 == i2 || i0 == i3 || i1 == i2 || i1 == i3) continue;
+        }
+        if (EMIT) pairs[base + n] = make_int2(qi, t);
+        n++;
       }
-      if (EMIT) pairs[base + n] = make_int2(qi, t);
-      n++;
     }
   }
   if (!EMIT && live) cnt[(long long)qi * P + part] = n;
