@@ -245,6 +245,56 @@ def test_s3_newton_iterations(cuda, case):
         assert err <= 1e-9
 
 
+def test_s3_large_mesh_lbvh_contacts_and_newton(cuda):
+    """A 19,928-face mesh (above the exhaustive contact search's 16,384-face
+    limit, so the LBVH walks run by default) with a 81,920-face target:
+    targets and the ~490k-pair contact list bit-exact, the exhaustive search
+    forced on the same positions equal too, and one Newton iteration within
+    tolerance."""
+    gv, gf, v, f = _pair(6, 8, 0.25)
+    assert len(f) > 16384
+    o, g = _systems(gv, gf, v, f)
+    o.update_target()
+    g.update_target()
+    assert np.array_equal(g.get("target").cpu().numpy(), o.get("target"))
+    o.part("detect")
+    g.part("detect")
+    ro, do = o.contacts()
+    rg, dg = g.contacts()
+    assert len(ro) > 100000 and np.array_equal(rg, ro) and _ulps(dg, do) == 0
+    g.ctx.debug_guard(2)
+    g.part("detect")
+    rg2, dg2 = g.contacts()
+    assert np.array_equal(rg2, ro) and _ulps(dg2, do) == 0
+    g.ctx.debug_guard(0)
+    g.set("q", o.get("q"))
+    for p in ("energy", "diff", "cg", "clamp", "ccd", "line_search"):
+        o.part(p)
+        g.part(p)
+    so, sg = o.scalars(), g.scalars()
+    err = np.abs(g.get("q").cpu().numpy() - o.get("q")).max()
+    print("large: %d faces, %d contacts, energy %.12e / %.12e, |dq| %.2e" % (len(f), len(ro), sg["energy"],
+                                                                          so["energy"], err))
+    assert sg["ls_halvings"] == so["ls_halvings"] and abs(sg["ccd_step"] - so["ccd_step"]) <= 1e-9
+    assert abs(sg["energy"] - so["energy"]) <= 1e-10 * abs(so["energy"]) and err <= 1e-9
+
+
+def test_s3_pamo_all_stages(cuda):
+    """PaMO(use_stage1=True, use_stage3=True): remeshing, stage 2, then stage
+    3 toward the input; the stage-3 part against the oracle on the same
+    stage-2 output."""
+    from pamo_amd import PaMO, synth
+    v, f = synth.displaced_icosphere(4, 9)
+    m = PaMO((v, f), use_stage1=True, use_stage3=True, verbose=False)
+    out_v, out_f = m.run(torch.from_numpy(v).cuda(), torch.from_numpy(f).cuda(), 0.05, min_verts=0)
+    s2v, s2f = m.last_stage2_output
+    vo, _ = o3.process(v, f, s2v, s2f, 5)
+    ext = float(np.max(v.max(0) - v.min(0)))
+    err = float(np.abs(out_v - vo).max()) / ext
+    print("PaMO stages 1+2+3: %d -> %d faces, max |GPU - oracle| / extent = %.3e" % (len(f), len(out_f), err))
+    assert np.array_equal(out_f, s2f) and np.isfinite(out_v).all() and err <= 1e-6
+
+
 @pytest.mark.parametrize("level,ratio", [(4, 0.02), (6, 0.05)])
 def test_s3_process_matches_oracle(cuda, level, ratio):
     from pamo_amd import safe_project as s3
