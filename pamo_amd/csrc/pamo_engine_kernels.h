@@ -572,16 +572,23 @@ __device__ __forceinline__ float ec_moved_term(const float* __restrict__ P, int4
   return 1.0f - clamp_ref(term, 0.0f, 1.0f);
 }
 
-// one fan's slots for the edge: serial length and skinny sums, counts, flip
+// Both fans' slots for the edge as one list (fan u, then fan v), G at a
+// time: serial length and skinny sums in that order, counts, flip.  A chunk
+// may hold the end of fan u and the start of fan v, so an edge takes
+// ceil((deg u + deg v) / G) passes instead of one per fan chunk.
 template <int G>
-__device__ __forceinline__ void ec_fan(const float* __restrict__ P, const int4* __restrict__ sface,
-                                       const float4* __restrict__ splane, const float4* __restrict__ slen, int a,
-                                       int b, int g, int moved, int other, V3 m, float& el, float& Qa, int& ne,
-                                       int& nt, int& flip) {
-  for (int base = a; base < b; base += G) {
-    int s = base + g;
+__device__ __forceinline__ void ec_fans(const float* __restrict__ P, const int4* __restrict__ sface,
+                                        const float4* __restrict__ splane, const float4* __restrict__ slen, int ua,
+                                        int ub, int va, int vb, int g, int u, int v, V3 m, float& el, float& Qa,
+                                        int& ne, int& nt, int& flip) {
+  const int nu = ub - ua, n = nu + (vb - va);
+  for (int base = 0; base < n; base += G) {
+    const int j = base + g;
     float tx = 0.0f, ty = 0.0f, tz = 0.0f, tq = 0.0f;
-    if (s < b) {
+    if (j < n) {
+      const bool fu = j < nu;
+      const int s = fu ? ua + j : va + (j - nu);
+      const int moved = fu ? u : v, other = fu ? v : u;
       int4 t = sface[s];
       float4 L = slen[s];
       if (t.x > t.y) { tx = L.x; ne++; }
@@ -593,12 +600,12 @@ __device__ __forceinline__ void ec_fan(const float* __restrict__ P, const int4* 
         nt++;
       }
     }
-    const int n = b - base < G ? b - base : G;
-    for (int j = 0; j < n; j++) {
-      el += __shfl(tx, j, G);
-      el += __shfl(ty, j, G);
-      el += __shfl(tz, j, G);
-      Qa += __shfl(tq, j, G);
+    const int cnt = n - base < G ? n - base : G;
+    for (int k = 0; k < cnt; k++) {
+      el += __shfl(tx, k, G);
+      el += __shfl(ty, k, G);
+      el += __shfl(tz, k, G);
+      Qa += __shfl(tq, k, G);
     }
   }
 }
@@ -674,8 +681,7 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__
   V3 m = edge_position(Q, u, v, v0, v1, placement);
   float el = 0.0f, Qa = 0.0f;
   int ne = 0, nt = 0, flip = 0;
-  ec_fan<G>(P, sface, splane, slen, ua, ub, g, u, v, m, el, Qa, ne, nt, flip);
-  ec_fan<G>(P, sface, splane, slen, va, vb, g, v, u, m, el, Qa, ne, nt, flip);
+  ec_fans<G>(P, sface, splane, slen, ua, ub, va, vb, g, u, v, m, el, Qa, ne, nt, flip);
   ne = ec_group_sum<G>(ne);
   nt = ec_group_sum<G>(nt);
   flip = ec_group_sum<G>(flip);
