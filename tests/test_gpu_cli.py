@@ -27,3 +27,19 @@ def test_cli_output_matches_oracle(cuda, tmp_path, ext):
     assert np.array_equal(of, wf)
     assert np.array_equal(ov.astype(np.float32), (wv + mean).astype(np.float32))
     assert rc == 0
+
+
+def test_cli_all_stages(cuda, tmp_path):
+    """With no --disable flag every stage runs (example.py's default):
+    remeshing, stage 2, then stage 3 toward the input.  The written mesh
+    has the stage-2 faces of that run and finite float64-projected
+    vertices."""
+    v, f = synth.displaced_icosphere(4, 5)
+    src = str(tmp_path / "in.obj")
+    dst = str(tmp_path / "out.obj")
+    meshio.save(src, v.astype(np.float32), f)
+    rc = main(["-i", src, "-o", dst, "-r", "0.05"])
+    ov, of = meshio.load(dst)
+    print("cli all stages: %d -> %d faces" % (len(f), len(of)), flush=True)
+    assert rc == 0 and len(of) > 0 and np.isfinite(ov).all()
+    assert of.min() >= 0 and of.max() < len(ov)
