@@ -1605,6 +1605,75 @@ __global__ void __launch_bounds__(S3_RED) k3_cg_step(int nV, int it, double* __r
   }
 }
 
+// k3_cg_step for meshes of at most 2 S3_RED vertices, each lane's vertices
+// held in registers between the phases (one load of the vectors, no reload
+// of the products or of z): the same values and sums as k3_cg_step.
+constexpr int S3_CG_REG = 2;
+__device__ __forceinline__ double block_sum_partial(double s, double* lds) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) lds[wave] = s;
+  __syncthreads();
+  double t = lds[lane & (S3_RED / 64 - 1)];
+#pragma unroll
+  for (int o = S3_RED / 128; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+  __syncthreads();
+  return t;
+}
+__global__ void __launch_bounds__(S3_RED) k3_cg_step_reg(int nV, int it, double* __restrict__ sc,
+                                                         const double* __restrict__ Av,
+                                                         const double* __restrict__ diag, double* __restrict__ v,
+                                                         double* __restrict__ p, double* __restrict__ r,
+                                                         double* __restrict__ z, const double* __restrict__ prod) {
+  __shared__ double lds[S3_RED];
+  double s0 = 0.0;
+  D3 rr[S3_CG_REG], vv[S3_CG_REG], pp[S3_CG_REG], aa[S3_CG_REG], dg[S3_CG_REG];
+#pragma unroll
+  for (int k = 0; k < S3_CG_REG; k++) {
+    const int i = threadIdx.x + k * S3_RED;
+    if (i < nV) {
+      s0 += prod[i];
+      rr[k] = ld3(r, i);
+      vv[k] = ld3(v, i);
+      pp[k] = ld3(p, i);
+      aa[k] = ld3(Av, i);
+      dg[k] = ld3(diag, i);
+    }
+  }
+  const double vAv = block_sum_partial(s0, lds);
+  const double zr_last = sc[SC_ZR0 + (it & 1)];
+  double s1 = 0.0;
+  D3 zz[S3_CG_REG];
+#pragma unroll
+  for (int k = 0; k < S3_CG_REG; k++) {
+    const int i = threadIdx.x + k * S3_RED;
+    if (i < nV) {
+      if (vAv > 1e-16) {
+        const double alpha = zr_last / vAv;
+        pp[k] = pp[k] + alpha * vv[k];
+        rr[k] = rr[k] - alpha * aa[k];
+        st3(p, i, pp[k]);
+        st3(r, i, rr[k]);
+      }
+      zz[k] = mk3(rr[k].x / dg[k].x, rr[k].y / dg[k].y, rr[k].z / dg[k].z);
+      st3(z, i, zz[k]);
+      s1 += dot(zz[k], rr[k]);
+    }
+  }
+  const double zr_new = block_sum_partial(s1, lds);
+  const double sfac = zr_new / zr_last;
+#pragma unroll
+  for (int k = 0; k < S3_CG_REG; k++) {
+    const int i = threadIdx.x + k * S3_RED;
+    if (i < nV) st3(v, i, zz[k] + sfac * vv[k]);
+  }
+  if (threadIdx.x == 0) {
+    sc[SC_VAV] = vAv;
+    sc[SC_ZR0 + ((it + 1) & 1)] = zr_new;
+  }
+}
+
 // ------------------------------------------------- clamp, CCD, search
 // the step limited to the detection radius around q_det
 // (solver_kernels.py:76-94)
