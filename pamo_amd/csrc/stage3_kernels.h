@@ -568,6 +568,7 @@ enum {
   SC_CCD = 11,     // u64 bits of the CCD step
   SC_MINANG = 12,  // u64 bits of the smallest corner angle (degrees)
   SC_GMAXABS = 13, // float bits of the target's largest |coordinate| (uint)
+  SC_TOTAL = 14,   // u64: 64-bit total of a count array (k3_total64)
   SC_COUNT = 16
 };
 
@@ -750,6 +751,20 @@ __global__ void __launch_bounds__(256) k3_to_float(long long n, const double* __
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) atomicMax(maxabs, __float_as_uint(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]))));
+}
+// the 64-bit total of n int counts, added into *out (a grid-stride loop
+// over at most S3_TF_BLOCKS workgroups of 256 lanes, one atomic each): the
+// int scans of the counts are taken only once this total is known to fit
+__global__ void __launch_bounds__(256) k3_total64(const int* __restrict__ cnt, long long n,
+                                                  unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long red[4];
+  unsigned long long t = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    t += (unsigned long long)(unsigned)max(cnt[i], 0);
+  for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(out, red[0] + red[1] + red[2] + red[3]);
 }
 __global__ void k3_iota(int n, int* __restrict__ a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

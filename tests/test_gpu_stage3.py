@@ -173,6 +173,27 @@ def test_s3_radius_halving(cuda):
     assert np.array_equal(g.contacts()[0], o.contacts()[0])
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+def test_s3_halving_without_end_fails(cuda, mode):
+    """More pairs within d_hat alone than max_blocks: halving the radius
+    stops changing the threshold, and the detection fails with an error
+    (the reference recurses without end there) under both searches; a
+    max_blocks beyond the int offsets is refused."""
+    from pamo_amd import safe_project as s3
+    from pamo_amd._lib import PamoError
+    gv, gf, v, f = close_spheres()
+    cfg = s3.Stage3Config()
+    cfg.max_blocks = 10
+    g = s3.Stage3System(cfg)
+    g.register_mesh(gv, gf, v, f)
+    g.ctx.debug_guard(mode)
+    with pytest.raises(PamoError, match="max_blocks"):
+        g.part("detect")
+    cfg.max_blocks = 1 << 31
+    with pytest.raises(PamoError, match="invalid configuration"):
+        s3.Stage3System(cfg, ctx=g.ctx)
+
+
 CASES = {"icosphere": lambda: _pair(4, 3), "close_spheres": close_spheres}
 
 
