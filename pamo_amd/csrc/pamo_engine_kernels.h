@@ -528,11 +528,57 @@ __device__ __forceinline__ V3 edge_position(const float* __restrict__ Q, int u, 
 // against 13.1 ms with 8, 14.6 ms with 2 and 20.3 ms with 16); the kernel is
 // instantiated for 2, 4 and 8 and the autotuner may pick another.
 constexpr int EC_G = 4;
-template <int G>
+
+// Lane K's x in every aligned group of G lanes.  Groups of 2 and 4 lanes
+// take a DPP quad permutation (one VALU move, which the compiler may fold
+// into the instruction that uses it); wider groups a bpermute through the
+// LDS crossbar.  Every lane of a group must be active.
+template <int G, int K>
+__device__ __forceinline__ int grp_bcast(int x) {
+  static_assert(K >= 0 && K < G, "source lane inside the group");
+  if constexpr (G == 4) {
+    return __builtin_amdgcn_mov_dpp(x, K | (K << 2) | (K << 4) | (K << 6), 0xF, 0xF, false);
+  } else if constexpr (G == 2) {
+    return __builtin_amdgcn_mov_dpp(x, K | (K << 2) | ((K + 2) << 4) | ((K + 2) << 6), 0xF, 0xF, false);
+  } else {
+    return __shfl(x, K, G);
+  }
+}
+template <int G, int K>
+__device__ __forceinline__ float grp_bcast(float x) {
+  return __int_as_float(grp_bcast<G, K>(__float_as_int(x)));
+}
+// lane l's x ^ mask partner (mask < G)
+template <int G, int MASK>
+__device__ __forceinline__ int grp_xor(int x) {
+  if constexpr (G <= 4) {
+    constexpr int a = 0 ^ MASK, b = 1 ^ MASK, c = 2 ^ MASK, d = 3 ^ MASK;
+    return __builtin_amdgcn_mov_dpp(x, a | (b << 2) | (c << 4) | (d << 6), 0xF, 0xF, false);
+  } else {
+    return __shfl_xor(x, MASK, G);
+  }
+}
+template <int G, int O = 1>
 __device__ __forceinline__ int ec_group_sum(int x) {
-#pragma unroll
-  for (int o = 1; o < G; o <<= 1) x += __shfl_xor(x, o, G);
-  return x;
+  if constexpr (O < G) return ec_group_sum<G, 2 * O>(x + grp_xor<G, O>(x));
+  else return x;
+}
+// acc += lane K's terms, K = 0 .. G-1 in order
+template <int G, int K = 0>
+__device__ __forceinline__ void ec_ordered_add(float& el, float& Qa, float tx, float ty, float tz, float tq) {
+  if constexpr (K < G) {
+    el += grp_bcast<G, K>(tx);
+    el += grp_bcast<G, K>(ty);
+    el += grp_bcast<G, K>(tz);
+    Qa += grp_bcast<G, K>(tq);
+    ec_ordered_add<G, K + 1>(el, Qa, tx, ty, tz, tq);
+  }
+}
+// the number of lanes K of the group whose iv equals this lane's iu
+template <int G, int K = 0>
+__device__ __forceinline__ int ec_matches(int iv, int iu) {
+  if constexpr (K < G) return (int)(grp_bcast<G, K>(iv) == iu) + ec_matches<G, K + 1>(iv, iu);
+  else return 0;
 }
 
 // The moved face's term (cusimp_free.cu:497-575), or -1 when it turns over
@@ -575,7 +621,9 @@ __device__ __forceinline__ float ec_moved_term(const float* __restrict__ P, int4
 // Both fans' slots for the edge as one list (fan u, then fan v), G at a
 // time: serial length and skinny sums in that order, counts, flip.  A chunk
 // may hold the end of fan u and the start of fan v, so an edge takes
-// ceil((deg u + deg v) / G) passes instead of one per fan chunk.
+// ceil((deg u + deg v) / G) passes instead of one per fan chunk.  Lanes past
+// the list's end contribute +0 terms, added like the others: the sums start
+// at +0 and hold no -0, so x + (+0) leaves them unchanged.
 template <int G>
 __device__ __forceinline__ void ec_fans(const float* __restrict__ P, const int4* __restrict__ sface,
                                         const float4* __restrict__ splane, const float4* __restrict__ slen, int ua,
@@ -600,13 +648,7 @@ __device__ __forceinline__ void ec_fans(const float* __restrict__ P, const int4*
         nt++;
       }
     }
-    const int cnt = n - base < G ? n - base : G;
-    for (int k = 0; k < cnt; k++) {
-      el += __shfl(tx, k, G);
-      el += __shfl(ty, k, G);
-      el += __shfl(tz, k, G);
-      Qa += __shfl(tq, k, G);
-    }
+    ec_ordered_add<G>(el, Qa, tx, ty, tz, tq);
   }
 }
 
@@ -657,14 +699,14 @@ __global__ void __launch_bounds__(BLOCK) k_edge_cost_g(const float* __restrict__
   int ua = fan_off[u], ub = fan_off[u + 1], va = fan_off[v], vb = fan_off[v + 1];
   // link condition: equal successor pairs; more than 2 -> maximum, not 2 ->
   // unchanged.  The group loads G slots of each fan at once and compares
-  // every pair through shuffles (successors are vertex indices, >= 0).
+  // every pair through lane broadcasts (successors are vertex indices, >= 0;
+  // lanes past a fan's end hold -1 and -2, which match nothing).
   int dup = 0;
   for (int a0 = ua; a0 < ub; a0 += G) {
     const int iu = a0 + g < ub ? fsucc[a0 + g] : -1;
     for (int b0 = va; b0 < vb; b0 += G) {
       const int iv = b0 + g < vb ? fsucc[b0 + g] : -2;
-      const int n = vb - b0 < G ? vb - b0 : G;
-      for (int j = 0; j < n; j++) dup += __shfl(iv, j, G) == iu;
+      dup += ec_matches<G>(iv, iu);
     }
   }
   dup = ec_group_sum<G>(dup);
