@@ -81,7 +81,8 @@ def parse():
                    help="sdf2m: skip the batched and served legs of the default line")
     p.add_argument("--leg-meshes", type=int, default=512, help="batched leg: meshes of the many-mesh mix")
     p.add_argument("--leg-jobs", type=int, default=1000, help="served leg: jobs of the Poisson stream")
-    p.add_argument("--leg-rate", type=float, default=300.0, help="served leg: arrival rate in jobs/s")
+    p.add_argument("--leg-rate", type=float, default=350.0, help="served leg: arrival rate in jobs/s")
+    p.add_argument("--serve-rounds", type=int, default=1, help="server: rounds per loop iteration")
     p.add_argument("--copy-probe", type=int, default=0,
                    help="MiB copied device-to-device once before timing: a copy kernel of known size that "
                         "calibrates the FETCH_SIZE / WRITE_SIZE counters in a rocprofv3 --pmc run")
@@ -564,7 +565,7 @@ def _served_leg(args):
     # warm-up: a few jobs through the plain path load every kernel
     _plain_each(meshes[:4], ctx, with_prep=True)
     got = {}
-    srv = batch.Server(engine=batch.BatchEngine(ctx=ctx))
+    srv = batch.Server(engine=batch.BatchEngine(ctx=ctx), rounds_per_step=args.serve_rounds)
     t0 = time.perf_counter()
     recs = srv.run(jobs, on_result=lambda i, v, f, info: got.__setitem__(i, (v, f)))
     log("served leg: %d jobs in %.1fs" % (len(jobs), time.perf_counter() - t0))
@@ -827,7 +828,7 @@ def run_serve(args):
         arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(meshes)))
         jobs = [dict(points=v, faces=f, ratio=r, arrival=float(t)) for (v, f, r), t in zip(meshes, arrivals)]
         eng = batch.BatchEngine(ctx=ctx)
-        srv = batch.Server(engine=eng)
+        srv = batch.Server(engine=eng, rounds_per_step=args.serve_rounds)
         t0 = time.perf_counter()
         last = [t0]
 
@@ -855,6 +856,7 @@ def run_serve(args):
                       dtype="f32", data="synthetic (seeded generators, in-process)",
                       config=dict(workload="serve", jobs=len(jobs), faces=F_total, rate_jobs_s=rate,
                                   arrival_seed=5, seeds="5000000+i", hbm_budget_gb=200,
+                                  rounds_per_step=args.serve_rounds,
                                   parallelism="single GPU, continuous batching"),
                       latency_ms=dict(p50=round(1e3 * float(np.percentile(lat, 50)), 2),
                                       p99=round(1e3 * float(np.percentile(lat, 99)), 2),

@@ -153,7 +153,10 @@ int pamo_prepare_host(const float* points, const int* faces, int V, int F, float
  * pamo_batch_submit copies one mesh in (centred vertices, faces, scale and
  * the run parameters of pamo_run_stage2; job is an opaque caller tag) and
  * returns its handle (>= 0).  The mesh joins at the next round, also while
- * others are mid-run. */
+ * others are mid-run.  The copies are stream-ordered (as hipMemcpyAsync):
+ * d_verts and d_faces must stay unchanged until later work on the stream,
+ * or a synchronisation, has passed the call.  Face indices are checked
+ * before the call returns (-2 when one lies outside [0, n_verts)). */
 int pamo_batch_submit(pamo_ctx* ctx, void* stream, const float* d_verts, const int* d_faces, int n_verts, int n_faces,
                       float scale, long long target, float threshold, int tolerance, long long iters, long long job);
 

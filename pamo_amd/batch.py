@@ -146,12 +146,15 @@ def simplify_batch(meshes, ratios, threshold=1e-3, tolerance=4, min_verts=0, eng
 class Server:
     """Continuous batching: jobs (points, faces, ratio, arrival time) are
     admitted when they have arrived and the packed state stays under the
-    HBM budget; every loop iteration runs one round for all admitted jobs.
+    HBM budget; every loop iteration runs rounds_per_step rounds for all
+    admitted jobs (the engine queues each round behind the previous one on
+    the device; jobs are admitted and results taken between iterations).
     Latency = completion time - arrival time (both on the host clock)."""
 
     def __init__(self, budget_bytes=200 << 30, threshold=1e-3, tolerance=4, min_verts=0, engine=None,
-                 prep_threads=4):
+                 prep_threads=4, rounds_per_step=1):
         self.eng = engine or BatchEngine()
+        self.rounds_per_step = max(1, int(rounds_per_step))
         self.budget = budget_bytes
         self.threshold = threshold
         self.tolerance = tolerance
@@ -217,10 +220,10 @@ class Server:
                     continue
                 st["max_live"] = max(st["max_live"], len(live))
                 ta = time.perf_counter()
-                self.eng.step(1)
+                self.eng.step(self.rounds_per_step)
                 tb = time.perf_counter()
                 st["step_s"] += tb - ta
-                st["rounds"] += 1
+                st["rounds"] += self.rounds_per_step
                 for h, _ in self.eng.poll():
                     ta = time.perf_counter()
                     v, f, info = self.eng.take(h)

@@ -1148,18 +1148,35 @@ __global__ void k_sorder_keys(const int* __restrict__ T0, const float* __restric
   keys[f] = mesh_morton_key(fmesh[f], mp, lo, hi, bits);
   vals[f] = f;
 }
+// An admitted mesh's vertex bounds: the ordered-integer min / max of
+// k_mesh_bounds decoded to floats (all zero for a mesh without vertices).
+__global__ void k_box_store(const int* __restrict__ scene, int any, float* __restrict__ box) {
+  const int q = threadIdx.x;
+  if (q >= 6) return;
+  int i = scene[q];
+  i = i >= 0 ? i : i ^ 0x7FFFFFFF;
+  box[q] = any ? __int_as_float(i) : 0.0f;
+}
+// each active mesh's box from its slot's admission bounds
+__global__ void k_mp_boxes(MeshParam* __restrict__ mp, int M, const int* __restrict__ kslot,
+                           const float* __restrict__ dbox) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= M) return;
+  const float* b = dbox + 6LL * kslot[k];
+#pragma unroll
+  for (int q = 0; q < 6; q++) mp[k].box[q] = b[q];
+}
 // keys of an admitted mesh's faces [f0, f0 + n): the Morton code of the box
 // centre in the mesh's vertex box (the mesh is appended last, so its
 // faces follow every earlier mesh's in the order)
 __global__ void k_sorder_keys_new(const int* __restrict__ T, const float* __restrict__ P, long long f0, int n,
-                                  float lo0, float lo1, float lo2, float hi0, float hi1, float hi2,
-                                  u64* __restrict__ keys, int* __restrict__ vals) {
+                                  const float* __restrict__ box, u64* __restrict__ keys, int* __restrict__ vals) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float4 lo, hi;
   face_box(P, ldt(T, (int)(f0 + i)), lo, hi);
   const float c[3] = {0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
-  const float bl[3] = {lo0, lo1, lo2}, bh[3] = {hi0, hi1, hi2};
+  const float bl[3] = {box[0], box[1], box[2]}, bh[3] = {box[3], box[4], box[5]};
   uint32_t q[3];
 #pragma unroll
   for (int a = 0; a < 3; a++) {

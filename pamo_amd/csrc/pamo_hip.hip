@@ -273,12 +273,16 @@ struct pamo_ctx {
     return true;
   }
   // grow without preserving contents
+  // Growth takes half again the request (an eighth in the fixed region of
+  // the no-allocation entry point): a batch that fills from empty (serving)
+  // regrows its round buffers a few times instead of dozens, and each
+  // regrowth frees (which waits for the device) and maps memory.
   bool ensure(Buf& b, size_t bytes) {
     if (bytes <= b.bytes && b.p) return true;
     if (capturing) return fail("buffer growth during graph capture");
     alloc_gen++;
-    size_t nb = std::max<size_t>(bytes + bytes / 8, 256);
-    if (arena) return carve(b, nb);
+    size_t nb = std::max<size_t>(bytes + bytes / 2, 256);
+    if (arena) return carve(b, std::max<size_t>(bytes + bytes / 8, 256));   // the fixed region's own rule
     if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.bytes = 0;
@@ -782,7 +786,7 @@ static void eng_free(pamo_ctx* c) {
                 &g.root, &g.root_end, &g.scene, &g.g, &g.keep, &g.kpos, &g.newk, &g.vkeep, &g.vpos, &g.npass,
                 &g.fsucc, &g.sface, &g.splane, &g.slen, &g.erank, &g.vedge, &g.uoff, &g.vmin, &g.wlist, &g.frank,
                 &g.ucnt, &g.oflag, &g.bfnb, &g.bfoff, &g.pk, &g.sorder, &g.sorder2, &g.nsort, &g.tbox, &g.bfq,
-                &g.bfw, &g.bfl, &g.bfn, &g.ms, &g.live, &g.pkd[0], &g.pkd[1]};
+                &g.bfw, &g.bfl, &g.bfn, &g.ms, &g.live, &g.pkd[0], &g.pkd[1], &g.dbox, &g.kslot};
   if (!c->arena) {
     for (Buf* b : all)
       if (b->p) (void)hipFree(b->p);
@@ -824,9 +828,9 @@ int pamo_batch_submit(pamo_ctx* ctx, void* stream, const float* d_verts, const i
   k_validate<<<nblk(n_faces), BLOCK, 0, c.st>>>(d_faces, n_faces, n_verts, c.cnt_dev(C_ERR));
   if (!c.check_launch("validate") || !c.read_counters()) return -1;
   if (c.hc[C_ERR]) { c.fail("invalid input: %d face indices out of range", c.hc[C_ERR]); return -2; }
-  int h = eng_admit(c, d_verts, d_faces, n_verts, n_faces, scale, target, threshold, tolerance, iters, job);
-  if (h < 0) return -1;
-  return c.hip(hipStreamSynchronize(c.st), "sync") ? h : -1;
+  // stream-ordered from here: the copies of the caller's buffers are
+  // enqueued on the stream and complete before any later work on it
+  return eng_admit(c, d_verts, d_faces, n_verts, n_faces, scale, target, threshold, tolerance, iters, job);
 }
 
 int pamo_batch_step(pamo_ctx* ctx, void* stream, int max_rounds, int* n_active) {
