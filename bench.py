@@ -66,6 +66,9 @@ def parse():
     p.add_argument("--fp64-seconds", type=float, default=150.0,
                    help="budget of the whole-run FP64 shadow (it stops after the round that passes it)")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--cpu-full", action="store_true",
+                   help="sdf7m: run the oracle's whole loop (one forward per round, progress lines) instead of a "
+                        "sample, and compare its output with the GPU's bit for bit")
     p.add_argument("--no-verify", action="store_true")
     p.add_argument("--placement", default="midpoint", choices=["midpoint", "optimal"],
                    help="plain workloads: where a collapse places its vertex (optimal: not the reference's)")
@@ -223,11 +226,11 @@ def run_plain(args):
                   h2d_ms=round(h2d_ms, 3))
     if not args.no_cpu:
         ncores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
-        if args.workload == "sdf7m":
+        if args.workload == "sdf7m" and not args.cpu_full:
             result["cpu_baseline"] = _cpu_plain(verts, f, scale, target, round_faces, ncores, args.cpu_seconds)
         else:
             result["cpu_baseline"] = _cpu_full(verts, f, scale, target, round_faces, ncores, out_v, out_f,
-                                               args.placement)
+                                               args.placement, rounds_loop=args.workload == "sdf7m")
         if args.cpu1_seconds > 0:
             one = _cpu_plain(verts, f, scale, target, round_faces, 1, args.cpu1_seconds)
             result["cpu_baseline"]["one_thread"] = {k: one[k] for k in ("value", "unit", "cores", "sample",
@@ -597,16 +600,28 @@ def _served_leg(args):
                 max_live_jobs=srv.stats.get("max_live"))
 
 
-def _cpu_full(verts, f, scale, target, round_faces, threads, out_v, out_f, placement):
+def _cpu_full(verts, f, scale, target, round_faces, threads, out_v, out_f, placement, rounds_loop=False):
     """The oracle's whole loop on the host with the given threads, timed over
-    the stage-2 window; its output against the GPU's, bit for bit."""
+    the stage-2 window; its output against the GPU's, bit for bit.  With
+    rounds_loop the loop is the reference's Python loop over one oracle
+    forward per round (oracle.run_rounds, the same results), which logs its
+    progress: a run of minutes reports as it goes."""
     import numpy as np
     import oracle
     threads = oracle.set_threads(threads)
     log("cpu oracle: %d threads, full run" % threads)
     t0 = time.perf_counter()
-    ov, of, st, orf = oracle.run(verts, f, scale, target, return_rounds=True,
-                                 placement=1 if placement == "optimal" else 0)
+    if rounds_loop:
+        last = [t0]
+
+        def progress(r, n):
+            if time.perf_counter() - last[0] > 15:
+                last[0] = time.perf_counter()
+                log("cpu oracle: round %d, %d faces, %.0fs" % (r, n, last[0] - t0))
+        ov, of, st, orf = oracle.run_rounds(verts, f, scale, target, progress=progress)
+    else:
+        ov, of, st, orf = oracle.run(verts, f, scale, target, return_rounds=True,
+                                     placement=1 if placement == "optimal" else 0)
     sec = time.perf_counter() - t0
     same = ov.tobytes() == np.ascontiguousarray(out_v).tobytes() and of.tobytes() == np.ascontiguousarray(out_f).tobytes()
     return dict(value=round(int(f.shape[0]) / sec / 1e6, 4), unit="Mfaces/s", cores=threads, kind="port",

@@ -208,6 +208,47 @@ def run_timed(verts, faces, scale, target, max_seconds, threshold=1e-3, toleranc
     return dict(rounds=n, exit_reason=int(st[3])), rf[:n].copy(), rs[:n].copy()
 
 
+def run_rounds(verts, faces, scale, target, threshold=1e-3, tolerance=4, iters=1000000, shuffle=SHUFFLE_FEISTEL,
+               progress=None):
+    """``run`` as the reference's Python loop (simp_cuda/pamo/__init__.py:
+    122-158): one ``Oracle.forward`` per round, the compaction by occ and map
+    in numpy, and progress(rounds, faces) after every round (for long runs
+    that report as they go).  Returns (verts, faces, stats, faces after each
+    round) like ``run(..., return_rounds=True)``."""
+    o = Oracle(shuffle)
+    pts = np.ascontiguousarray(verts, np.float32)
+    tris = np.ascontiguousarray(faces, np.int32)
+    undo = np.zeros(0, np.int32)
+    count, is_stuck, init, rounds, exit_reason, rf = 0, 0, True, 0, 4, []
+    for _ in range(iters):
+        n_prev = tris.shape[0]
+        p2, t2, occ, vmap, undo = o.forward(pts, tris, undo, scale, threshold, is_stuck, init)
+        init = False
+        rounds += 1
+        pts = p2[occ.reshape(-1).astype(bool)]
+        t2 = t2[t2[:, 0] >= 0]
+        tris = np.ascontiguousarray(vmap.reshape(-1)[t2], np.int32)
+        n_cur = tris.shape[0]
+        rf.append(n_cur)
+        if progress is not None:
+            progress(rounds, n_cur)
+        if n_cur <= target or n_cur <= 10:
+            exit_reason = 1 if n_cur <= target else 2
+            break
+        if n_cur == n_prev:
+            count += 1
+        else:
+            count = 0
+            is_stuck = 0
+        if count >= 2:
+            is_stuck = 1
+        if count == tolerance:
+            exit_reason = 3
+            break
+    stats = dict(rounds=rounds, V=int(pts.shape[0]), F=int(tris.shape[0]), exit_reason=exit_reason)
+    return pts, tris, stats, np.array(rf, np.int32)
+
+
 def fp64_shadow(verts, faces, scale, target, threshold=1e-3, tolerance=4, max_seconds=0.0,
                 shuffle=SHUFFLE_FEISTEL):
     """The FP32 round loop with an FP64 shadow of every round: costs of all

@@ -394,6 +394,22 @@ def test_run_reaches_target_or_tolerance():
     assert pv.verify(ov, of)["ok"]
 
 
+@pytest.mark.parametrize("target_ratio", [0.05, 0.0])
+def test_python_round_loop_equals_c_loop(target_ratio):
+    """run_rounds (the reference's loop in Python over one forward per round,
+    reporting progress) gives the C loop's output bit for bit, round by
+    round, to the target and to the tolerance stop."""
+    v, f = synth.displaced_icosphere(4, 8)
+    verts, scale, _ = oracle.preprocess(v, f)
+    target = int(target_ratio * f.shape[0])
+    ov, of, st, rf = oracle.run(verts, f, scale, target, return_rounds=True)
+    seen = []
+    pv_, pf_, pst, prf = oracle.run_rounds(verts, f, scale, target, progress=lambda r, n: seen.append((r, n)))
+    assert pv_.tobytes() == ov.tobytes() and pf_.tobytes() == of.tobytes()
+    assert list(prf) == list(rf) and [n for _, n in seen] == list(rf)
+    assert pst["rounds"] == st["rounds"] and pst["exit_reason"] == st["exit_reason"]
+
+
 @pytest.mark.parametrize("tol", [1, 4, 7])
 def test_run_without_collapses_stops_after_tolerance_rounds(tol):
     """Threshold 0 admits no collapse on a sphere (every quantized cost is
