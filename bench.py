@@ -597,7 +597,12 @@ def _served_leg(args):
                            service="measured for every job (preparation, upload, pamo_run_stage2); FIFO queue over "
                                    "the same arrivals"),
                 served_over_plain=round(served / cap, 2), results_equal_plain="%d of %d" % (same, len(meshes)),
-                max_live_jobs=srv.stats.get("max_live"))
+                max_live_jobs=srv.stats.get("max_live"),
+                device_gb_peak=round(srv.stats["device_bytes_peak"] / 1e9, 3),
+                live_input_faces_peak=srv.stats["live_faces_peak"],
+                device_bytes_per_live_input_face=round(srv.stats["device_bytes_peak"] /
+                                                       max(srv.stats["live_faces_peak"], 1), 1),
+                admission_bytes_per_face=batch.BYTES_PER_FACE)
 
 
 def _cpu_full(verts, f, scale, target, round_faces, threads, out_v, out_f, placement, rounds_loop=False):

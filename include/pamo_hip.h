@@ -36,6 +36,13 @@ void pamo_ctx_destroy(pamo_ctx* ctx);
 /* Last error message of the context (empty string when none). */
 const char* pamo_last_error(const pamo_ctx* ctx);
 
+/* Device bytes the context holds: its buffers, the batched engine's and the
+ * results not yet taken, and the stage-3 system's (the fixed region's size
+ * for a context in the no-allocation mode).  No counterpart in the
+ * reference; for callers that admit work by HBM footprint.  -1 for a null
+ * context. */
+long long pamo_device_bytes(pamo_ctx* ctx);
+
 /* Replaces CUDSP_Free.forward / CUSimp_Free::forward
  * (simp_cuda/src/pybind.cpp:52-72, simp_cuda/src/cusimp_free.cu:891-1048):
  * one guarded simplification round.  d_points [n_pts,3] f32, d_tris

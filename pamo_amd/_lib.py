@@ -44,6 +44,8 @@ def lib():
     L.pamo_ctx_destroy.restype = None
     L.pamo_last_error.argtypes = [P]
     L.pamo_last_error.restype = ctypes.c_char_p
+    L.pamo_device_bytes.argtypes = [P]
+    L.pamo_device_bytes.restype = ctypes.c_longlong
     L.pamo_free_forward.argtypes = [P, P, P, P, P, I, I, I, F, F, I, I]
     L.pamo_free_forward.restype = I
     L.pamo_sp_forward.argtypes = [P, P, P, P, I, I, F, F, I]
@@ -148,6 +150,11 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def device_bytes(self):
+        """Device bytes this context holds (its buffers, the batched
+        engine's and the stage-3 system's)."""
+        return int(self._L.pamo_device_bytes(self.handle))
 
     def error(self):
         msg = self._L.pamo_last_error(self.handle)

@@ -22,8 +22,12 @@ from concurrent.futures import ThreadPoolExecutor
 
 from . import _lib, prepare_host
 
-# device bytes per face of the packed state (positions, faces and their
-# round-start copies, fans, edges, costs, claims, LBVH nodes, scan scratch)
+# device bytes per admitted input face of the packed state (positions,
+# faces and their round-start copies, fans, edges, costs, claims, LBVH
+# nodes, scan scratch, growth headroom).  Serving the 1,000-job mix of 1k to
+# 400k-face meshes, the context's peak device bytes (pamo_device_bytes) were
+# 375 per admitted input face at the peak (12.3 GB for 32.8M faces), so the
+# admission estimate stays above what the engine allocates.
 BYTES_PER_FACE = 420
 
 
@@ -178,8 +182,12 @@ class Server:
         # the job's arrival on, overlapping the rounds already on the device
         pool = ThreadPoolExecutor(max_workers=max(1, self.prep_threads))
         prepared = {}
-        # host seconds per part of the loop, rounds run, peak admitted jobs
-        st = dict(step_s=0.0, submit_s=0.0, prep_wait_s=0.0, take_s=0.0, idle_s=0.0, rounds=0, max_live=0)
+        # host seconds per part of the loop, rounds run, peak admitted jobs,
+        # peak device bytes of the engine's context and peak admitted input
+        # faces (the measure behind BYTES_PER_FACE)
+        st = dict(step_s=0.0, submit_s=0.0, prep_wait_s=0.0, take_s=0.0, idle_s=0.0, rounds=0, max_live=0,
+                  device_bytes_peak=0, live_faces_peak=0)
+        live_faces = 0
         self.stats = st
         t0 = time.perf_counter()
         done = 0
@@ -212,6 +220,8 @@ class Server:
                     st["submit_s"] += tc - tb
                     live[h] = (i, m, need, tc - t0)
                     used += need
+                    live_faces += len(jobs[i]["faces"])
+                    st["live_faces_peak"] = max(st["live_faces_peak"], live_faces)
                 if not live:
                     if nxt < len(order):
                         ta = time.perf_counter()
@@ -224,12 +234,14 @@ class Server:
                 tb = time.perf_counter()
                 st["step_s"] += tb - ta
                 st["rounds"] += self.rounds_per_step
+                st["device_bytes_peak"] = max(st["device_bytes_peak"], self.eng.ctx.device_bytes())
                 for h, _ in self.eng.poll():
                     ta = time.perf_counter()
                     v, f, info = self.eng.take(h)
                     st["take_s"] += time.perf_counter() - ta
                     i, m, need, t_adm = live.pop(h)
                     used -= need
+                    live_faces -= len(jobs[i]["faces"])
                     t_end = time.perf_counter() - t0
                     recs[i] = dict(job=i, arrival=jobs[i]["arrival"], admitted=t_adm, finished=t_end,
                                    latency=t_end - jobs[i]["arrival"], faces_in=len(jobs[i]["faces"]),

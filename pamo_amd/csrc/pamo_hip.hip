@@ -262,6 +262,15 @@ struct pamo_ctx {
 
   // nb bytes from the arena (no-allocation mode); the buffer's previous
   // slice, if any, stays unused until the region is set up again
+  // every Buf member of the context (engine and stage-3 buffers are their own)
+  std::vector<Buf*> bufs() {
+    return {&pts_map, &table, &edge_cost, &undo_list, &tmp_undo, &P, &P0, &occ, &T, &T0, &cnt, &fan_off, &fan,
+           &ecnt, &eoff, &edges, &Q, &tri_min, &win, &winpos, &collapsed, &qflag, &qf, &qpos, &af, &apos,
+           &qlist, &pairs, &ovf, &seg_lo, &seg_hi, &leaf_face, &lbox_lo, &lbox_hi, &keys0, &keys1, &vals0, &vals1, &hist, &kleft, &kright,
+           &kparent, &rfirst, &rlast, &nodeA, &nodeB, &S, &S_first, &mflag, &mpos, &marked,
+           &scan_desc, &scan_desc_g, &counters, &cstats, &shuf_flags, &shuf_pos, &RV[0], &RV[1], &RF[0], &RF[1], &keep,
+           &kpos, &undo_in, &rm_key, &rm_st, &rm_votes, &rm_ids, &rm_toff, &rm_coff, &rm_code, &rm_tab, &rm_V, &rm_F};
+  }
   bool carve(Buf& b, size_t nb) {
     const size_t off = (arena_used + 255) & ~(size_t)255;
     if (off > arena_cap || nb > arena_cap - off)
@@ -446,13 +455,7 @@ pamo_ctx::~pamo_ctx() {
   (void)hipDeviceSynchronize();
   eng_free(this);
   s3_free(this);
-  Buf* all[] = {&pts_map, &table, &edge_cost, &undo_list, &tmp_undo, &P, &P0, &occ, &T, &T0, &cnt, &fan_off, &fan,
-                &ecnt, &eoff, &edges, &Q, &tri_min, &win, &winpos, &collapsed, &qflag, &qf, &qpos, &af, &apos,
-                &qlist, &pairs, &ovf, &seg_lo, &seg_hi, &leaf_face, &lbox_lo, &lbox_hi, &keys0, &keys1, &vals0, &vals1, &hist, &kleft, &kright,
-                &kparent, &rfirst, &rlast, &nodeA, &nodeB, &S, &S_first, &mflag, &mpos, &marked,
-                &scan_desc, &scan_desc_g, &counters, &cstats, &shuf_flags, &shuf_pos, &RV[0], &RV[1], &RF[0], &RF[1], &keep,
-                &kpos, &undo_in, &rm_key, &rm_st, &rm_votes, &rm_ids, &rm_toff, &rm_coff, &rm_code, &rm_tab, &rm_V, &rm_F};
-  for (Buf* b : all)
+  for (Buf* b : bufs())
     if (b->p) (void)hipFree(b->p);
   for (auto& r : prof_open) { ev_pool.push_back(r.a); ev_pool.push_back(r.b); }
   for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
@@ -775,20 +778,24 @@ int pamo_ctx::forward(const float* d_pts, const int* d_tris, const int* d_undo, 
 #include "pamo_remesh.inc"
 #include "pamo_stage3.inc"
 
+// every Buf member of Engine
+static std::vector<Buf*> eng_bufs(Engine& g) {
+  return {&g.P, &g.P0, &g.T, &g.T0, &g.vmesh, &g.fmesh, &g.nP, &g.nT, &g.nvm, &g.nfm, &g.undo_in, &g.undo_in2,
+         &g.undo_out, &g.tmp, &g.marked, &g.pm, &g.table, &g.ecost, &g.pm2, &g.table2, &g.ecost2, &g.roff,
+         &g.mp, &g.voff, &g.foff, &g.eoffm, &g.uioff, &g.uioff2, &g.nustk, &g.h, &g.lcnt, &g.ncoll, &g.cut,
+         &g.cutoff, &g.uobase, &g.hm, &g.part, &g.first, &g.nvu, &g.nvoff, &g.nfoff, &g.seg_a, &g.seg_b,
+         &g.root, &g.root_end, &g.scene, &g.g, &g.keep, &g.kpos, &g.newk, &g.vkeep, &g.vpos, &g.npass,
+         &g.fsucc, &g.sface, &g.splane, &g.slen, &g.erank, &g.vedge, &g.uoff, &g.vmin, &g.wlist, &g.frank,
+         &g.ucnt, &g.oflag, &g.bfnb, &g.bfoff, &g.pk, &g.sorder, &g.sorder2, &g.nsort, &g.tbox, &g.bfq,
+         &g.bfw, &g.bfl, &g.bfn, &g.ms, &g.live, &g.pkd[0], &g.pkd[1], &g.dbox, &g.kslot};
+}
+
 static void eng_free(pamo_ctx* c) {
   if (!c->eng) return;
   Engine& g = *c->eng;
-  // every Buf member of Engine
-  Buf* all[] = {&g.P, &g.P0, &g.T, &g.T0, &g.vmesh, &g.fmesh, &g.nP, &g.nT, &g.nvm, &g.nfm, &g.undo_in, &g.undo_in2,
-                &g.undo_out, &g.tmp, &g.marked, &g.pm, &g.table, &g.ecost, &g.pm2, &g.table2, &g.ecost2, &g.roff,
-                &g.mp, &g.voff, &g.foff, &g.eoffm, &g.uioff, &g.uioff2, &g.nustk, &g.h, &g.lcnt, &g.ncoll, &g.cut,
-                &g.cutoff, &g.uobase, &g.hm, &g.part, &g.first, &g.nvu, &g.nvoff, &g.nfoff, &g.seg_a, &g.seg_b,
-                &g.root, &g.root_end, &g.scene, &g.g, &g.keep, &g.kpos, &g.newk, &g.vkeep, &g.vpos, &g.npass,
-                &g.fsucc, &g.sface, &g.splane, &g.slen, &g.erank, &g.vedge, &g.uoff, &g.vmin, &g.wlist, &g.frank,
-                &g.ucnt, &g.oflag, &g.bfnb, &g.bfoff, &g.pk, &g.sorder, &g.sorder2, &g.nsort, &g.tbox, &g.bfq,
-                &g.bfw, &g.bfl, &g.bfn, &g.ms, &g.live, &g.pkd[0], &g.pkd[1], &g.dbox, &g.kslot};
+
   if (!c->arena) {
-    for (Buf* b : all)
+    for (Buf* b : eng_bufs(g))
       if (b->p) (void)hipFree(b->p);
     for (auto& s : g.slots) {
       if (s.outV.p) (void)hipFree(s.outV.p);
@@ -1068,6 +1075,23 @@ int pamo_ctx_create(pamo_ctx** out, int device, int shuffle_mode) {
 void pamo_ctx_destroy(pamo_ctx* ctx) { delete ctx; }
 
 const char* pamo_last_error(const pamo_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+long long pamo_device_bytes(pamo_ctx* ctx) {
+  if (!ctx) return -1;
+  if (ctx->arena) return (long long)ctx->arena_cap;
+  long long n = 0;
+  for (Buf* b : ctx->bufs())
+    if (b->p) n += (long long)b->bytes;
+  if (ctx->eng) {
+    for (Buf* b : eng_bufs(*ctx->eng))
+      if (b->p) n += (long long)b->bytes;
+    for (const EngSlot& s : ctx->eng->slots) n += (long long)(s.outV.p ? s.outV.bytes : 0) + (s.outF.p ? s.outF.bytes : 0);
+  }
+  if (ctx->s3)
+    for (Buf* b : ctx->s3->buffers())
+      if (b->p) n += (long long)b->bytes;
+  return n;
+}
 
 int pamo_free_forward(pamo_ctx* ctx, void* stream, const float* d_points, const int* d_tris, const int* d_undo,
                       int n_undo, int n_pts, int n_tris, float scale, float threshold, int is_stuck, int init) {

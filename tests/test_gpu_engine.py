@@ -101,15 +101,24 @@ def test_admission_mid_stream(cuda):
         assert _same(got[i][0], ov) and _same(got[i][1], of), i
 
 
-def test_server_results_match(cuda):
+@pytest.mark.parametrize("rounds_per_step", [1, 3])
+def test_server_results_match(cuda, rounds_per_step):
+    """Served results equal the oracle's, with one or several rounds per
+    loop iteration; the loop records the engine's peak device bytes, which
+    cover at least the packed positions and faces of the admitted meshes."""
     meshes = _meshes()[:4]
     jobs = [dict(points=v, faces=f, ratio=0.01, arrival=0.002 * i) for i, (v, f) in enumerate(meshes)]
     out = {}
-    recs = batch.Server().run(jobs, on_result=lambda i, v, f, info: out.__setitem__(i, (v, f)))
+    srv = batch.Server(rounds_per_step=rounds_per_step)
+    recs = srv.run(jobs, on_result=lambda i, v, f, info: out.__setitem__(i, (v, f)))
     assert all(r is not None and r["latency"] >= 0 for r in recs)
     for i, (v, f) in enumerate(meshes):
         ov, of, _ = _oracle_result(v, f, 0.01)
         assert _same(out[i][0], ov) and _same(out[i][1], of), i
+    st = srv.stats
+    assert st["live_faces_peak"] >= max(len(f) for _, f in meshes)
+    assert st["device_bytes_peak"] >= 24 * st["live_faces_peak"]
+    assert srv.eng.ctx.device_bytes() >= st["device_bytes_peak"] // 2 > 0
 
 
 @pytest.mark.parametrize("guard", [1, 2])
