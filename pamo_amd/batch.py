@@ -113,6 +113,14 @@ class BatchEngine:
                     candidate_pairs=int(out[5]))
 
 
+def _prepare_pinned(points, faces):
+    """prepare_host with the vertices and faces copied into pinned host
+    memory (new tensors from torch's pinned allocator), so that their upload
+    is an asynchronous copy on the stream."""
+    hv, ht, scale, mean = prepare_host(points, faces)
+    return torch.from_numpy(hv).pin_memory(), torch.from_numpy(ht).pin_memory(), scale, mean
+
+
 def prepare(points, faces, device):
     """prepare_host, with the vertices and faces moved to the device."""
     verts, tri, scale, tris_mean = prepare_host(points, faces)
@@ -178,8 +186,10 @@ class Server:
         live = {}
         used = 0
         recs = [None] * len(jobs)
-        # host preparation (centring and scale) runs on worker threads from
-        # the job's arrival on, overlapping the rounds already on the device
+        # host preparation (centring and scale, then a copy into pinned
+        # memory) runs on worker threads from the job's arrival on,
+        # overlapping the rounds already on the device; the upload at
+        # admission is then an asynchronous copy
         pool = ThreadPoolExecutor(max_workers=max(1, self.prep_threads))
         prepared = {}
         # host seconds per part of the loop, rounds run, peak admitted jobs,
@@ -197,7 +207,7 @@ class Server:
                 while nxt < len(order) and (jobs[order[nxt]]["arrival"] <= now or not realtime):
                     i = order[nxt]
                     waiting.append(i)
-                    prepared[i] = pool.submit(prepare_host, jobs[i]["points"], jobs[i]["faces"])
+                    prepared[i] = pool.submit(_prepare_pinned, jobs[i]["points"], jobs[i]["faces"])
                     nxt += 1
                 while waiting:
                     i = waiting[0]
@@ -213,7 +223,7 @@ class Server:
                     hv, ht, s, m = prepared.pop(i).result()
                     tb = time.perf_counter()
                     target = max(int(jobs[i]["ratio"] * len(jobs[i]["faces"])), self.min_verts)
-                    h = self.eng.submit(torch.from_numpy(hv).to(dev), torch.from_numpy(ht).to(dev), s, target,
+                    h = self.eng.submit(hv.to(dev, non_blocking=True), ht.to(dev, non_blocking=True), s, target,
                                         self.threshold, self.tolerance, job=i)
                     tc = time.perf_counter()
                     st["prep_wait_s"] += tb - ta
