@@ -247,11 +247,12 @@ __global__ void __launch_bounds__(BLOCK) k_leaf_boxes(const int* __restrict__ le
   int l = blockIdx.x * blockDim.x + threadIdx.x;
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   if (l < n) {
-    I3 t = ldt(T, leaf_face[l]);
+    const int f = leaf_face[l];
+    I3 t = ldt(T, f);
     V3 a = ldp(P, t.i), b = ldp(P, t.j), c = ldp(P, t.k);
     lo[0] = fminf(a.x, fminf(b.x, c.x)); lo[1] = fminf(a.y, fminf(b.y, c.y)); lo[2] = fminf(a.z, fminf(b.z, c.z));
     hi[0] = fmaxf(a.x, fmaxf(b.x, c.x)); hi[1] = fmaxf(a.y, fmaxf(b.y, c.y)); hi[2] = fmaxf(a.z, fmaxf(b.z, c.z));
-    lbox_lo[l] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+    lbox_lo[l] = make_float4(lo[0], lo[1], lo[2], __int_as_float(f));   // .w: the leaf's face
     lbox_hi[l] = make_float4(hi[0], hi[1], hi[2], 0.0f);
   }
   float v[6] = {lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]};
@@ -507,7 +508,6 @@ __global__ void __launch_bounds__(BLOCK) k_bottom_up_local(const int* __restrict
                                                            const float4* __restrict__ lbox_lo,
                                                            const float4* __restrict__ lbox_hi,
                                                            const int* __restrict__ sorted_leaf,
-                                                           const int* __restrict__ leaf_face,
                                                            const int* __restrict__ parent, const int* __restrict__ left,
                                                            const int* __restrict__ right, const int* __restrict__ rfirst,
                                                            const int* __restrict__ rlast, float4* nodeA, float4* nodeB,
@@ -528,7 +528,8 @@ __global__ void __launch_bounds__(BLOCK) k_bottom_up_local(const int* __restrict
     blo[threadIdx.x][0] = lo.x; blo[threadIdx.x][1] = lo.y; blo[threadIdx.x][2] = lo.z;
     bhi[threadIdx.x][0] = hi.x; bhi[threadIdx.x][1] = hi.y; bhi[threadIdx.x][2] = hi.z;
     nodeA[n - 1 + l] = make_float4(lo.x, lo.y, lo.z, __int_as_float(escape_after(l, n, rfirst)));
-    nodeB[n - 1 + l] = make_float4(hi.x, hi.y, hi.z, __int_as_float(~leaf_face[src]));
+    // (the leaf's face rides in lo.w: one gathered record fewer per leaf)
+    nodeB[n - 1 + l] = make_float4(hi.x, hi.y, hi.z, __int_as_float(~__float_as_int(lo.w)));
     rlo[threadIdx.x][0] = lo.x; rlo[threadIdx.x][1] = lo.y; rlo[threadIdx.x][2] = lo.z;
     rhi[threadIdx.x][0] = hi.x; rhi[threadIdx.x][1] = hi.y; rhi[threadIdx.x][2] = hi.z;
   } else {

@@ -1055,9 +1055,12 @@ __global__ void __launch_bounds__(BLOCK) k_leaf_boxes_b(const int* __restrict__ 
                                                         float4* __restrict__ lbox_lo, float4* __restrict__ lbox_hi) {
   int l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= *n_ptr) return;
-  I3 t = ldt(T, leaf_face[l]);
+  const int f = leaf_face[l];
+  I3 t = ldt(T, f);
   V3 a = ldp(P, t.i), b = ldp(P, t.j), c = ldp(P, t.k);
-  lbox_lo[l] = make_float4(fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)), fminf(a.z, fminf(b.z, c.z)), 0.0f);
+  // .w of the low corner: the leaf's face (read by the node boxes)
+  lbox_lo[l] = make_float4(fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)), fminf(a.z, fminf(b.z, c.z)),
+                           __int_as_float(f));
   lbox_hi[l] = make_float4(fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)), fmaxf(a.z, fmaxf(b.z, c.z)), 0.0f);
 }
 // vertex bounds of one mesh at admission (ordered-int atomics, 6 slots)

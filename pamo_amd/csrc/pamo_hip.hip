@@ -468,7 +468,7 @@ pamo_ctx::~pamo_ctx() {
 bool pamo_ctx::node_boxes(const int* n_alive, const int* sorted_leaf, long long Fl) {
   if (!ensure(seg_lo, (size_t)seg_entries(Fl) * 16) || !ensure(seg_hi, (size_t)seg_entries(Fl) * 16)) return false;
   k_bottom_up_local<<<nblk(Fl), BLOCK, 0, st>>>(n_alive, ptr<float4>(lbox_lo), ptr<float4>(lbox_hi), sorted_leaf,
-                                               ptr<int>(leaf_face), ptr<int>(kparent), ptr<int>(kleft),
+                                               ptr<int>(kparent), ptr<int>(kleft),
                                                ptr<int>(kright), ptr<int>(rfirst), ptr<int>(rlast), ptr<float4>(nodeA),
                                                ptr<float4>(nodeB), Fl, ptr<float4>(seg_lo), ptr<float4>(seg_hi));
   for (int j0 = 8; (1LL << j0) < Fl; j0 += 8) {
