@@ -238,6 +238,11 @@ __global__ void k_scene_init(int* scene) {
   else if (threadIdx.x < 6) scene[threadIdx.x] = (int)0x80000000;
 }
 
+// Leaf boxes of the LBVH builds are interleaved records: the low corner of
+// leaf l at lbox[2 l] (its face in .w), the high corner at lbox[2 l + 1];
+// callers pass lbox_lo = lbox and lbox_hi = lbox + 1.  The node boxes gather
+// the leaves through the Morton order, one 32-byte record per leaf.
+constexpr int LBOX_STRIDE = 2;
 __global__ void __launch_bounds__(BLOCK) k_leaf_boxes(const int* __restrict__ leaf_face, const int* __restrict__ n_ptr,
                                                       const float* __restrict__ P, const int* __restrict__ T,
                                                       float4* __restrict__ lbox_lo, float4* __restrict__ lbox_hi,
@@ -252,8 +257,8 @@ __global__ void __launch_bounds__(BLOCK) k_leaf_boxes(const int* __restrict__ le
     V3 a = ldp(P, t.i), b = ldp(P, t.j), c = ldp(P, t.k);
     lo[0] = fminf(a.x, fminf(b.x, c.x)); lo[1] = fminf(a.y, fminf(b.y, c.y)); lo[2] = fminf(a.z, fminf(b.z, c.z));
     hi[0] = fmaxf(a.x, fmaxf(b.x, c.x)); hi[1] = fmaxf(a.y, fmaxf(b.y, c.y)); hi[2] = fmaxf(a.z, fmaxf(b.z, c.z));
-    lbox_lo[l] = make_float4(lo[0], lo[1], lo[2], __int_as_float(f));   // .w: the leaf's face
-    lbox_hi[l] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+    lbox_lo[LBOX_STRIDE * l] = make_float4(lo[0], lo[1], lo[2], __int_as_float(f));   // .w: the leaf's face
+    lbox_hi[LBOX_STRIDE * l] = make_float4(hi[0], hi[1], hi[2], 0.0f);
   }
   float v[6] = {lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]};
 #pragma unroll
@@ -296,7 +301,7 @@ __global__ void k_morton(const float4* __restrict__ lbox_lo, const float4* __res
   if (l >= n) return;
   float slo[3] = {ord2f(scene[0]), ord2f(scene[1]), ord2f(scene[2])};
   float shi[3] = {ord2f(scene[3]), ord2f(scene[4]), ord2f(scene[5])};
-  float4 lo = lbox_lo[l], hi = lbox_hi[l];
+  float4 lo = lbox_lo[LBOX_STRIDE * l], hi = lbox_hi[LBOX_STRIDE * l];
   float c[3] = {0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
   uint32_t q[3];
 #pragma unroll
@@ -524,7 +529,7 @@ __global__ void __launch_bounds__(BLOCK) k_bottom_up_local(const int* __restrict
   const int l = b0 + threadIdx.x;
   if (l <= b1) {
     int src = sorted_leaf[l];
-    float4 lo = lbox_lo[src], hi = lbox_hi[src];
+    float4 lo = lbox_lo[LBOX_STRIDE * src], hi = lbox_hi[LBOX_STRIDE * src];
     blo[threadIdx.x][0] = lo.x; blo[threadIdx.x][1] = lo.y; blo[threadIdx.x][2] = lo.z;
     bhi[threadIdx.x][0] = hi.x; bhi[threadIdx.x][1] = hi.y; bhi[threadIdx.x][2] = hi.z;
     nodeA[n - 1 + l] = make_float4(lo.x, lo.y, lo.z, __int_as_float(escape_after(l, n, rfirst)));

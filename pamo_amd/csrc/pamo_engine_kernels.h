@@ -1059,9 +1059,10 @@ __global__ void __launch_bounds__(BLOCK) k_leaf_boxes_b(const int* __restrict__ 
   I3 t = ldt(T, f);
   V3 a = ldp(P, t.i), b = ldp(P, t.j), c = ldp(P, t.k);
   // .w of the low corner: the leaf's face (read by the node boxes)
-  lbox_lo[l] = make_float4(fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)), fminf(a.z, fminf(b.z, c.z)),
-                           __int_as_float(f));
-  lbox_hi[l] = make_float4(fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)), fmaxf(a.z, fmaxf(b.z, c.z)), 0.0f);
+  lbox_lo[LBOX_STRIDE * l] = make_float4(fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)),
+                                         fminf(a.z, fminf(b.z, c.z)), __int_as_float(f));
+  lbox_hi[LBOX_STRIDE * l] = make_float4(fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)),
+                                         fmaxf(a.z, fmaxf(b.z, c.z)), 0.0f);
 }
 // vertex bounds of one mesh at admission (ordered-int atomics, 6 slots)
 __global__ void __launch_bounds__(BLOCK) k_mesh_bounds(const float* __restrict__ P, long long n, int* box) {
@@ -1122,7 +1123,7 @@ __global__ void k_morton_b(const float4* __restrict__ lbox_lo, const float4* __r
                            int* __restrict__ vals, int bits) {
   int l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= *n_ptr) return;
-  keys[l] = mesh_morton_key(fmesh[leaf_face[l]], mp, lbox_lo[l], lbox_hi[l], bits);
+  keys[l] = mesh_morton_key(fmesh[leaf_face[l]], mp, lbox_lo[LBOX_STRIDE * l], lbox_hi[LBOX_STRIDE * l], bits);
   vals[l] = l;
 }
 // the face's box (the expressions of k_leaf_boxes_b)

@@ -467,7 +467,7 @@ pamo_ctx::~pamo_ctx() {
 // span blocks.
 bool pamo_ctx::node_boxes(const int* n_alive, const int* sorted_leaf, long long Fl) {
   if (!ensure(seg_lo, (size_t)seg_entries(Fl) * 16) || !ensure(seg_hi, (size_t)seg_entries(Fl) * 16)) return false;
-  k_bottom_up_local<<<nblk(Fl), BLOCK, 0, st>>>(n_alive, ptr<float4>(lbox_lo), ptr<float4>(lbox_hi), sorted_leaf,
+  k_bottom_up_local<<<nblk(Fl), BLOCK, 0, st>>>(n_alive, ptr<float4>(lbox_lo), ptr<float4>(lbox_lo) + 1, sorted_leaf,
                                                ptr<int>(kparent), ptr<int>(kleft),
                                                ptr<int>(kright), ptr<int>(rfirst), ptr<int>(rlast), ptr<float4>(nodeA),
                                                ptr<float4>(nodeB), Fl, ptr<float4>(seg_lo), ptr<float4>(seg_hi));
@@ -485,7 +485,7 @@ bool pamo_ctx::node_boxes(const int* n_alive, const int* sorted_leaf, long long 
 // nodes in nodeA/nodeB.
 bool pamo_ctx::build_bvh(int Fn) {
   const long long Fl = std::max(Fn, 1);
-  if (!ensure(lbox_lo, Fl * 16) || !ensure(lbox_hi, Fl * 16) || !ensure(keys0, Fl * 4) || !ensure(keys1, Fl * 4) ||
+  if (!ensure(lbox_lo, Fl * 32) || !ensure(lbox_hi, Fl * 16) || !ensure(keys0, Fl * 4) || !ensure(keys1, Fl * 4) ||
       !ensure(vals0, Fl * 4) || !ensure(vals1, Fl * 4) || !ensure(kleft, Fl * 4) || !ensure(kright, Fl * 4) ||
       !ensure(kparent, 2 * Fl * 4) || !ensure(rfirst, Fl * 4) || !ensure(rlast, Fl * 4) ||
       !ensure(nodeA, 2 * Fl * 16) || !ensure(nodeB, 2 * Fl * 16))
@@ -497,8 +497,8 @@ bool pamo_ctx::build_bvh(int Fn) {
                                        (2.0 * Fn - 1) * 24);
   k_scene_init<<<1, 64, 0, st>>>(scene);
   k_leaf_boxes<<<nblk(Fl), BLOCK, 0, st>>>(ptr<int>(leaf_face), n_alive, ptr<float>(P), ptr<int>(T),
-                                          ptr<float4>(lbox_lo), ptr<float4>(lbox_hi), scene);
-  k_morton<<<nblk(Fl), BLOCK, 0, st>>>(ptr<float4>(lbox_lo), ptr<float4>(lbox_hi), n_alive, scene,
+                                          ptr<float4>(lbox_lo), ptr<float4>(lbox_lo) + 1, scene);
+  k_morton<<<nblk(Fl), BLOCK, 0, st>>>(ptr<float4>(lbox_lo), ptr<float4>(lbox_lo) + 1, n_alive, scene,
                                       ptr<uint32_t>(keys0), ptr<int>(vals0));
   if (!check_launch("leaf boxes")) return false;
   // radix sort: 4 passes of 8 bits over 30-bit Morton codes
