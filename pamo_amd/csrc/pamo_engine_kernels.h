@@ -1411,12 +1411,16 @@ __global__ void __launch_bounds__(BLOCK) k_rs_hist64(const u64* __restrict__ key
   __syncthreads();
   hist[(long long)threadIdx.x * ntiles + blockIdx.x] = hh[threadIdx.x];
 }
+// LDS slot of the split's position i: one slot of padding per eight, so the
+// readback of eight consecutive 64-bit keys per lane (lanes 64 bytes apart,
+// two bank positions) spreads over the banks
+__device__ __forceinline__ int rs_skew(int i) { return i + (i >> 3); }
 __global__ void __launch_bounds__(BLOCK) k_rs_scatter64(const u64* __restrict__ kin, const int* __restrict__ vin,
                                                         const int* __restrict__ n_ptr, int shift, int ntiles,
                                                         const int* __restrict__ hoff, u64* __restrict__ kout,
                                                         int* __restrict__ vout) {
-  __shared__ u64 sk[RS_TILE];
-  __shared__ int sv[RS_TILE];
+  __shared__ u64 sk[RS_TILE + RS_TILE / 8];   // slot i at rs_skew(i)
+  __shared__ int sv[RS_TILE + RS_TILE / 8];
   __shared__ int lds[BLOCK / 64];
   __shared__ int dstart[256];
   __shared__ int goff[256];
@@ -1445,12 +1449,15 @@ __global__ void __launch_bounds__(BLOCK) k_rs_scatter64(const u64* __restrict__ 
     for (int i = 0; i < RS_ITEMS; i++) {
       int b = (int)((k[i] >> (shift + bit)) & 1ull);
       int dst = b ? (total_zeros + obefore + oc++) : (zbefore + zc++);
-      sk[dst] = k[i];
-      sv[dst] = v[i];
+      sk[rs_skew(dst)] = k[i];
+      sv[rs_skew(dst)] = v[i];
     }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < RS_ITEMS; i++) { k[i] = sk[threadIdx.x * RS_ITEMS + i]; v[i] = sv[threadIdx.x * RS_ITEMS + i]; }
+    for (int i = 0; i < RS_ITEMS; i++) {
+      k[i] = sk[rs_skew(threadIdx.x * RS_ITEMS + i)];
+      v[i] = sv[rs_skew(threadIdx.x * RS_ITEMS + i)];
+    }
     __syncthreads();
   }
   dstart[threadIdx.x] = RS_TILE;
@@ -1460,7 +1467,7 @@ __global__ void __launch_bounds__(BLOCK) k_rs_scatter64(const u64* __restrict__ 
     int li = threadIdx.x * RS_ITEMS + i;
     if (li < cnt) {
       int d = (int)((k[i] >> shift) & 255ull);
-      if (li == 0 || (int)((sk[li - 1] >> shift) & 255ull) != d) dstart[d] = li;
+      if (li == 0 || (int)((sk[rs_skew(li - 1)] >> shift) & 255ull) != d) dstart[d] = li;
     }
   }
   __syncthreads();
